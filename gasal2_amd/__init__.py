@@ -1,0 +1,510 @@
+"""gasal2_amd -- MI355X-native batched pairwise alignment with GASAL2's API.
+
+Python mirror of the reference's C/C++ interface (nahmedraja/GASAL2
+src/gasal.h, src/args_parser.h, src/gasal_align.h, src/ctors.h,
+src/host_batch.h, src/interfaces.h) over the C-ABI of ``libgasal2_amd.so``:
+the same function names, argument meanings, enum values and struct fields.
+The device work is hand-written gfx950 HIP in that library; nothing here
+computes alignments.  If the library is missing, every entry point raises.
+"""
+import ctypes as C
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgasal2_amd.so")
+SYNTH_PATH = os.path.join(_HERE, "libgasal2_synth.so")
+
+
+# ----------------------------------------------------------------- enums ---
+class comp_start(enum.IntEnum):          # src/gasal.h:37-41
+    WITHOUT_START = 0
+    WITH_START = 1
+    WITH_TB = 2
+
+
+class Bool(enum.IntEnum):                # src/gasal.h:45-48
+    FALSE = 0
+    TRUE = 1
+
+
+class data_source(enum.IntEnum):         # src/gasal.h:50-55
+    NONE = 0
+    QUERY = 1
+    TARGET = 2
+    BOTH = 3
+
+
+class algo_type(enum.IntEnum):           # src/gasal.h:57-65
+    UNKNOWN = 0
+    GLOBAL = 1
+    SEMI_GLOBAL = 2
+    LOCAL = 3
+    MICROLOCAL = 4
+    BANDED = 5
+    KSW = 6
+
+
+class operation_on_seq(enum.IntEnum):    # src/gasal.h:67-72
+    FORWARD_NATURAL = 0
+    REVERSE_NATURAL = 1
+    FORWARD_COMPLEMENT = 2
+    REVERSE_COMPLEMENT = 3
+
+
+globals().update({m.name: m for e in (comp_start, Bool, data_source, algo_type, operation_on_seq) for m in e})
+
+N_CODE_DEFAULT = 0x4E
+
+
+# --------------------------------------------------------------- structs ---
+class host_batch_t(C.Structure):          # src/gasal.h:75-83
+    pass
+
+
+host_batch_t._fields_ = [("data", C.POINTER(C.c_uint8)), ("page_size", C.c_uint32), ("data_size", C.c_uint32),
+                         ("offset", C.c_uint32), ("is_locked", C.c_int), ("next", C.POINTER(host_batch_t))]
+
+
+class gasal_res_t(C.Structure):           # src/gasal.h:86-95
+    _fields_ = [("aln_score", C.POINTER(C.c_int32)), ("query_batch_end", C.POINTER(C.c_int32)),
+                ("target_batch_end", C.POINTER(C.c_int32)), ("query_batch_start", C.POINTER(C.c_int32)),
+                ("target_batch_start", C.POINTER(C.c_int32)), ("cigar", C.POINTER(C.c_uint8)),
+                ("n_cigar_ops", C.POINTER(C.c_uint32))]
+
+
+_P8, _P32 = C.POINTER(C.c_uint8), C.POINTER(C.c_uint32)
+
+
+class gasal_gpu_storage_t(C.Structure):   # src/gasal.h:98-150 (+ impl)
+    _fields_ = [("unpacked_query_batch", _P8), ("unpacked_target_batch", _P8),
+                ("packed_query_batch", _P32), ("packed_target_batch", _P32),
+                ("query_batch_offsets", _P32), ("target_batch_offsets", _P32),
+                ("query_batch_lens", _P32), ("target_batch_lens", _P32),
+                ("host_seed_scores", _P32), ("seed_scores", _P32),
+                ("extensible_host_unpacked_query_batch", C.POINTER(host_batch_t)),
+                ("extensible_host_unpacked_target_batch", C.POINTER(host_batch_t)),
+                ("host_query_op", _P8), ("host_target_op", _P8), ("query_op", _P8), ("target_op", _P8),
+                ("host_query_batch_offsets", _P32), ("host_target_batch_offsets", _P32),
+                ("host_query_batch_lens", _P32), ("host_target_batch_lens", _P32),
+                ("host_res", C.POINTER(gasal_res_t)), ("device_cpy", C.POINTER(gasal_res_t)),
+                ("device_res", C.POINTER(gasal_res_t)), ("host_res_second", C.POINTER(gasal_res_t)),
+                ("device_res_second", C.POINTER(gasal_res_t)), ("device_cpy_second", C.POINTER(gasal_res_t)),
+                ("gpu_max_query_batch_bytes", C.c_uint32), ("gpu_max_target_batch_bytes", C.c_uint32),
+                ("host_max_query_batch_bytes", C.c_uint32), ("host_max_target_batch_bytes", C.c_uint32),
+                ("gpu_max_n_alns", C.c_uint32), ("host_max_n_alns", C.c_uint32), ("current_n_alns", C.c_uint32),
+                ("packed_tb_matrix_size", C.c_uint64), ("packed_tb_matrices", C.c_void_p),
+                ("str", C.c_void_p), ("is_free", C.c_int), ("id", C.c_int), ("impl", C.c_void_p)]
+
+
+class gasal_gpu_storage_v(C.Structure):   # src/gasal.h:153-156
+    _fields_ = [("n", C.c_int), ("a", C.POINTER(gasal_gpu_storage_t))]
+
+
+class gasal_subst_scores(C.Structure):    # src/gasal.h:160-165
+    _fields_ = [("match", C.c_int32), ("mismatch", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
+
+
+class gasal2_params_t(C.Structure):       # include/gasal2_ext.h
+    _fields_ = [(n, C.c_int32) for n in ("sa", "sb", "gapo", "gape", "start_pos", "print_out", "n_threads", "k_band",
+                                         "secondBest", "isPacked", "isReverseComplement",
+                                         "semiglobal_skipping_head", "semiglobal_skipping_tail", "algo")]
+
+
+# ------------------------------------------------------------ the library ---
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol include/*.h declares.
+_SIGS = {
+    "gasal_copy_subst_scores": (None, [C.POINTER(gasal_subst_scores)]),
+    "gasal_aln_async": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal_is_aln_async_done": (C.c_int, [C.POINTER(gasal_gpu_storage_t)]),
+    "gasal_init_gpu_storage_v": (gasal_gpu_storage_v, [C.c_int]),
+    "gasal_init_streams": (None, [C.POINTER(gasal_gpu_storage_v), C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "gasal_gpu_mem_alloc": (None, [C.POINTER(gasal_gpu_storage_t), C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "gasal_gpu_mem_free": (None, [C.POINTER(gasal_gpu_storage_t), C.c_void_p]),
+    "gasal_destroy_streams": (None, [C.POINTER(gasal_gpu_storage_v), C.c_void_p]),
+    "gasal_destroy_gpu_storage_v": (None, [C.POINTER(gasal_gpu_storage_v)]),
+    "gasal_host_batch_new": (C.POINTER(host_batch_t), [C.c_uint32, C.c_uint32]),
+    "gasal_host_batch_destroy": (None, [C.POINTER(host_batch_t)]),
+    "gasal_host_batch_getlast": (C.POINTER(host_batch_t), [C.POINTER(host_batch_t)]),
+    "gasal_host_batch_reset": (None, [C.POINTER(gasal_gpu_storage_t)]),
+    "gasal_host_batch_fill": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_char_p, C.c_uint32, C.c_int]),
+    "gasal_host_batch_add": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_char_p, C.c_uint32, C.c_int]),
+    "gasal_host_batch_addbase": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_char, C.c_int]),
+    "gasal_host_batch_print": (None, [C.POINTER(host_batch_t)]),
+    "gasal_host_batch_printall": (None, [C.POINTER(host_batch_t)]),
+    "gasal_host_alns_resize": (None, [C.POINTER(gasal_gpu_storage_t), C.c_int, C.c_void_p]),
+    "gasal_op_fill": (None, [C.POINTER(gasal_gpu_storage_t), C.POINTER(C.c_uint8), C.c_uint32, C.c_int]),
+    "gasal_set_device": (None, [C.c_int, C.c_bool]),
+    "gasal_res_new_host": (C.POINTER(gasal_res_t), [C.c_uint32, C.c_void_p]),
+    "gasal_res_new_device": (C.POINTER(gasal_res_t), [C.POINTER(gasal_res_t)]),
+    "gasal_res_new_device_cpy": (C.POINTER(gasal_res_t), [C.c_uint32, C.c_void_p]),
+    "gasal_res_destroy_host": (None, [C.POINTER(gasal_res_t)]),
+    "gasal_res_destroy_device": (None, [C.POINTER(gasal_res_t), C.POINTER(gasal_res_t)]),
+    "gasal2_fail": (None, [C.c_int]),
+    "gasal2_parameters_new": (C.c_void_p, []),
+    "gasal2_parameters_free": (None, [C.c_void_p]),
+    "gasal2_parameters_get": (None, [C.c_void_p, C.POINTER(gasal2_params_t)]),
+    "gasal2_parameters_set": (None, [C.c_void_p, C.POINTER(gasal2_params_t)]),
+    "gasal2_set_n_code": (None, [C.c_int32]),
+    "gasal2_get_n_code": (C.c_int32, []),
+    "gasal2_set_n_penalty": (None, [C.c_int32, C.c_int32]),
+    "gasal2_set_error_mode": (None, [C.c_int32]),
+    "gasal2_error_count": (C.c_int32, []),
+    "gasal2_last_error": (C.c_char_p, []),
+    "gasal2_upload": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal2_run_resident": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal2_download": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal2_enable_timing": (None, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
+    "gasal2_stage_ms": (C.c_float, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
+    "gasal2_get_cigar": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32]),
+    "gasal2_set_dispatch": (None, [C.c_int32]),
+}
+
+
+def lib():
+    """The loaded native library; raises NativeLibraryMissing if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(f"{LIB_PATH} not built: run `python -m gasal2_amd.build`")
+        L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+# ------------------------------------------------------------ Parameters ---
+class Parameters:
+    """class Parameters (src/args_parser.h:24-68) with the reference defaults
+    (src/args_parser.cpp:12-31).  Holds a native Parameters object; field
+    changes are pushed to it before every call that takes it."""
+
+    _FIELDS = [f for f, _ in gasal2_params_t._fields_]
+
+    def __init__(self, **kw):
+        self.sa, self.sb, self.gapo, self.gape = 1, 4, 6, 1
+        self.start_pos = comp_start.WITHOUT_START
+        self.print_out, self.n_threads, self.k_band = 0, 1, 0
+        self.secondBest = Bool.FALSE
+        self.isPacked = False
+        self.isReverseComplement = False
+        self.semiglobal_skipping_head = data_source.TARGET
+        self.semiglobal_skipping_tail = data_source.TARGET
+        self.algo = algo_type.UNKNOWN
+        for k, v in kw.items():
+            if k not in self._FIELDS:
+                raise AttributeError(k)
+            setattr(self, k, v)
+        self._native = None
+
+    def native(self):
+        L = lib()
+        if self._native is None:
+            self._native = L.gasal2_parameters_new()
+        pod = gasal2_params_t(**{f: int(getattr(self, f)) for f in self._FIELDS})
+        L.gasal2_parameters_set(self._native, C.byref(pod))
+        return self._native
+
+    def __del__(self):
+        if getattr(self, "_native", None) is not None and _lib is not None:
+            _lib.gasal2_parameters_free(self._native)
+            self._native = None
+
+
+# ----------------------------------------------------- reference-named API ---
+def gasal_copy_subst_scores(subst):
+    lib().gasal_copy_subst_scores(C.byref(subst))
+
+
+def gasal_init_gpu_storage_v(n_streams):
+    return lib().gasal_init_gpu_storage_v(n_streams)
+
+
+def gasal_init_streams(v, max_query_len, max_target_len, max_n_alns, params):
+    lib().gasal_init_streams(C.byref(v), max_query_len, max_target_len, max_n_alns, params.native())
+
+
+def gasal_destroy_streams(v, params):
+    lib().gasal_destroy_streams(C.byref(v), params.native())
+
+
+def gasal_destroy_gpu_storage_v(v):
+    lib().gasal_destroy_gpu_storage_v(C.byref(v))
+
+
+def gasal_host_batch_fill(storage, idx, data, src):
+    return lib().gasal_host_batch_fill(C.byref(storage), idx, bytes(data), len(data), int(src))
+
+
+def gasal_op_fill(storage, ops, src):
+    arr = (C.c_uint8 * len(ops))(*[int(o) for o in ops])
+    lib().gasal_op_fill(C.byref(storage), arr, len(ops), int(src))
+
+
+def gasal_host_alns_resize(storage, new_max, params):
+    lib().gasal_host_alns_resize(C.byref(storage), new_max, params.native())
+
+
+def gasal_aln_async(storage, q_bytes, t_bytes, n_alns, params):
+    lib().gasal_aln_async(C.byref(storage), q_bytes, t_bytes, n_alns, params.native())
+
+
+def gasal_is_aln_async_done(storage):
+    return lib().gasal_is_aln_async_done(C.byref(storage))
+
+
+def gasal_set_device(gpu_select=0, is_printing_prop=True):
+    lib().gasal_set_device(gpu_select, is_printing_prop)
+
+
+def set_n_code(n_code):
+    lib().gasal2_set_n_code(n_code)
+
+
+def set_n_penalty(penalty):
+    """penalty None == N_PENALTY undefined (src/gasal_kernels.h:47-55)."""
+    lib().gasal2_set_n_penalty(0 if penalty is None else 1, 0 if penalty is None else int(penalty))
+
+
+# ----------------------------------------------------------------- batches ---
+class Batch:
+    """Host-side pair batch laid out as a GASAL2 host batch: each sequence
+    padded with N_CODE to a multiple of 8 (src/host_batch.cpp:100-102)."""
+
+    def __init__(self, q, t, qoff, toff, qlen, tlen, qop=None, top=None, seeds=None):
+        n = len(qlen)
+        self.q = np.ascontiguousarray(q, dtype=np.uint8)
+        self.t = np.ascontiguousarray(t, dtype=np.uint8)
+        self.qoff = np.ascontiguousarray(qoff, dtype=np.uint32)
+        self.toff = np.ascontiguousarray(toff, dtype=np.uint32)
+        self.qlen = np.ascontiguousarray(qlen, dtype=np.uint32)
+        self.tlen = np.ascontiguousarray(tlen, dtype=np.uint32)
+        self.qop = np.zeros(n, np.uint8) if qop is None else np.ascontiguousarray(qop, dtype=np.uint8)
+        self.top = np.zeros(n, np.uint8) if top is None else np.ascontiguousarray(top, dtype=np.uint8)
+        self.seeds = np.zeros(n, np.uint32) if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint32)
+
+    @property
+    def n(self):
+        return len(self.qlen)
+
+    def subset(self, idx):
+        """A new batch of the selected pairs, re-laid-out contiguously."""
+        qs = [self.query(i) for i in idx]
+        ts = [self.target(i) for i in idx]
+        return make_batch(qs, ts, qop=self.qop[idx], top=self.top[idx], seeds=self.seeds[idx])
+
+    def query(self, i):
+        return bytes(self.q[self.qoff[i]:self.qoff[i] + self.qlen[i]])
+
+    def target(self, i):
+        return bytes(self.t[self.toff[i]:self.toff[i] + self.tlen[i]])
+
+
+def make_batch(queries, targets, qop=None, top=None, seeds=None, n_code=N_CODE_DEFAULT):
+    def lay(seqs):
+        offs, lens, chunks, o = [], [], [], 0
+        for s in seqs:
+            s = bytes(s)
+            pad = (-len(s)) % 8
+            offs.append(o)
+            lens.append(len(s))
+            chunks.append(s + bytes([n_code]) * pad)
+            o += len(s) + pad
+        buf = np.frombuffer(b"".join(chunks), dtype=np.uint8).copy() if chunks else np.zeros(0, np.uint8)
+        return buf, offs, lens
+
+    q, qoff, qlen = lay(queries)
+    t, toff, tlen = lay(targets)
+    return Batch(q, t, qoff, toff, qlen, tlen, qop, top, seeds)
+
+
+_synth = None
+
+
+def synth_batch(cfg, seed, n, n_code=N_CODE_DEFAULT):
+    """Seeded synthetic workload `cfg` (1..5, see csrc/synth.cpp)."""
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise NativeLibraryMissing(f"{SYNTH_PATH} not built: run `python -m gasal2_amd.build`")
+        S = C.CDLL(SYNTH_PATH)
+        S.synth_generate.restype = C.c_void_p
+        S.synth_generate.argtypes = [C.c_int, C.c_uint64, C.c_uint32]
+        S.synth_query_bytes.restype = C.c_uint64
+        S.synth_query_bytes.argtypes = [C.c_void_p]
+        S.synth_target_bytes.restype = C.c_uint64
+        S.synth_target_bytes.argtypes = [C.c_void_p]
+        S.synth_export.argtypes = [C.c_void_p, C.c_uint8] + [C.c_void_p] * 9
+        S.synth_free.argtypes = [C.c_void_p]
+        _synth = S
+    h = _synth.synth_generate(cfg, seed, n)
+    try:
+        qb, tb = _synth.synth_query_bytes(h), _synth.synth_target_bytes(h)
+        q, t = np.empty(qb, np.uint8), np.empty(tb, np.uint8)
+        qoff, toff, qlen, tlen, seeds = (np.empty(n, np.uint32) for _ in range(5))
+        qop, top = np.empty(n, np.uint8), np.empty(n, np.uint8)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        _synth.synth_export(h, n_code, ptr(q), ptr(t), ptr(qoff), ptr(toff), ptr(qlen), ptr(tlen), ptr(qop),
+                            ptr(top), ptr(seeds))
+    finally:
+        _synth.synth_free(h)
+    return Batch(q, t, qoff, toff, qlen, tlen, qop, top, seeds)
+
+
+# ------------------------------------------------------------ one-shot run ---
+class Results(dict):
+    pass
+
+
+def _arr(ptr, n, dtype):
+    if not ptr:
+        return None
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+class Aligner:
+    """Owns one storage (one HIP stream) sized for a batch and runs batches
+    through the reference call sequence: gasal_host_batch_fill per sequence (or
+    one bulk page copy), gasal_op_fill, gasal_aln_async, poll
+    gasal_is_aln_async_done (src/gasal_align.cu:29-326)."""
+
+    def __init__(self, params, max_query_len, max_target_len, max_n_alns, scores=(1, 4, 6, 1),
+                 n_code=N_CODE_DEFAULT, n_penalty=None):
+        L = lib()
+        self.params = params
+        set_n_code(n_code)
+        set_n_penalty(n_penalty)
+        gasal_copy_subst_scores(gasal_subst_scores(*scores))
+        self.v = gasal_init_gpu_storage_v(1)
+        gasal_init_streams(self.v, max_query_len + 7, max_target_len + 7, max_n_alns, params)
+        self.s = self.v.a[0]
+        self._L = L
+
+    def close(self):
+        if self.v is not None:
+            gasal_destroy_streams(self.v, self.params)
+            gasal_destroy_gpu_storage_v(self.v)
+            self.v = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _fill(self, batch, per_pair):
+        s = self.s
+        n = batch.n
+        if n > s.host_max_n_alns:
+            gasal_host_alns_resize(s, n, self.params)
+        if per_pair:
+            qi = ti = 0
+            for i in range(n):
+                s.host_query_batch_offsets[i] = qi
+                s.host_target_batch_offsets[i] = ti
+                qi = gasal_host_batch_fill(s, qi, batch.query(i), data_source.QUERY)
+                ti = gasal_host_batch_fill(s, ti, batch.target(i), data_source.TARGET)
+                s.host_query_batch_lens[i] = int(batch.qlen[i])
+                s.host_target_batch_lens[i] = int(batch.tlen[i])
+            return qi, ti
+        # bulk: the batch is already padded; one copy into the first page
+        for src, buf, offs, lens, dst_off, dst_len in (
+                (data_source.QUERY, batch.q, batch.qoff, batch.qlen, s.host_query_batch_offsets, s.host_query_batch_lens),
+                (data_source.TARGET, batch.t, batch.toff, batch.tlen, s.host_target_batch_offsets, s.host_target_batch_lens)):
+            page = (s.extensible_host_unpacked_query_batch if src == data_source.QUERY
+                    else s.extensible_host_unpacked_target_batch).contents
+            if page.page_size < len(buf):
+                raise ValueError("batch larger than the storage's first host page; use per_pair=True")
+            C.memmove(page.data, buf.ctypes.data, len(buf))
+            page.data_size = len(buf)
+            C.memmove(dst_off, offs.ctypes.data, 4 * n)
+            C.memmove(dst_len, lens.ctypes.data, 4 * n)
+        return len(batch.q), len(batch.t)
+
+    def run(self, batch, per_pair=None, timing=False):
+        s, p, L = self.s, self.params, self._L
+        n = batch.n
+        if per_pair is None:
+            per_pair = n <= 2000
+        qb, tb = self._fill(batch, per_pair)
+        s.current_n_alns = n
+        if p.isReverseComplement:
+            C.memmove(s.host_query_op, batch.qop.ctypes.data, n)
+            C.memmove(s.host_target_op, batch.top.ctypes.data, n)
+        if p.algo == algo_type.KSW:
+            C.memmove(s.host_seed_scores, batch.seeds.ctypes.data, 4 * n)
+        if timing:
+            L.gasal2_enable_timing(C.byref(s), 1)
+        gasal_aln_async(s, qb, tb, n, p)
+        import time
+        while True:
+            r = gasal_is_aln_async_done(s)
+            if r == 0:
+                break
+            if r == -2:
+                raise RuntimeError("nothing launched")
+            time.sleep(0.0005)
+        return self._collect(n, qb)
+
+    def _collect(self, n, qb):
+        s, p = self.s, self.params
+        h = s.host_res.contents
+        out = Results()
+        out["score"] = _arr(h.aln_score, n, np.int32)
+        out["qend"] = _arr(h.query_batch_end, n, np.int32)
+        out["tend"] = _arr(h.target_batch_end, n, np.int32)
+        out["qstart"] = _arr(h.query_batch_start, n, np.int32)
+        out["tstart"] = _arr(h.target_batch_start, n, np.int32)
+        if p.secondBest and s.host_res_second:
+            h2 = s.host_res_second.contents
+            out["s_score"] = _arr(h2.aln_score, n, np.int32)
+            out["s_qend"] = _arr(h2.query_batch_end, n, np.int32)
+            out["s_tend"] = _arr(h2.target_batch_end, n, np.int32)
+        if p.start_pos == comp_start.WITH_TB:
+            out["n_ops"] = _arr(h.n_cigar_ops, n, np.uint32)
+            out["cigar_qlayout"] = _arr(h.cigar, qb, np.uint8)
+        return out
+
+    def cigar(self, k, cap=1 << 20):
+        buf = (C.c_uint8 * cap)()
+        n = self._L.gasal2_get_cigar(C.byref(self.s), k, buf, cap)
+        return bytes(buf[:min(n, cap)])
+
+    def stage_ms(self, stage):
+        return self._L.gasal2_stage_ms(C.byref(self.s), stage)
+
+
+def cigar_to_string(cg):
+    """Forward CIGAR text with equal adjacent runs merged, as the reference
+    harness prints it (test_prog/test_prog.cpp:382-428)."""
+    if not cg:
+        return ""
+    ops = "MXDI"
+    out = []
+    last = cg[-1] & 3
+    count = cg[-1] >> 2
+    for b in reversed(cg[:-1]):
+        op = b & 3
+        if op == last:
+            count += b >> 2
+        else:
+            out.append(f"{count}{ops[last]}")
+            count = b >> 2
+        last = op
+    out.append(f"{count}{ops[last]}")
+    return "".join(out)

@@ -1,0 +1,78 @@
+// engine.h -- internal interface between the host runtime (runtime.cpp) and the
+// device pipeline (kernels.hip).  Not installed; the public API is include/.
+#ifndef GASAL2_ENGINE_H
+#define GASAL2_ENGINE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+// Scores as the kernels see them, passed by value (the reference keeps them in
+// __constant__ memory, src/gasal_kernels.h:29-33).
+struct G2Scores {
+    int32_t match, mismatch, gapo, gape, gapoe;
+    uint32_t n_value;   // N_CODE & 0xF (src/gasal_kernels.h:37)
+    int32_t n_code;     // full N_CODE byte (reverse kernel compares with it)
+    int32_t has_np;     // N_PENALTY defined
+    int32_t np;         // N_PENALTY
+};
+
+// Algorithm switches (Parameters fields, src/args_parser.h:30-48).
+struct G2Opts {
+    int32_t algo, start, second, head, tail, k_band, is_packed, is_rc;
+};
+
+// Device view of one batch plus the workspace it runs in.
+struct G2Dev {
+    // inputs
+    const uint8_t *uq, *ut;          // unpacked bytes
+    uint32_t *pq, *pt;               // packed words (alias uq/ut when is_packed)
+    const uint32_t *qoff, *toff, *qlen, *tlen;
+    const uint8_t *qop, *top;
+    const uint32_t *seeds;
+    uint32_t q_bytes, t_bytes, n;
+    uint32_t max_qw, max_tw;         // max words over the batch (host-computed)
+    // outputs (device)
+    int32_t *score, *qend, *tend, *qstart, *tstart;
+    int32_t *s_score, *s_qend, *s_tend;
+    uint32_t *n_ops;
+    uint8_t *cigar;                  // per-pair slots of cigar_cap bytes (pair order)
+    uint32_t cigar_cap;
+    uint8_t *cigar_qlayout;          // reference layout: CIGAR at query offsets
+    // workspace
+    uint32_t *keys, *keys_alt, *perm, *perm_alt;
+    void *sort_tmp;
+    size_t sort_tmp_bytes;
+    uint32_t *bnd;                   // boundary rows: per group bnd_cols*64 dwords
+    uint32_t bnd_cols;
+    uint32_t *tb;                    // direction words: per group tb_tw*tb_qw8*64
+    uint32_t tb_tw, tb_qw8;
+    int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
+    uint32_t eh_cols;
+    uint32_t *aux;                   // small device counters (work queue etc.)
+};
+
+struct G2Timing {
+    bool enabled;
+    hipEvent_t ev[6];   // start, after pack, after rc, after dp, after tb, end
+};
+
+// Bytes of device workspace needed for a batch (host side sizing).
+size_t g2_sort_tmp_bytes(uint32_t n);
+
+// Enqueue the whole device pipeline on `stream`.  Returns hipSuccess or the
+// first launch error.
+hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t stream, G2Timing *tm);
+
+// Dispatch policy for LOCAL (0 auto, 1 lane per pair, 2 wavefront per pair).
+void g2_set_dispatch(int policy);
+
+static inline uint32_t g2_regs(uint32_t len) { return (len >> 3) + ((len & 7) ? 1u : 0u); }
+
+// CIGAR slot size that always holds a pair's CIGAR (see get_tb walk bounds).
+static inline uint32_t g2_cigar_capacity(uint32_t max_qlen, uint32_t max_tlen) {
+    uint32_t q8 = 8 * g2_regs(max_qlen), t8 = 8 * g2_regs(max_tlen);
+    return q8 + t8 + (q8 + t8) / 32 + 16;
+}
+
+#endif

@@ -1,0 +1,928 @@
+// kernels.hip -- device pipeline of the batched alignment path for gfx950.
+//
+// Stage order follows gasal_aln_async (reference src/gasal_align.cu:190-283):
+//   pack -> reverse/complement -> sort pairs by shape -> DP -> traceback.
+//
+// DP kernels here give ONE LANE PER PAIR.  Pairs are sorted by (query words,
+// target words) so the 64 lanes of a wave walk the same loop bounds, and each
+// wave owns a contiguous slice of device scratch laid out [column][lane] so the
+// per-column boundary row (H,E as two int16 halves of one dword -- the
+// reference's short2, which also reproduces its int16 truncation between 8-row
+// strips) is read and written with fully coalesced 256-byte accesses.  The
+// direction nibbles of WITH_TB are stored the same way, [strip][column][lane].
+// Each lane walks its pair in the reference's order (8-row strips over the
+// target, columns left to right, rows top to bottom), so every order-dependent
+// output (end tie-breaks, second best, early exits) is the reference's.
+#include "engine.h"
+
+#include <string.h>
+#include <rocprim/rocprim.hpp>
+
+namespace {
+
+constexpr int32_t MINUS_INF = -32768;   // SHRT_MIN, src/gasal_kernels.h:35
+constexpr int WAVE = 64;
+
+enum { A_GLOBAL = 1, A_SEMI_GLOBAL = 2, A_LOCAL = 3, A_BANDED = 5, A_KSW = 6 };
+enum { S_WITHOUT_START = 0, S_WITH_START = 1, S_WITH_TB = 2 };
+enum { D_NONE = 0, D_QUERY = 1, D_TARGET = 2, D_BOTH = 3 };
+
+__device__ __forceinline__ int32_t sx16(uint32_t v) { return (int32_t)(int16_t)(uint16_t)(v & 0xFFFFu); }
+__device__ __forceinline__ uint32_t pk16(int32_t h, int32_t e) { return ((uint32_t)h & 0xFFFFu) | ((uint32_t)e << 16); }
+__device__ __forceinline__ int32_t t16(int32_t v) { return (int32_t)(int16_t)(uint16_t)((uint32_t)v & 0xFFFFu); }
+__device__ __forceinline__ int32_t dregs(uint32_t len) { return (int32_t)((len >> 3) + ((len & 7) ? 1 : 0)); }
+
+// DEV_GET_SUB_SCORE_LOCAL (src/gasal_kernels.h:40-42, :49-51)
+__device__ __forceinline__ int32_t sub_local(const G2Scores &c, int32_t nval, uint32_t r, uint32_t g) {
+    int32_t s = (r == g) ? c.match : -c.mismatch;
+    return (r == c.n_value || g == c.n_value) ? nval : s;
+}
+// DEV_GET_SUB_SCORE_GLOBAL (src/gasal_kernels.h:44-46, :53-54)
+__device__ __forceinline__ int32_t sub_global(const G2Scores &c, uint32_t r, uint32_t g) {
+    int32_t s = (r == g) ? c.match : -c.mismatch;
+    return (c.has_np && (r == c.n_value || g == c.n_value)) ? -c.np : s;
+}
+
+struct LaneCtx {
+    uint32_t grp, lane, slot, pair;
+    uint32_t qlen, tlen;
+    int32_t QW, TW;
+    const uint32_t *Q, *T;
+    uint32_t *bd;     // boundary row: element c at bd[c * WAVE]
+};
+
+__device__ __forceinline__ bool lane_setup(const G2Dev &d, LaneCtx &x) {
+    x.lane = threadIdx.x & (WAVE - 1);
+    x.grp = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    x.slot = x.grp * WAVE + x.lane;
+    if (x.slot >= d.n) return false;
+    x.pair = d.perm[x.slot];
+    x.qlen = d.qlen[x.pair];
+    x.tlen = d.tlen[x.pair];
+    x.QW = dregs(x.qlen);
+    x.TW = dregs(x.tlen);
+    x.Q = d.pq + (d.qoff[x.pair] >> 3);
+    x.T = d.pt + (d.toff[x.pair] >> 3);
+    x.bd = d.bnd + (size_t)x.grp * d.bnd_cols * WAVE + x.lane;
+    return true;
+}
+
+// ------------------------------------------------------------------ pack ----
+// gasal_pack_kernel (src/kernels/pack_rc_seqs.h:13-53): byte b -> nibble b&15,
+// 8 bases per word, base 0 in bits 31:28.  16 bytes in, 8 bytes out per thread.
+__device__ __forceinline__ uint32_t pack8(uint32_t lo, uint32_t hi) {
+    uint32_t r = ((lo & 15u) << 28) | (((lo >> 8) & 15u) << 24) | (((lo >> 16) & 15u) << 20) | (((lo >> 24) & 15u) << 16);
+    r |= ((hi & 15u) << 12) | (((hi >> 8) & 15u) << 8) | (((hi >> 16) & 15u) << 4) | ((hi >> 24) & 15u);
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_pack(const uint4 *__restrict__ in, uint2 *__restrict__ out, uint32_t n16,
+                                              const uint2 *__restrict__ in_tail, uint32_t *__restrict__ out_tail,
+                                              uint32_t has_tail) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) {
+        uint4 v = in[i];
+        uint2 o;
+        o.x = pack8(v.x, v.y);
+        o.y = pack8(v.z, v.w);
+        out[i] = o;
+    }
+    if (has_tail && blockIdx.x == 0 && threadIdx.x == 0) {
+        uint2 v = *in_tail;
+        *out_tail = pack8(v.x, v.y);
+    }
+}
+
+// -------------------------------------------------------- reverse / comp ----
+// gasal_reversecomplement_kernel (src/kernels/pack_rc_seqs.h:56-212).  Shift
+// amounts of 32 give 0, the result of PTX shl/shr the reference relies on.
+__device__ __forceinline__ uint32_t shl32(uint32_t x, uint32_t n) { return n >= 32 ? 0u : (x << n); }
+__device__ __forceinline__ uint32_t shr32(uint32_t x, uint32_t n) { return n >= 32 ? 0u : (x >> n); }
+__device__ __forceinline__ uint32_t nib_reverse(uint32_t x) {
+    x = ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
+    return __builtin_bswap32(x);
+}
+__device__ __forceinline__ uint32_t nib_complement(uint32_t w) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k += 4) {
+        uint32_t nt = (w >> k) & 15u;
+        nt = nt == 1u ? 4u : nt == 4u ? 1u : nt == 3u ? 7u : nt == 7u ? 3u : nt;
+        r |= nt << k;
+    }
+    return r;
+}
+
+__device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
+    uint32_t regs = (len >> 3) + ((len & 7) ? 1 : 0);
+    if (regs == 0) return;
+    uint32_t to_swap = (regs >> 1) + (regs & 1);
+    if (op & 1u) {
+        uint32_t nbr_N = 0;
+        uint32_t last = w[regs - 1];
+        for (int jj = 0; jj < 32; jj += 4) nbr_N += (((last & (0x0Fu << jj)) >> jj) == (uint32_t)n_code);
+        nbr_N = (nbr_N << 2) & 0xFFu;
+        for (uint32_t i = 0; i < to_swap; i++) {
+            uint32_t rpac_1 = w[i];
+            uint32_t wm2 = (regs >= 2 + i) ? w[regs - 2 - i] : 0u;   // word -1 is read but discarded
+            uint32_t wm1 = w[regs - 1 - i];
+            uint32_t rpac_2 = shl32(wm2, 32 - nbr_N) | shr32(wm1, nbr_N);
+            uint32_t rev1 = nib_reverse(rpac_1), rev2 = nib_reverse(rpac_2);
+            uint32_t lowmask = shl32(1u, nbr_N) - 1u;
+            uint32_t to_queue_1 = shl32(rev1, nbr_N) | (wm1 & lowmask);
+            uint32_t to_queue_2 = (wm2 & (0xFFFFFFFFu - lowmask)) | shr32(rev1, 32 - nbr_N);
+            w[i] = rev2;
+            w[regs - 1 - i] = to_queue_1;
+            if (i != to_swap - 1) w[regs - 2 - i] = to_queue_2;
+        }
+    }
+    if (op & 2u)
+        for (uint32_t i = 0; i < regs; i++) w[i] = nib_complement(w[i]);
+}
+
+__global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    uint32_t qo = d.qop[k], to = d.top[k];
+    if (qo == 0 && to == 0) return;
+    if (qo) rc_one(d.pq + (d.qoff[k] >> 3), d.qlen[k], qo, n_code);
+    if (to) rc_one(d.pt + (d.toff[k] >> 3), d.tlen[k], to, n_code);
+}
+
+// ------------------------------------------------------------------ sort ----
+__global__ __launch_bounds__(256) void k_keys(G2Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
+    qw = qw > 0xFFFFu ? 0xFFFFu : qw;
+    tw = tw > 0xFFFFu ? 0xFFFFu : tw;
+    d.keys[k] = (qw << 16) | tw;
+    d.perm[k] = k;
+}
+
+// ---------------------------------------------------------------- LOCAL -----
+// gasal_local_kernel<LOCAL,S,B> (src/kernels/local_kernel_template.h:71-519).
+template <int START, bool SECOND>
+__global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    constexpr bool TB = START == S_WITH_TB;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    uint32_t *__restrict__ bd = x.bd;
+    uint32_t *__restrict__ tbp = nullptr;
+    if (TB) tbp = d.tb + (size_t)x.grp * d.tb_tw * d.tb_qw8 * WAVE + x.lane;
+
+    for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;   // initHD (:118-120)
+
+    int32_t maxHH = 0, maxXY_y = 0, prev_maxHH = 0, maxXY_x = 0;
+    int32_t sec = 0, prev_sec = 0, secx = 0, secy = 0;
+    for (int32_t i = 0; i < TW; i++) {
+        int32_t h[9], f[9], p[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
+        const uint32_t gpac = x.T[i];
+        uint32_t gb[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        const int32_t gidx = i << 3;
+        uint32_t nxt[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
+        for (int32_t j = 0; j < QW; j++) {
+            const uint32_t rpac = x.Q[j];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
+            if (j + 1 < QW) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t ridx = j * 8 + kk;
+                const uint32_t rbase = (rpac >> (28 - 4 * kk)) & 15u;
+                h[0] = sx16(cur[kk]);
+                int32_t e = sx16(cur[kk] >> 16);
+                uint32_t dir = 0;
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                    const int32_t tmp = p[m] + s;
+                    int32_t hm = max(max(tmp, f[m]), max(e, 0));
+                    if (TB) {
+                        const int sh = 28 - ((m - 1) << 2);
+                        const uint32_t m_or_x = tmp >= p[m] ? 0u : 1u;                     // :49
+                        dir |= hm == tmp ? (m_or_x << sh) : (hm == f[m] ? (3u << sh) : (2u << sh)); // :53
+                        dir |= (tmp - gapoe) > (f[m] - gape) ? 0u : (1u << (sh + 3));      // :54
+                        dir |= (tmp - gapoe) > (e - gape) ? 0u : (1u << (sh + 2));         // :56
+                    }
+                    f[m] = max(tmp - gapoe, f[m] - gape);
+                    e = max(tmp - gapoe, e - gape);
+                    maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
+                    maxHH = max(maxHH, hm);
+                    if (SECOND) {                                                          // :145-150
+                        bool ov = (sec < hm) && (maxHH > hm);
+                        secy = ov ? gidx + (m - 1) : secy;
+                        sec = ov ? hm : sec;
+                    }
+                    p[m] = h[m - 1];
+                    h[m] = hm;
+                }
+                bd[ridx * WAVE] = pk16(h[8], e);
+                maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
+                if (SECOND) {                                                              // :160-164
+                    secx = (prev_sec < maxHH) ? ridx : secx;
+                    prev_sec = max(sec, prev_sec);
+                }
+                prev_maxHH = max(maxHH, prev_maxHH);
+                if (TB) tbp[((size_t)i * d.tb_qw8 + ridx) * WAVE] = dir;
+            }
+        }
+    }
+    const uint32_t pr = x.pair;
+    d.score[pr] = maxHH;
+    d.qend[pr] = maxXY_x;
+    d.tend[pr] = maxXY_y;
+    if (SECOND) { d.s_score[pr] = sec; d.s_qend[pr] = secx; d.s_tend[pr] = secy; }
+
+    if (START == S_WITH_START) {                                                           // :441-511
+        const int32_t fwd_score = maxHH;
+        const int32_t rend_reg = ((maxXY_x >> 3) + 1) < QW ? ((maxXY_x >> 3) + 1) : QW;
+        const int32_t gend_reg = ((maxXY_y >> 3) + 1) < TW ? ((maxXY_y >> 3) + 1) : TW;
+        const uint32_t *Qe = x.Q + (rend_reg - 1), *Te = x.T + (gend_reg - 1);
+        maxHH = 0; prev_maxHH = 0; maxXY_x = 0; maxXY_y = 0;
+        for (int32_t cc = 0; cc < 8 * rend_reg; cc++) bd[cc * WAVE] = 0u;
+        int32_t gidx = ((gend_reg << 3) + 8) - 1;
+        bool go = maxHH < fwd_score;
+        for (int32_t i = 0; i < gend_reg && go; i++) {
+            int32_t h[9], f[9], p[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
+            const uint32_t gpac = *(Te - i);
+            uint32_t gb[9];
+#pragma unroll
+            for (int m = 1; m < 9; m++) gb[m] = (gpac >> (4 * (m - 1))) & 15u;
+            gidx -= 8;
+            int32_t ridx = (rend_reg << 3) - 1;
+            int32_t gi = 0;
+            for (int32_t j = 0; j < rend_reg && go; j++) {
+                const uint32_t rpac = *(Qe - j);
+                for (int kk = 0; kk < 8 && go; kk++) {
+                    const uint32_t rbase = (rpac >> (4 * kk)) & 15u;
+                    uint32_t hd = bd[gi * WAVE];
+                    h[0] = sx16(hd);
+                    int32_t e = sx16(hd >> 16);
+#pragma unroll
+                    for (int m = 1; m < 9; m++) {                                          // CORE_LOCAL_COMPUTE_START
+                        const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                        const int32_t tmp = p[m] + s;
+                        int32_t hm = max(max(tmp, f[m]), max(e, 0));
+                        f[m] = max(tmp - gapoe, f[m] - gape);
+                        e = max(tmp - gapoe, e - gape);
+                        maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;                 // :41
+                        maxHH = max(maxHH, hm);
+                        p[m] = h[m - 1];
+                        h[m] = hm;
+                    }
+                    bd[gi * WAVE] = pk16(h[8], e);
+                    maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
+                    prev_maxHH = max(maxHH, prev_maxHH);
+                    ridx--; gi++;
+                    go = maxHH < fwd_score;
+                }
+            }
+        }
+        d.qstart[pr] = maxXY_x;
+        d.tstart[pr] = maxXY_y;
+    }
+}
+
+// --------------------------------------------------------------- GLOBAL -----
+// gasal_global_kernel<S> (src/kernels/global.h:30-303).
+template <bool TB>
+__global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    const int32_t gapo = c.gapo, gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    uint32_t *__restrict__ bd = x.bd;
+    uint32_t *__restrict__ tbp = nullptr;
+    if (TB) tbp = d.tb + (size_t)x.grp * d.tb_tw * d.tb_qw8 * WAVE + x.lane;
+    bd[0] = pk16(0, MINUS_INF);                                                            // :57-60
+    for (int32_t cc = 1; cc < 8 * QW; cc++) bd[cc * WAVE] = pk16(-(gapo + gape * cc), MINUS_INF);
+    const int32_t rowsel = 8 - ((TW << 3) - (int32_t)x.tlen);                              // :299
+    const int32_t lastcol = (int32_t)x.qlen - 1;
+    int32_t score = 0;
+    int32_t r = 1;
+    for (int32_t i = 0; i < TW; i++) {
+        int32_t h[9], f[9], p[9];
+        h[0] = 0; p[0] = 0; f[0] = 0;
+#pragma unroll
+        for (int m = 1; m < 9; m++) {                                                      // :67-71
+            h[m] = 0;
+            f[m] = MINUS_INF;
+            p[m] = (r + m - 1) == 1 ? 0 : -(gapo + gape * (r + m - 2));
+        }
+        r += 8;
+        const uint32_t gpac = x.T[i];
+        uint32_t gb[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        for (int32_t j = 0; j < QW; j++) {
+            const uint32_t rpac = x.Q[j];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = bd[(j * 8 + kk) * WAVE];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t ridx = j * 8 + kk;
+                const uint32_t rbase = (rpac >> (28 - 4 * kk)) & 15u;
+                h[0] = sx16(cur[kk]);
+                int32_t e = sx16(cur[kk] >> 16);
+                uint32_t dir = 0;
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t s = sub_global(c, rbase, gb[m]);
+                    const int32_t tmp = p[m] + s;
+                    int32_t hm = max(max(tmp, f[m]), e);
+                    if (TB) {
+                        const int sh = 28 - ((m - 1) << 2);
+                        const uint32_t m_or_x = tmp >= p[m] ? 0u : 1u;
+                        dir |= hm == tmp ? (m_or_x << sh) : (hm == f[m] ? (3u << sh) : (2u << sh));
+                        dir |= (tmp - gapoe) > (f[m] - gape) ? 0u : (1u << (sh + 3));
+                        dir |= (tmp - gapoe) > (e - gape) ? 0u : (1u << (sh + 2));
+                    }
+                    f[m] = max(tmp - gapoe, f[m] - gape);
+                    e = max(tmp - gapoe, e - gape);
+                    p[m] = h[m - 1];
+                    h[m] = hm;
+                }
+                bd[ridx * WAVE] = pk16(h[8], e);
+                if (TB) tbp[((size_t)i * d.tb_qw8 + ridx) * WAVE] = dir;
+                if (ridx == lastcol && i == TW - 1) {                                      // :98-103
+#pragma unroll
+                    for (int m = 1; m < 9; m++) score = (m == rowsel) ? h[m] : score;
+                }
+            }
+        }
+    }
+    d.score[x.pair] = score;
+}
+
+// ---------------------------------------------------------- SEMI_GLOBAL -----
+// gasal_semi_global_kernel<T,S,B,HEAD,TAIL> (src/kernels/semiglobal_kernel_template.h:39-388).
+template <int HEAD>
+__device__ __forceinline__ uint32_t semi_init_word(int32_t cc, const G2Scores &c) {
+    if (HEAD == D_QUERY || HEAD == D_BOTH) return 0u;                                      // :87-92
+    return cc == 0 ? pk16(0, MINUS_INF) : pk16(-(c.gapo + c.gape * cc), MINUS_INF);       // :93-99
+}
+template <int HEAD>
+__device__ __forceinline__ int32_t semi_init_h(int32_t cc, const G2Scores &c) {
+    if (HEAD == D_QUERY || HEAD == D_BOTH) return 0;
+    return cc == 0 ? 0 : t16(-(c.gapo + c.gape * cc));
+}
+template <int HEAD>
+__device__ __forceinline__ void semi_strip_init(const G2Scores &c, int32_t *h, int32_t *f, int32_t *p, int32_t &u) {
+    if (HEAD == D_TARGET || HEAD == D_BOTH) {                                              // :114-121
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = MINUS_INF; p[m] = 0; }
+    } else {                                                                               // :122-129 (r == u)
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            int32_t uu = u + m - 1;
+            h[m] = -(c.gapo + (c.gape * (uu - 1)));
+            f[m] = MINUS_INF;
+            p[m] = uu == 1 ? 0 : -(c.gapo + (c.gape * (uu - 1)));
+        }
+        u += 8;
+    }
+}
+__device__ __forceinline__ void semi_column(const G2Scores &c, int32_t nval, uint32_t rbase, const uint32_t *gb,
+                                            int32_t *h, int32_t *f, int32_t *p, uint32_t &hd) {
+    h[0] = sx16(hd);
+    int32_t e = sx16(hd >> 16);
+    int32_t prev_hm_diff = h[0] - c.gapoe;
+#pragma unroll
+    for (int m = 1; m < 9; m++) {                                                          // :17-28
+        const int32_t s = sub_local(c, nval, rbase, gb[m]);
+        int32_t cur = h[m] - c.gapoe;
+        f[m] = max(cur, f[m] - c.gape);
+        cur = p[m] + s;
+        cur = max(cur, f[m]);
+        e = max(prev_hm_diff, e - c.gape);
+        cur = max(cur, e);
+        h[m] = cur;
+        p[m] = prev_hm_diff + c.gapoe;
+        prev_hm_diff = cur - c.gapoe;
+    }
+    hd = pk16(h[8], e);
+}
+// Reversed copy word k of a sequence (:244-266): base 8k+b at bits 28-4b is the
+// original base len-1-(8k+b); positions past the start are nibble 0.
+__device__ __forceinline__ uint32_t rev_word(const uint32_t *W, int32_t len, int32_t k) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        int32_t ii = len - 1 - (8 * k + b);
+        uint32_t nib = ii >= 0 ? ((W[ii >> 3] >> (28 - 4 * (ii & 7))) & 15u) : 0u;
+        r |= nib << (28 - 4 * b);
+    }
+    return r;
+}
+
+template <int START, bool SECOND, int HEAD, int TAIL>
+__global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t QW = x.QW, TW = x.TW;
+    const int32_t read_len = (int32_t)x.qlen, ref_len = (int32_t)x.tlen;
+    uint32_t *__restrict__ bd = x.bd;
+    constexpr bool TT = (TAIL == D_TARGET || TAIL == D_BOTH);
+    constexpr bool TQ = (TAIL == D_QUERY || TAIL == D_BOTH);
+
+    int32_t maxHH = MINUS_INF, maxXY_x = ref_len, maxXY_y = read_len;                      // :49,63-64
+    int32_t sec = MINUS_INF, secx = ref_len, secy = read_len;                              // :71-74
+    for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = semi_init_word<HEAD>(cc, c);
+    int32_t h[9], f[9], p[9];
+    int32_t u = 1;
+    h[0] = 0; p[0] = 0; f[0] = 0;
+    for (int32_t i = 0; i < TW; i++) {
+        const int32_t gidx = i << 3;
+        semi_strip_init<HEAD>(c, h, f, p, u);
+        const uint32_t gpac = x.T[i];
+        uint32_t gb[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        for (int32_t j = 0; j < QW; j++) {
+            const uint32_t rpac = x.Q[j];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t ridx = j * 8 + kk;
+                uint32_t hd = bd[ridx * WAVE];
+                semi_column(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
+                bd[ridx * WAVE] = hd;
+                if (TT && ridx + 1 == read_len) {                                          // :160-178
+#pragma unroll
+                    for (int m = 1; m < 9; m++) {
+                        bool in = (gidx + m - 1) < ref_len;
+                        bool upd = h[m] > maxHH && in;
+                        maxXY_y = upd ? gidx + (m - 1) : maxXY_y;
+                        maxHH = upd ? h[m] : maxHH;
+                        if (SECOND) {
+                            bool ov = h[m] > sec && h[m] < maxHH && in;
+                            secy = ov ? gidx + (m - 1) : secy;
+                            sec = ov ? h[m] : sec;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (TQ) {                                                                              // :185-214
+        const int32_t lim = max(max(8 * QW, read_len), ref_len);
+        for (int32_t m = 0; m < lim; m++) {
+            int32_t st = m < 8 * QW ? sx16(bd[m * WAVE]) : semi_init_h<HEAD>(m, c);
+            if (st > maxHH && m < read_len) { maxXY_x = m; maxHH = st; }
+            if (SECOND) {
+                bool ov = st > sec && st < maxHH && m < ref_len;
+                secx = ov ? m : secx;
+                sec = ov ? st : sec;
+            }
+        }
+        if (maxXY_x != ref_len) maxXY_y = read_len;
+        if (SECOND && secx != ref_len) secy = read_len;
+    }
+    const uint32_t pr = x.pair;
+    d.score[pr] = maxHH;
+    d.tend[pr] = maxXY_y;
+    d.qend[pr] = maxXY_x;
+    if (SECOND) { d.s_score[pr] = sec; d.s_tend[pr] = secy; d.s_qend[pr] = secx; }
+
+    if (START == S_WITH_START) {                                                           // :227-384
+        const int32_t fwd_score = maxHH;
+        const int32_t dd = TW - ((maxXY_y >> 3) + 1);
+        const int32_t gend_reg = dd > 0 ? dd - 1 : dd;                                      // :273
+        maxHH = MINUS_INF;
+        maxXY_y = 0;
+        for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = semi_init_word<HEAD>(cc, c);
+        u = 1;
+        h[0] = 0; p[0] = 0;
+        for (int32_t i = gend_reg; i < TW && maxHH < fwd_score; i++) {
+            const int32_t gidx = i << 3;
+            semi_strip_init<HEAD>(c, h, f, p, u);
+            const uint32_t gpac = i >= 0 ? rev_word(x.T, ref_len, i) : 0u;
+            uint32_t gb[9];
+#pragma unroll
+            for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            for (int32_t j = 0; j < QW && maxHH < fwd_score; j++) {
+                const uint32_t rpac = rev_word(x.Q, read_len, j);
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) {
+                    const int32_t ridx = j * 8 + kk;
+                    uint32_t hd = bd[ridx * WAVE];
+                    semi_column(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
+                    bd[ridx * WAVE] = hd;
+                    if (TT && ridx + 1 == read_len) {
+#pragma unroll
+                        for (int m = 1; m < 9; m++) {
+                            bool upd = h[m] > maxHH && (gidx + m - 1) < ref_len;
+                            maxXY_y = upd ? gidx + (m - 1) : maxXY_y;
+                            maxHH = upd ? h[m] : maxHH;
+                        }
+                    }
+                }
+            }
+        }
+        if (TQ) {                                                                          // :362-378
+            const int32_t lim = max(8 * QW, read_len);
+            for (int32_t m = 0; m < lim; m++) {
+                int32_t st = m < 8 * QW ? sx16(bd[m * WAVE]) : semi_init_h<HEAD>(m, c);
+                if (st > maxHH && m < read_len) { maxXY_x = m; maxHH = st; }
+            }
+            if (maxXY_x != ref_len) maxXY_y = read_len;
+        }
+        d.tstart[pr] = (ref_len - 1) - maxXY_y;                                           // :380-381
+        d.qstart[pr] = (read_len - 1) - maxXY_x;
+    }
+}
+
+// --------------------------------------------------------------- BANDED -----
+// gasal_banded_tiled_kernel (src/kernels/banded.h:10-139).
+__global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_band_width) {
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    uint32_t *__restrict__ bd = x.bd;
+    for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;
+    int32_t maxHH = 0, prev_maxHH = 0, maxXY_x = 0, maxXY_y = 0;
+    const int32_t other = TW - (QW - k_band_width);                                        // :35
+    for (int32_t i = 0; i < TW; i++) {
+        int32_t h[9], f[9], p[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
+        const uint32_t gpac = x.T[i];
+        uint32_t gb[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        const int32_t gidx = i << 3;
+        const int32_t j0 = max(0, i - other + 1);                                          // :83-85
+        const int32_t last_tile = min(k_band_width + i, QW);
+        for (int32_t j = j0; j < last_tile; j++) {
+            const uint32_t rpac = x.Q[j];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t ridx = j * 8 + kk;
+                const uint32_t rbase = (rpac >> (28 - 4 * kk)) & 15u;
+                uint32_t hd = bd[ridx * WAVE];
+                h[0] = sx16(hd);
+                int32_t e = sx16(hd >> 16);
+#pragma unroll
+                for (int m = 1; m < 9; m++) {                                              // :100-115
+                    const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                    f[m] = max(h[m] - gapoe, f[m] - gape);
+                    int32_t hm = p[m] + s;
+                    hm = max(hm, f[m]);
+                    hm = max(hm, 0);
+                    e = max(h[m - 1] - gapoe, e - gape);
+                    hm = max(hm, e);
+                    maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
+                    maxHH = max(maxHH, hm);
+                    p[m] = h[m - 1];
+                    h[m] = hm;
+                }
+                bd[ridx * WAVE] = pk16(h[8], e);
+                maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
+                prev_maxHH = max(maxHH, prev_maxHH);
+            }
+        }
+    }
+    d.score[x.pair] = maxHH;
+    d.qend[x.pair] = maxXY_x;
+    d.tend[x.pair] = maxXY_y;
+}
+
+// ------------------------------------------------------------------ KSW -----
+// gasal_ksw_kernel<B> (src/kernels/ksw_kernel_template.h:46-199).  The per-pair
+// eh[] row lives in device scratch laid out [column][lane] (int2 per entry).
+__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const uint32_t qlen = x.qlen, tlen = x.tlen;
+    const uint32_t query_batch_regs = (qlen >> 3) + 1, target_batch_regs = (tlen >> 3) + 1;
+    const uint32_t h0 = d.seeds[x.pair];
+    const int o_del = c.gapo, o_ins = c.gapo, e_del = c.gape, e_ins = c.gape;
+    const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
+    int2 *__restrict__ eh = reinterpret_cast<int2 *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
+    for (uint32_t jj = 0; jj <= qlen + 1; jj++) eh[(size_t)jj * WAVE] = make_int2(0, 0);
+    int j;
+    eh[0].x = (int32_t)h0;                                                                 // :78-81
+    eh[WAVE].x = h0 > (uint32_t)oe_ins ? (int32_t)(h0 - (uint32_t)oe_ins) : 0;
+    for (j = 2; j <= (int)qlen && eh[(size_t)(j - 1) * WAVE].x > e_ins; ++j)
+        eh[(size_t)j * WAVE].x = eh[(size_t)(j - 1) * WAVE].x - e_ins;
+    int max = (int32_t)h0, max_i = -1, max_j = -1, max_ie = -1, gscore = -1;
+    int beg = 0, end = (int)qlen;
+    for (uint32_t tt = 0; tt < target_batch_regs; tt++) {
+        const uint32_t gpac = x.T[tt];
+        for (uint32_t tb = 0; tb < 8; tb++) {
+            const int i = (int)(tt * 8 + tb);
+            if (i >= (int)tlen) break;                                                     // :98-99
+            const uint32_t gbase = (gpac >> (32 - (tb + 1) * 4)) & 0x0Fu;
+            int t, f = 0, h1, m = 0, mj = -1;
+            if (beg == 0) {
+                h1 = (int32_t)(h0 - (uint32_t)(o_del + e_del * (i + 1)));
+                if (h1 < 0) h1 = 0;
+            } else
+                h1 = 0;
+            // j after the tile loops (only the last tile's values matter: :113-152)
+            const int T = (int)(qlen >> 3);
+            for (int jj = beg; jj < end; jj++) {
+                const uint32_t rpac = x.Q[jj >> 3];
+                const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
+                int2 pv = eh[(size_t)jj * WAVE];
+                int h, M = pv.x, e = pv.y;
+                pv.x = h1;
+                const int32_t s = sub_local(c, nval, rbase, gbase);
+                M = M ? M + s : 0;
+                h = M > e ? M : e;
+                h = h > f ? h : f;
+                h1 = h;
+                mj = m > h ? mj : jj;
+                m = m > h ? m : h;
+                t = M - oe_del; t = t > 0 ? t : 0;
+                e -= e_del; e = e > t ? e : t;
+                pv.y = e;
+                eh[(size_t)jj * WAVE] = pv;
+                t = M - oe_ins; t = t > 0 ? t : 0;
+                f -= e_ins; f = f > t ? f : t;
+            }
+            {
+                int be = beg > end ? beg : end;
+                j = be < 8 * T ? 8 * T : be;
+                j = j > 8 * T + 7 ? 8 * T + 7 : j;
+            }
+            eh[(size_t)end * WAVE] = make_int2(h1, 0);                                     // :153-154
+            if (j == (int)qlen) {
+                max_ie = gscore > h1 ? max_ie : i;
+                gscore = gscore > h1 ? gscore : h1;
+            }
+            if (m == 0) break;                                                             // :159-160
+            if (m > max) { max = m; max_i = i; max_j = mj; }
+            for (j = beg; (j < end) && eh[(size_t)j * WAVE].x == 0 && eh[(size_t)j * WAVE].y == 0; ++j) ;
+            beg = j;
+            for (j = end; (j >= beg) && eh[(size_t)j * WAVE].x == 0 && eh[(size_t)j * WAVE].y == 0; --j) ;
+            end = j + 2 < (int)qlen ? j + 2 : (int)qlen;
+        }
+    }
+    const uint32_t pr = x.pair;
+    if (gscore <= 0 || gscore <= max - 5) {                                                // :188-197
+        d.score[pr] = max; d.qend[pr] = max_j + 1; d.tend[pr] = max_i + 1;
+    } else {
+        d.score[pr] = gscore; d.qend[pr] = (int32_t)qlen; d.tend[pr] = max_ie + 1;
+    }
+}
+
+// --------------------------------------------------------------- get_tb -----
+// gasal_get_tb<T> (src/kernels/get_tb.h:4-149).  Reads the direction words the
+// DP kernel wrote for this pair's wave slot; cells outside the pair's computed
+// matrix read as 0.  Writes the pair's CIGAR into its own slot.
+template <bool LOCALT>
+__global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t grp = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    const uint32_t slot = grp * WAVE + lane;
+    if (slot >= d.n) return;
+    const uint32_t pr = d.perm[slot];
+    const uint32_t read_len = d.qlen[pr], ref_len = d.tlen[pr];
+    const int32_t QW = dregs(read_len), TW = dregs(ref_len);
+    const uint32_t *__restrict__ tbp = d.tb + (size_t)grp * d.tb_tw * d.tb_qw8 * WAVE + lane;
+    const int64_t ncells = (int64_t)TW * 8 * QW * 8;
+    uint8_t *__restrict__ cg = d.cigar + (size_t)pr * d.cigar_cap;
+    int i, j, total_score = 0, curr_score = 0;
+    if (LOCALT) { i = d.tend[pr]; j = d.qend[pr]; total_score = d.score[pr]; }
+    else { i = (int)ref_len; j = (int)read_len; }
+    uint32_t prev_op_to_fill = 0;
+    const int read_len_8 = 8 * QW;
+    int n_ops = 0, op_select = 3, op_shift = 0, count = 0, offset = 0;
+    uint32_t op_to_fill;
+    int64_t prev_word = -1;
+    uint32_t word = 0;
+    while (i >= 0 && j >= 0) {
+        const int64_t cell = ((((int64_t)(i >> 3) * read_len_8) << 3) + ((int64_t)j << 3) + (i & 7));
+        uint32_t cell_op = 0;
+        if (cell < ncells) {
+            const int64_t wi = cell >> 3;   // word = strip * 8QW + column
+            if (wi != prev_word) {
+                const int64_t strip = wi / read_len_8, col = wi - strip * read_len_8;
+                word = tbp[((size_t)strip * d.tb_qw8 + (size_t)col) * WAVE];
+                prev_word = wi;
+            }
+            cell_op = (word >> (28 - 4 * (cell & 7))) & 15u;
+        }
+        const uint32_t op = (cell_op >> op_shift) & (uint32_t)op_select;
+        op_to_fill = op == 0 || op_select == 3 ? op : (uint32_t)op_shift;
+        op_select = op == 0 || (op == 1 && op_select == 3) ? 3 : 1;
+        op_shift = op == 0 || (op == 1 && op_select == 3) ? 0 : ((op == 2 || op == 3) ? (int)op : op_shift);
+        if (count < 63 && op_to_fill == prev_op_to_fill) {
+            count++;
+        } else {
+            if (count > 0) { cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2)); n_ops++; }
+            count = 1;
+        }
+        if (LOCALT) {
+            curr_score += ((op_to_fill == 2 || op_to_fill == 3) && prev_op_to_fill != op_to_fill) ? -c.gapoe
+                        : ((op_to_fill == 2 || op_to_fill == 3) ? -c.gape : (op_to_fill == 1 ? -c.mismatch : c.match));
+            if (curr_score == total_score) break;
+        }
+        prev_op_to_fill = op_to_fill;
+        i = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 2 ? i - 1 : i;
+        j = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 3 ? j - 1 : j;
+    }
+    cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2));
+    n_ops++;
+    if (!LOCALT) {
+        while (i >= 0) { uint32_t rc = (i + 1) <= 63 ? (uint32_t)(i + 1) : 63u; cg[offset++] = (uint8_t)(2u | (rc << 2)); n_ops++; i -= 63; }
+        while (j >= 0) { uint32_t rc = (j + 1) <= 63 ? (uint32_t)(j + 1) : 63u; cg[offset++] = (uint8_t)(3u | (rc << 2)); n_ops++; j -= 63; }
+    }
+    if (LOCALT) { d.tstart[pr] = i; d.qstart[pr] = j; }
+    d.n_ops[pr] = (uint32_t)n_ops;
+}
+
+// Copies each pair's CIGAR to the reference's layout (bytes at the pair's query
+// offset, src/gasal_align.h:52), clipped to its padded query length.
+__global__ __launch_bounds__(256) void k_cigar_qlayout(G2Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    uint32_t nops = d.n_ops[k];
+    uint32_t slot = 8u * (uint32_t)dregs(d.qlen[k]);
+    uint32_t nb = nops < slot ? nops : slot;
+    const uint8_t *src = d.cigar + (size_t)k * d.cigar_cap;
+    uint8_t *dst = d.cigar_qlayout + d.qoff[k];
+    for (uint32_t b = 0; b < nb; b++) dst[b] = src[b];
+}
+
+// WITH_TB for algorithms without a traceback kernel: the reference copies the
+// query bytes as "CIGAR" and the query lengths as n_cigar_ops
+// (src/gasal_align.cu:280-283).
+__global__ __launch_bounds__(256) void k_no_tb(G2Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    uint32_t ql = d.qlen[k];
+    d.n_ops[k] = ql;
+    uint8_t *dst = d.cigar + (size_t)k * d.cigar_cap;
+    uint64_t off = d.qoff[k];
+    for (uint32_t b = 0; b < ql && b < d.cigar_cap; b++) dst[b] = (off + b < d.q_bytes) ? d.uq[off + b] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, uint32_t n) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+int g_dispatch = 0;
+
+}  // namespace
+
+void g2_set_dispatch(int policy) { g_dispatch = policy; }
+
+size_t g2_sort_tmp_bytes(uint32_t n) {
+    size_t bytes = 0;
+    rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+    (void)rocprim::radix_sort_pairs_desc((void *)nullptr, bytes, kb, vb, n, 0, 32, (hipStream_t)0);
+    return bytes + 256;
+}
+
+#define G2_LAUNCH_CHECK()                        \
+    do {                                         \
+        hipError_t e_ = hipGetLastError();       \
+        if (e_ != hipSuccess) return e_;         \
+    } while (0)
+
+template <int START, bool SECOND>
+static void launch_semi_ht(int head, int tail, dim3 g, dim3 b, hipStream_t s, const G2Dev &d, const G2Scores &c) {
+#define G2_SEMI_T(H)                                                                                         \
+    switch (tail) {                                                                                          \
+        case D_NONE: k_semi<START, SECOND, H, D_NONE><<<g, b, 0, s>>>(d, c); break;                          \
+        case D_QUERY: k_semi<START, SECOND, H, D_QUERY><<<g, b, 0, s>>>(d, c); break;                        \
+        case D_TARGET: k_semi<START, SECOND, H, D_TARGET><<<g, b, 0, s>>>(d, c); break;                      \
+        default: k_semi<START, SECOND, H, D_BOTH><<<g, b, 0, s>>>(d, c); break;                              \
+    }
+    switch (head) {
+        case D_NONE: G2_SEMI_T(D_NONE); break;
+        case D_QUERY: G2_SEMI_T(D_QUERY); break;
+        case D_TARGET: G2_SEMI_T(D_TARGET); break;
+        default: G2_SEMI_T(D_BOTH); break;
+    }
+#undef G2_SEMI_T
+}
+
+hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t s, G2Timing *tm) {
+    const bool timing = tm && tm->enabled;
+    if (timing) (void)hipEventRecord(tm->ev[0], s);
+    // 1. pack (src/gasal_align.cu:190-201)
+    if (!op.is_packed) {
+        auto packside = [&](const uint8_t *u, uint32_t *p, uint32_t bytes) {
+            uint32_t n16 = bytes / 16;
+            uint32_t tail = (bytes % 16) ? 1u : 0u;
+            uint32_t blocks = (n16 + 255) / 256;
+            if (blocks > 8192) blocks = 8192;
+            if (blocks == 0) blocks = 1;
+            k_pack<<<blocks, 256, 0, s>>>(reinterpret_cast<const uint4 *>(u), reinterpret_cast<uint2 *>(p), n16,
+                                          reinterpret_cast<const uint2 *>(u + (size_t)n16 * 16), p + (size_t)n16 * 2, tail);
+        };
+        packside(d.uq, d.pq, d.q_bytes);
+        packside(d.ut, d.pt, d.t_bytes);
+        G2_LAUNCH_CHECK();
+    }
+    if (timing) (void)hipEventRecord(tm->ev[1], s);
+    const uint32_t nb256 = (d.n + 255) / 256;
+    // 2. reverse / complement (src/gasal_align.cu:232-246)
+    if (op.is_rc) {
+        k_rc<<<nb256, 256, 0, s>>>(d, sc.n_code);
+        G2_LAUNCH_CHECK();
+    }
+    if (timing) (void)hipEventRecord(tm->ev[2], s);
+    // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds
+    k_keys<<<nb256, 256, 0, s>>>(d);
+    G2_LAUNCH_CHECK();
+    {
+        rocprim::double_buffer<uint32_t> kb(d.keys, d.keys_alt), vb(d.perm, d.perm_alt);
+        size_t bytes = d.sort_tmp_bytes;
+        hipError_t e = rocprim::radix_sort_pairs_desc(d.sort_tmp, bytes, kb, vb, d.n, 0, 32, s);
+        if (e != hipSuccess) return e;
+        if (vb.current() != d.perm) {
+            hipError_t e2 = hipMemcpyAsync(d.perm, vb.current(), (size_t)d.n * 4, hipMemcpyDeviceToDevice, s);
+            if (e2 != hipSuccess) return e2;
+        }
+    }
+    // 4. DP (dispatch table src/gasal_align.h:7-107)
+    const uint32_t groups = (d.n + WAVE - 1) / WAVE;
+    const dim3 grid(groups), blk(WAVE);
+    const bool tb = op.start == S_WITH_TB;
+    const bool second = op.second != 0;
+    switch (op.algo) {
+        case A_LOCAL:
+            if (op.start == S_WITH_TB) {
+                if (second) k_local<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
+                else k_local<S_WITH_TB, false><<<grid, blk, 0, s>>>(d, sc);
+            } else if (op.start == S_WITH_START) {
+                if (second) k_local<S_WITH_START, true><<<grid, blk, 0, s>>>(d, sc);
+                else k_local<S_WITH_START, false><<<grid, blk, 0, s>>>(d, sc);
+            } else {
+                if (second) k_local<S_WITHOUT_START, true><<<grid, blk, 0, s>>>(d, sc);
+                else k_local<S_WITHOUT_START, false><<<grid, blk, 0, s>>>(d, sc);
+            }
+            break;
+        case A_GLOBAL:
+            if (tb) k_global<true><<<grid, blk, 0, s>>>(d, sc);
+            else k_global<false><<<grid, blk, 0, s>>>(d, sc);
+            break;
+        case A_SEMI_GLOBAL:
+            if (op.start == S_WITH_START) {
+                if (second) launch_semi_ht<S_WITH_START, true>(op.head, op.tail, grid, blk, s, d, sc);
+                else launch_semi_ht<S_WITH_START, false>(op.head, op.tail, grid, blk, s, d, sc);
+            } else {
+                if (second) launch_semi_ht<S_WITHOUT_START, true>(op.head, op.tail, grid, blk, s, d, sc);
+                else launch_semi_ht<S_WITHOUT_START, false>(op.head, op.tail, grid, blk, s, d, sc);
+            }
+            break;
+        case A_BANDED:
+            k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
+            break;
+        case A_KSW:
+            k_ksw<<<grid, blk, 0, s>>>(d, sc);
+            break;
+        default:
+            break;  // UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21)
+    }
+    G2_LAUNCH_CHECK();
+    // second-best arrays of algorithms that never write them are defined as 0
+    if (second && !(op.algo == A_LOCAL || op.algo == A_SEMI_GLOBAL)) {
+        k_zero_i32<<<nb256, 256, 0, s>>>(d.s_score, d.n);
+        k_zero_i32<<<nb256, 256, 0, s>>>(d.s_qend, d.n);
+        k_zero_i32<<<nb256, 256, 0, s>>>(d.s_tend, d.n);
+        G2_LAUNCH_CHECK();
+    }
+    if (timing) (void)hipEventRecord(tm->ev[3], s);
+    // 5. traceback (src/gasal_align.h:45-53,60-68)
+    if (tb) {
+        if (op.algo == A_LOCAL) k_get_tb<true><<<grid, blk, 0, s>>>(d, sc);
+        else if (op.algo == A_GLOBAL) k_get_tb<false><<<grid, blk, 0, s>>>(d, sc);
+        else k_no_tb<<<nb256, 256, 0, s>>>(d);
+        G2_LAUNCH_CHECK();
+        if (d.cigar_qlayout) {
+            k_cigar_qlayout<<<nb256, 256, 0, s>>>(d);
+            G2_LAUNCH_CHECK();
+        }
+    }
+    if (timing) {
+        (void)hipEventRecord(tm->ev[4], s);
+        (void)hipEventRecord(tm->ev[5], s);
+    }
+    return hipSuccess;
+}

@@ -1,0 +1,1060 @@
+// runtime.cpp -- host side of the drop-in API (include/*.h).
+//
+// Behaviour follows the reference's host runtime: storage construction
+// (src/ctors.cpp), paged pinned host batches (src/host_batch.cpp), result
+// buffers (src/res.cpp), resizing/op fill/device selection (src/interfaces.cpp),
+// Parameters (src/args_parser.cpp) and the asynchronous launcher
+// (src/gasal_align.cu).  The device work itself is g2_run() (kernels.hip).
+#include "gasal_header.h"
+#include "res.h"
+#include "engine.h"
+
+#include <string.h>
+#include <stdarg.h>
+#include <math.h>
+#include <string>
+#include <vector>
+#include <mutex>
+#include <algorithm>
+
+// ------------------------------------------------------------ globals ------
+namespace {
+
+G2Scores g_scores = {0, 0, 0, 0, 0, (uint32_t)(N_CODE & 0xF), N_CODE,
+#ifdef N_PENALTY
+                     1, N_PENALTY
+#else
+                     0, 0
+#endif
+};
+
+int g_error_mode = 0;
+int g_error_count = 0;
+std::string g_last_error;
+std::mutex g_err_mu;
+
+struct Impl {
+    uint32_t *keys = nullptr, *keys_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
+    size_t n_cap = 0;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    uint32_t *bnd = nullptr;
+    size_t bnd_bytes = 0;
+    uint32_t *tb = nullptr;
+    size_t tb_bytes = 0;
+    int32_t *eh = nullptr;
+    size_t eh_bytes = 0;
+    uint32_t *nops = nullptr;
+    size_t nops_cap = 0;
+    uint8_t *cigar = nullptr;
+    size_t cigar_bytes = 0;
+    uint8_t *cigar_q = nullptr;
+    size_t cigar_q_bytes = 0;
+    int32_t *scratch_i32 = nullptr;  // stand-ins for result arrays the mode does not expose
+    size_t scratch_i32_cap = 0;
+    uint32_t *aux = nullptr;
+    // last batch
+    uint32_t last_n = 0, last_max_q = 0, last_max_t = 0, last_cigar_cap = 0;
+    bool last_tb = false;
+    G2Timing tm;
+    bool tm_init = false;
+};
+
+Impl *impl_of(gasal_gpu_storage_t *s) {
+    if (!s->impl) s->impl = new Impl();
+    return (Impl *)s->impl;
+}
+
+void report(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    fputs(buf, stderr);
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error = buf;
+    g_error_count++;
+}
+
+}  // namespace
+
+// Error exit (src/gasal.h:20).  In error mode 1 it returns; callers return too.
+extern "C" void gasal2_fail(int code) {
+    if (g_error_mode == 0) exit(code);
+}
+
+#define G2_FAIL_RET(...)                 \
+    do {                                 \
+        report(__VA_ARGS__);             \
+        gasal2_fail(EXIT_FAILURE);       \
+        return;                          \
+    } while (0)
+
+#define G2_CHECK(call)                                                                                  \
+    do {                                                                                                \
+        hipError_t e__ = (call);                                                                        \
+        if (e__ != hipSuccess) {                                                                        \
+            report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n",              \
+                   hipGetErrorString(e__), (int)e__, __LINE__, __FILE__);                               \
+            gasal2_fail(EXIT_FAILURE);                                                                  \
+        }                                                                                               \
+    } while (0)
+
+extern "C" void gasal2_set_error_mode(int32_t mode) { g_error_mode = mode; }
+extern "C" int32_t gasal2_error_count(void) { return g_error_count; }
+extern "C" const char *gasal2_last_error(void) { return g_last_error.c_str(); }
+extern "C" void gasal2_set_n_code(int32_t n_code) { g_scores.n_code = n_code; g_scores.n_value = (uint32_t)n_code & 0xFu; }
+extern "C" int32_t gasal2_get_n_code(void) { return g_scores.n_code; }
+extern "C" void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty) {
+    g_scores.has_np = has_penalty ? 1 : 0;
+    g_scores.np = penalty;
+}
+extern "C" void gasal2_set_dispatch(int32_t policy) { g2_set_dispatch(policy); }
+
+// ------------------------------------------------------------ Parameters ---
+// src/args_parser.cpp:8-39 (defaults), :41-44, :46-52, :54-75, :77-95, :97-240, :242-250
+Parameters::Parameters(int argc_, char **argv_) {
+    sa = 1; sb = 4; gapo = 6; gape = 1;
+    start_pos = WITHOUT_START;
+    print_out = 0;
+    n_threads = 1;
+    k_band = 0;
+    isPacked = false;
+    isReverseComplement = false;
+    secondBest = FALSE;
+    semiglobal_skipping_head = TARGET;
+    semiglobal_skipping_tail = TARGET;
+    algo = UNKNOWN;
+    query_batch_fasta_filename = "";
+    target_batch_fasta_filename = "";
+    argc = argc_;
+    argv = argv_;
+}
+
+Parameters::~Parameters() {
+    query_batch_fasta.close();
+    target_batch_fasta.close();
+}
+
+void Parameters::print() {
+    std::cerr << "sa=" << sa << " , sb=" << sb << " , gapo=" << gapo << " , gape=" << gape << std::endl;
+    std::cerr << "start_pos=" << start_pos << " , print_out=" << print_out << " , n_threads=" << n_threads << std::endl;
+    std::cerr << "semiglobal_skipping_head=" << semiglobal_skipping_head
+              << " , semiglobal_skipping_tail=" << semiglobal_skipping_tail << " , algo=" << algo << std::endl;
+    std::cerr << std::boolalpha << "isPacked = " << isPacked << " , secondBest = " << secondBest << std::endl;
+    std::cerr << "query_batch_fasta_filename=" << query_batch_fasta_filename
+              << " , target_batch_fasta_filename=" << target_batch_fasta_filename << std::endl;
+}
+
+void Parameters::failure(fail_type f) {
+    switch (f) {
+        case NOT_ENOUGH_ARGS:
+            std::cerr << "Not enough Parameters. Required: -y AL_TYPE file1.fasta file2.fasta. See help (--help, -h) for usage. " << std::endl;
+            break;
+        case WRONG_ARG:
+            std::cerr << "Wrong argument. See help (--help, -h) for usage. " << std::endl;
+            break;
+        case WRONG_FILES:
+            std::cerr << "File error: either a file doesn't exist, or cannot be opened." << std::endl;
+            break;
+        default:
+            break;
+    }
+    exit(1);
+}
+
+void Parameters::help() {
+    std::cerr << "Usage: ./test_prog.out [-a] [-b] [-q] [-r] [-s] [-t] [-p] [-n] [-y] <query_batch.fasta> <target_batch.fasta>" << std::endl;
+    std::cerr << "Options: -a INT    match score [" << sa << "]" << std::endl;
+    std::cerr << "         -b INT    mismatch penalty [" << sb << "]" << std::endl;
+    std::cerr << "         -q INT    gap open penalty [" << gapo << "]" << std::endl;
+    std::cerr << "         -r INT    gap extension penalty [" << gape << "]" << std::endl;
+    std::cerr << "         -s        find the start position" << std::endl;
+    std::cerr << "         -t        compute traceback. With this option enabled, \"-s\" has no effect as start position will always be computed with traceback" << std::endl;
+    std::cerr << "         -p        print the alignment results" << std::endl;
+    std::cerr << "         -n INT    Number of threads [" << n_threads << "]" << std::endl;
+    std::cerr << "         -y AL_TYPE       Alignment type . Must be \"local\", \"semi_global\", \"global\", \"ksw\" " << std::endl;
+    std::cerr << "         -x HEAD TAIL     specifies, for semi-global alignment, wha should be skipped for heads and tails of the sequences. (NONE, QUERY, TARGET, BOTH)" << std::endl;
+    std::cerr << "         -k INT    Band width in case \"banded\" is selected." << std::endl;
+    std::cerr << "         --help, -h : displays this message." << std::endl;
+    std::cerr << "         --second-best   displays second best score (WITHOUT_START only)." << std::endl;
+    std::cerr << "Single-pack multi-Parameters (e.g. -sp) is not supported." << std::endl;
+    std::cerr << "		  " << std::endl;
+}
+
+static bool parse_source(const std::string &s, data_source &out) {
+    if (s == "NONE") out = NONE;
+    else if (s == "TARGET") out = TARGET;
+    else if (s == "QUERY") out = QUERY;
+    else if (s == "BOTH") out = BOTH;
+    else return false;
+    return true;
+}
+
+void Parameters::parse() {
+    for (int c = 1; c < argc; c++) {
+        std::string a(argv[c]);
+        if (a == "--help" || a == "-h") { help(); exit(0); }
+    }
+    if (argc < 4) failure(NOT_ENOUGH_ARGS);
+    int c;
+    for (c = 1; c < argc - 2; c++) {
+        std::string cur(argv[c]);
+        if (cur.size() >= 2 && cur[0] == '-' && cur[1] == '-') {
+            if (cur == "--help") { help(); exit(0); }
+            if (cur == "--second-best") secondBest = TRUE;
+        } else if (!cur.empty() && cur[0] == '-') {
+            if (cur.size() > 2) failure(WRONG_ARG);
+            char param = cur.size() > 1 ? cur[1] : 0;
+            auto next = [&]() -> std::string { c++; return c < argc ? std::string(argv[c]) : std::string(); };
+            switch (param) {
+                case 'y': {
+                    std::string v = next();
+                    if (v == "local") algo = LOCAL;
+                    else if (v == "semi_global") algo = SEMI_GLOBAL;
+                    else if (v == "global") algo = GLOBAL;
+                    else if (v == "ksw") algo = KSW;
+                    break;
+                }
+                case 'a': sa = std::stoi(next()); break;
+                case 'b': sb = std::stoi(next()); break;
+                case 'q': gapo = std::stoi(next()); break;
+                case 'r': gape = std::stoi(next()); break;
+                case 's': start_pos = WITH_START; break;
+                case 't': start_pos = WITH_TB; break;
+                case 'p': print_out = 1; break;
+                case 'n': n_threads = std::stoi(next()); break;
+                case 'k': k_band = std::stoi(next()); break;
+                case 'x': {
+                    if (!parse_source(next(), semiglobal_skipping_head)) failure(WRONG_ARG);
+                    if (!parse_source(next(), semiglobal_skipping_tail)) failure(WRONG_ARG);
+                    break;
+                }
+                default: break;
+            }
+        } else {
+            failure(WRONG_ARG);
+        }
+    }
+    query_batch_fasta_filename = std::string(argv[c]);
+    c++;
+    target_batch_fasta_filename = std::string(argv[c]);
+    fileopen();
+}
+
+void Parameters::fileopen() {
+    query_batch_fasta.open(query_batch_fasta_filename, std::ifstream::in);
+    if (!query_batch_fasta) failure(WRONG_FILES);
+    target_batch_fasta.open(target_batch_fasta_filename);
+    if (!target_batch_fasta) failure(WRONG_FILES);
+}
+
+extern "C" Parameters *gasal2_parameters_new(void) { return new Parameters(0, nullptr); }
+extern "C" void gasal2_parameters_free(Parameters *p) { delete p; }
+extern "C" void gasal2_parameters_get(const Parameters *p, gasal2_params_t *o) {
+    o->sa = p->sa; o->sb = p->sb; o->gapo = p->gapo; o->gape = p->gape;
+    o->start_pos = p->start_pos; o->print_out = p->print_out; o->n_threads = p->n_threads;
+    o->k_band = p->k_band; o->secondBest = p->secondBest; o->isPacked = p->isPacked ? 1 : 0;
+    o->isReverseComplement = p->isReverseComplement ? 1 : 0;
+    o->semiglobal_skipping_head = p->semiglobal_skipping_head;
+    o->semiglobal_skipping_tail = p->semiglobal_skipping_tail;
+    o->algo = p->algo;
+}
+extern "C" void gasal2_parameters_set(Parameters *p, const gasal2_params_t *i) {
+    p->sa = i->sa; p->sb = i->sb; p->gapo = i->gapo; p->gape = i->gape;
+    p->start_pos = (comp_start)i->start_pos; p->print_out = i->print_out; p->n_threads = i->n_threads;
+    p->k_band = i->k_band; p->secondBest = (Bool)(i->secondBest ? 1 : 0); p->isPacked = i->isPacked != 0;
+    p->isReverseComplement = i->isReverseComplement != 0;
+    p->semiglobal_skipping_head = (data_source)i->semiglobal_skipping_head;
+    p->semiglobal_skipping_tail = (data_source)i->semiglobal_skipping_tail;
+    p->algo = (algo_type)i->algo;
+}
+
+// ------------------------------------------------------------ host batch ---
+// src/host_batch.cpp:11-22
+extern "C" host_batch_t *gasal_host_batch_new(uint32_t batch_bytes, uint32_t offset) {
+    host_batch_t *res = (host_batch_t *)calloc(1, sizeof(host_batch_t));
+    G2_CHECK(hipHostMalloc((void **)&(res->data), (size_t)batch_bytes * sizeof(uint8_t), hipHostMallocDefault));
+    res->page_size = batch_bytes;
+    res->data_size = 0;
+    res->is_locked = 0;
+    res->offset = offset;
+    res->next = NULL;
+    return res;
+}
+
+// src/host_batch.cpp:24-41
+extern "C" void gasal_host_batch_destroy(host_batch_t *res) {
+    if (res == NULL) {
+        report("[GASAL ERROR] Trying to free a NULL pointer\n");
+        gasal2_fail(1);
+        return;
+    }
+    while (res) {
+        host_batch_t *nx = res->next;
+        if (res->data != NULL) G2_CHECK(hipHostFree(res->data));
+        free(res);
+        res = nx;
+    }
+}
+
+// src/host_batch.cpp:43-47
+extern "C" host_batch_t *gasal_host_batch_getlast(host_batch_t *arg) {
+    while (arg->next != NULL) arg = arg->next;
+    return arg;
+}
+
+// src/host_batch.cpp:49-75
+extern "C" void gasal_host_batch_reset(gasal_gpu_storage_t *gpu_storage) {
+    host_batch_t *lists[2] = {gpu_storage->extensible_host_unpacked_query_batch,
+                              gpu_storage->extensible_host_unpacked_target_batch};
+    for (int i = 0; i < 2; i++)
+        for (host_batch_t *p = lists[i]; p != NULL; p = p->next) {
+            p->data_size = 0;
+            p->offset = 0;
+            p->is_locked = 0;
+        }
+}
+
+// src/host_batch.cpp:79-153
+extern "C" uint32_t gasal_host_batch_fill(gasal_gpu_storage_t *gpu_storage, uint32_t idx, const char *data, uint32_t size,
+                                          data_source SRC) {
+    host_batch_t *cur_page = NULL;
+    uint32_t *p_batch_bytes = NULL;
+    if (SRC == QUERY) {
+        cur_page = gpu_storage->extensible_host_unpacked_query_batch;
+        p_batch_bytes = &(gpu_storage->host_max_query_batch_bytes);
+    } else if (SRC == TARGET) {
+        cur_page = gpu_storage->extensible_host_unpacked_target_batch;
+        p_batch_bytes = &(gpu_storage->host_max_target_batch_bytes);
+    } else {
+        return idx;
+    }
+    uint32_t nbr_N = 0;
+    while ((size + nbr_N) % 8) nbr_N++;
+    while (cur_page->is_locked) cur_page = cur_page->next;
+    if (cur_page->next == NULL && cur_page->page_size - cur_page->data_size < size + nbr_N) {
+        fprintf(stderr,
+                "[GASAL WARNING:] Trying to write %d bytes while only %d remain (%s) (block size %d, filled %d bytes).\n"
+                "                 Allocating a new block of size %d, total size available reaches %d. Doing this repeadtedly slows down the execution.\n",
+                size + nbr_N, cur_page->page_size - cur_page->data_size, (SRC == QUERY ? "query" : "target"),
+                cur_page->page_size, cur_page->data_size, cur_page->page_size * 2, *p_batch_bytes + cur_page->page_size * 2);
+        uint32_t newsize = cur_page->page_size * 2;
+        while (newsize < size + nbr_N) newsize *= 2;
+        host_batch_t *res = gasal_host_batch_new(newsize, cur_page->offset + cur_page->data_size);
+        cur_page->next = res;
+        cur_page->is_locked = 1;
+        *p_batch_bytes = *p_batch_bytes + newsize;
+        cur_page = cur_page->next;
+    }
+    if (cur_page->next != NULL && cur_page->page_size - cur_page->data_size < size + nbr_N) {
+        cur_page->next->offset = cur_page->offset + cur_page->data_size;
+        cur_page->is_locked = 1;
+        cur_page = cur_page->next;
+    }
+    if (cur_page->page_size - cur_page->data_size >= size + nbr_N) {
+        memcpy(&(cur_page->data[idx - cur_page->offset]), data, size);
+        memset(&(cur_page->data[idx + size - cur_page->offset]), g_scores.n_code, nbr_N);
+        idx = idx + size + nbr_N;
+        cur_page->data_size += size + nbr_N;
+    }
+    return idx;
+}
+
+// src/host_batch.cpp:162-223
+extern "C" uint32_t gasal_host_batch_add(gasal_gpu_storage_t *gpu_storage, uint32_t idx, const char *data, uint32_t size,
+                                         data_source SRC) {
+    host_batch_t *cur_page = NULL;
+    uint32_t *p_batch_bytes = NULL;
+    if (SRC == QUERY) {
+        cur_page = gpu_storage->extensible_host_unpacked_query_batch;
+        p_batch_bytes = &(gpu_storage->host_max_query_batch_bytes);
+    } else if (SRC == TARGET) {
+        cur_page = gpu_storage->extensible_host_unpacked_target_batch;
+        p_batch_bytes = &(gpu_storage->host_max_target_batch_bytes);
+    } else {
+        return idx;
+    }
+    for (;;) {
+        if (*p_batch_bytes >= idx + size && (cur_page->next == NULL || cur_page->next->offset >= idx + size)) {
+            memcpy(&(cur_page->data[idx - cur_page->offset]), data, size);
+            idx = idx + size;
+            if (idx - cur_page->offset > cur_page->data_size) cur_page->data_size = idx - cur_page->offset;
+            return idx;
+        } else if (*p_batch_bytes >= idx + size && cur_page->next != NULL && cur_page->next->offset < idx + size) {
+            cur_page = cur_page->next;
+        } else {
+            fprintf(stderr,
+                    "[GASAL WARNING:] Trying to write %d bytes at position %d on host memory (%s) while only  %d bytes are available. Therefore, allocating %d bytes more on CPU. Repeating this many times can provoke a degradation of performance.\n",
+                    size, idx, (SRC == QUERY ? "query" : "target"), *p_batch_bytes, *p_batch_bytes * 2);
+            *p_batch_bytes += *p_batch_bytes;
+            while (*p_batch_bytes < size) *p_batch_bytes += *p_batch_bytes;
+            host_batch_t *res = gasal_host_batch_new(*p_batch_bytes, idx);
+            cur_page->next = res;
+            cur_page = cur_page->next;
+        }
+    }
+}
+
+// src/host_batch.cpp:156-159
+extern "C" uint32_t gasal_host_batch_addbase(gasal_gpu_storage_t *gpu_storage, uint32_t idx, const char base,
+                                             data_source SRC) {
+    return gasal_host_batch_add(gpu_storage, idx, &base, 1, SRC);
+}
+
+// src/host_batch.cpp:228-244
+extern "C" void gasal_host_batch_print(host_batch_t *res) {
+    fprintf(stderr, "[GASAL PRINT] Page data: offset=%d, next_offset=%d, data size=%d, page size=%d\n", res->offset,
+            (res->next != NULL ? (int)res->next->offset : -1), res->data_size, res->page_size);
+}
+extern "C" void gasal_host_batch_printall(host_batch_t *res) {
+    for (; res != NULL; res = res->next) {
+        gasal_host_batch_print(res);
+        if (res->next != NULL) fprintf(stderr, "+--->");
+    }
+}
+
+// ------------------------------------------------------------ results ------
+// src/res.cpp:8-73
+extern "C" gasal_res_t *gasal_res_new_host(uint32_t max_n_alns, Parameters *params) {
+    gasal_res_t *res = (gasal_res_t *)calloc(1, sizeof(gasal_res_t));
+    G2_CHECK(hipHostMalloc((void **)&(res->aln_score), (size_t)max_n_alns * sizeof(int32_t), hipHostMallocDefault));
+    if (params->algo != GLOBAL) {
+        if (params->start_pos == WITH_START || params->start_pos == WITH_TB) {
+            G2_CHECK(hipHostMalloc((void **)&(res->query_batch_start), (size_t)max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+            G2_CHECK(hipHostMalloc((void **)&(res->target_batch_start), (size_t)max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        }
+        G2_CHECK(hipHostMalloc((void **)&(res->query_batch_end), (size_t)max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        G2_CHECK(hipHostMalloc((void **)&(res->target_batch_end), (size_t)max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    if (params->start_pos == WITH_TB)
+        G2_CHECK(hipHostMalloc((void **)&(res->n_cigar_ops), (size_t)max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+    return res;
+}
+
+// src/res.cpp:76-101
+extern "C" gasal_res_t *gasal_res_new_device(gasal_res_t *device_cpy) {
+    gasal_res_t *d_c = NULL;
+    G2_CHECK(hipMalloc((void **)&d_c, sizeof(gasal_res_t)));
+    G2_CHECK(hipMemcpy(d_c, device_cpy, sizeof(gasal_res_t), hipMemcpyHostToDevice));
+    return d_c;
+}
+
+// src/res.cpp:106-140
+extern "C" gasal_res_t *gasal_res_new_device_cpy(uint32_t max_n_alns, Parameters *params) {
+    gasal_res_t *res = (gasal_res_t *)calloc(1, sizeof(gasal_res_t));
+    G2_CHECK(hipMalloc((void **)&(res->aln_score), (size_t)max_n_alns * sizeof(int32_t)));
+    if (params->algo != GLOBAL) {
+        if (params->start_pos == WITH_START || params->start_pos == WITH_TB) {
+            G2_CHECK(hipMalloc((void **)&(res->query_batch_start), (size_t)max_n_alns * sizeof(uint32_t)));
+            G2_CHECK(hipMalloc((void **)&(res->target_batch_start), (size_t)max_n_alns * sizeof(uint32_t)));
+        }
+        G2_CHECK(hipMalloc((void **)&(res->query_batch_end), (size_t)max_n_alns * sizeof(uint32_t)));
+        G2_CHECK(hipMalloc((void **)&(res->target_batch_end), (size_t)max_n_alns * sizeof(uint32_t)));
+    }
+    return res;
+}
+
+// src/res.cpp:143-158
+extern "C" void gasal_res_destroy_host(gasal_res_t *res) {
+    if (res == NULL) return;
+    if (res->aln_score) G2_CHECK(hipHostFree(res->aln_score));
+    if (res->query_batch_start) G2_CHECK(hipHostFree(res->query_batch_start));
+    if (res->target_batch_start) G2_CHECK(hipHostFree(res->target_batch_start));
+    if (res->query_batch_end) G2_CHECK(hipHostFree(res->query_batch_end));
+    if (res->target_batch_end) G2_CHECK(hipHostFree(res->target_batch_end));
+    if (res->n_cigar_ops) G2_CHECK(hipHostFree(res->n_cigar_ops));
+    if (res->cigar) G2_CHECK(hipHostFree(res->cigar));
+    free(res);
+}
+
+// src/res.cpp:160-177
+extern "C" void gasal_res_destroy_device(gasal_res_t *device_res, gasal_res_t *device_cpy) {
+    if (device_cpy == NULL || device_res == NULL) return;
+    if (device_cpy->aln_score) G2_CHECK(hipFree(device_cpy->aln_score));
+    if (device_cpy->query_batch_start) G2_CHECK(hipFree(device_cpy->query_batch_start));
+    if (device_cpy->target_batch_start) G2_CHECK(hipFree(device_cpy->target_batch_start));
+    if (device_cpy->query_batch_end) G2_CHECK(hipFree(device_cpy->query_batch_end));
+    if (device_cpy->target_batch_end) G2_CHECK(hipFree(device_cpy->target_batch_end));
+    if (device_cpy->cigar) G2_CHECK(hipFree(device_cpy->cigar));
+    G2_CHECK(hipFree(device_res));
+    free(device_cpy);
+}
+
+// ------------------------------------------------------------ interfaces ---
+// src/interfaces.cpp:81-96
+extern "C" void gasal_op_fill(gasal_gpu_storage_t *gpu_storage_t, uint8_t *data, uint32_t nbr_seqs_in_stream,
+                              data_source SRC) {
+    uint8_t *host_op = NULL;
+    if (SRC == QUERY) host_op = gpu_storage_t->host_query_op;
+    else if (SRC == TARGET) host_op = gpu_storage_t->host_target_op;
+    if (host_op) memcpy(host_op, data, nbr_seqs_in_stream);
+}
+
+template <typename T>
+static T *host_realloc(T *src, uint32_t new_n, uint32_t old_n) {
+    if (new_n < old_n) {
+        report("[GASAL ERROR] cudoHostRealloc: invalid sizes. New size < old size (%d < %d)", new_n, old_n);
+        gasal2_fail(EXIT_FAILURE);
+        return src;
+    }
+    T *dst = NULL;
+    G2_CHECK(hipHostMalloc((void **)&dst, (size_t)new_n * sizeof(T), hipHostMallocDefault));
+    if (src) {
+        memcpy(dst, src, (size_t)old_n * sizeof(T));
+        G2_CHECK(hipHostFree(src));
+    }
+    return dst;
+}
+
+// src/interfaces.cpp:26-78
+extern "C" void gasal_host_alns_resize(gasal_gpu_storage_t *gpu_storage, int new_max_alns, Parameters *params) {
+    fprintf(stderr, "[GASAL WARNING] Resizing gpu_storage from %d sequences to %d sequences... ",
+            gpu_storage->host_max_n_alns, new_max_alns);
+    uint32_t old = gpu_storage->host_max_n_alns, nn = (uint32_t)new_max_alns;
+    gpu_storage->host_query_op = host_realloc(gpu_storage->host_query_op, nn, old);
+    gpu_storage->host_target_op = host_realloc(gpu_storage->host_target_op, nn, old);
+    if (params->algo == KSW) gpu_storage->host_seed_scores = host_realloc(gpu_storage->host_seed_scores, nn, old);
+    gpu_storage->host_query_batch_lens = host_realloc(gpu_storage->host_query_batch_lens, nn, old);
+    gpu_storage->host_target_batch_lens = host_realloc(gpu_storage->host_target_batch_lens, nn, old);
+    gpu_storage->host_query_batch_offsets = host_realloc(gpu_storage->host_query_batch_offsets, nn, old);
+    gpu_storage->host_target_batch_offsets = host_realloc(gpu_storage->host_target_batch_offsets, nn, old);
+    uint8_t *cig = gpu_storage->host_res ? gpu_storage->host_res->cigar : NULL;
+    if (gpu_storage->host_res) gpu_storage->host_res->cigar = NULL;
+    gasal_res_destroy_host(gpu_storage->host_res);
+    gpu_storage->host_res = gasal_res_new_host(nn, params);
+    gpu_storage->host_res->cigar = cig;
+    if (params->secondBest) {
+        gasal_res_destroy_host(gpu_storage->host_res_second);
+        gpu_storage->host_res_second = gasal_res_new_host(nn, params);
+    }
+    gpu_storage->host_max_n_alns = nn;
+    fprintf(stderr, " done. This can harm performance.\n");
+}
+
+// src/interfaces.cpp:98-128
+extern "C" void gasal_set_device(int gpu_select, bool isPrintingProp) {
+    if (isPrintingProp) {
+        int num_devices = 0;
+        (void)hipGetDeviceCount(&num_devices);
+        fprintf(stderr, "Found %d GPUs\n", num_devices);
+        if (gpu_select > num_devices - 1) {
+            report("Error: can't select device %d when only %d devices are selected (range from 0 to %d)\n", gpu_select,
+                   num_devices, num_devices - 1);
+            gasal2_fail(EXIT_FAILURE);
+            return;
+        }
+        if (num_devices > 0) {
+            hipDeviceProp_t prop;
+            for (int dev = 0; dev < num_devices; dev++) {
+                (void)hipGetDeviceProperties(&prop, dev);
+                fprintf(stderr, "\tGPU %d: %s\n", dev, prop.name);
+            }
+            (void)hipGetDeviceProperties(&prop, gpu_select);
+            fprintf(stderr, "Selected device %d : %s\n", gpu_select, prop.name);
+            (void)hipSetDevice(gpu_select);
+        }
+    } else {
+        (void)hipSetDevice(gpu_select);
+    }
+}
+
+// ------------------------------------------------------------ ctors --------
+// src/ctors.cpp:17-23
+extern "C" gasal_gpu_storage_v gasal_init_gpu_storage_v(int n_streams) {
+    gasal_gpu_storage_v v;
+    v.a = (gasal_gpu_storage_t *)calloc((size_t)n_streams, sizeof(gasal_gpu_storage_t));
+    v.n = n_streams;
+    return v;
+}
+
+static inline uint32_t pad8u(uint32_t n) { return n % 8 ? n + (8 - (n % 8)) : n; }
+
+// src/ctors.cpp:26-128
+extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max_query_len, int max_target_len,
+                                   int max_n_alns, Parameters *params) {
+    uint32_t mq8 = pad8u((uint32_t)max_query_len), mt8 = pad8u((uint32_t)max_target_len);
+    uint32_t host_max_query_batch_bytes = (uint32_t)max_n_alns * mq8;
+    uint32_t gpu_max_query_batch_bytes = (uint32_t)max_n_alns * mq8;
+    uint32_t host_max_target_batch_bytes = (uint32_t)max_n_alns * mt8;
+    uint32_t gpu_max_target_batch_bytes = (uint32_t)max_n_alns * mt8;
+    uint32_t host_max_n_alns = (uint32_t)max_n_alns, gpu_max_n_alns = (uint32_t)max_n_alns;
+    for (int i = 0; i < gpu_storage_vec->n; i++) {
+        gasal_gpu_storage_t *s = &(gpu_storage_vec->a[i]);
+        s->extensible_host_unpacked_query_batch = gasal_host_batch_new(host_max_query_batch_bytes, 0);
+        s->extensible_host_unpacked_target_batch = gasal_host_batch_new(host_max_target_batch_bytes, 0);
+        G2_CHECK(hipMalloc((void **)&(s->unpacked_query_batch), (size_t)gpu_max_query_batch_bytes + 16));
+        G2_CHECK(hipMalloc((void **)&(s->unpacked_target_batch), (size_t)gpu_max_target_batch_bytes + 16));
+        G2_CHECK(hipHostMalloc((void **)&(s->host_query_op), host_max_n_alns, hipHostMallocDefault));
+        G2_CHECK(hipHostMalloc((void **)&(s->host_target_op), host_max_n_alns, hipHostMallocDefault));
+        memset(s->host_query_op, 0, host_max_n_alns);
+        memset(s->host_target_op, 0, host_max_n_alns);
+        G2_CHECK(hipMalloc((void **)&(s->query_op), gpu_max_n_alns));
+        G2_CHECK(hipMalloc((void **)&(s->target_op), gpu_max_n_alns));
+        if (params->isPacked) {
+            s->packed_query_batch = (uint32_t *)s->unpacked_query_batch;
+            s->packed_target_batch = (uint32_t *)s->unpacked_target_batch;
+        } else {
+            G2_CHECK(hipMalloc((void **)&(s->packed_query_batch), (size_t)(gpu_max_query_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+            G2_CHECK(hipMalloc((void **)&(s->packed_target_batch), (size_t)(gpu_max_target_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+        }
+        if (params->algo == KSW) {
+            G2_CHECK(hipHostMalloc((void **)&(s->host_seed_scores), (size_t)host_max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+            G2_CHECK(hipMalloc((void **)&(s->seed_scores), (size_t)host_max_n_alns * sizeof(uint32_t)));
+        } else {
+            s->host_seed_scores = NULL;
+            s->seed_scores = NULL;
+        }
+        G2_CHECK(hipHostMalloc((void **)&(s->host_query_batch_lens), (size_t)host_max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        G2_CHECK(hipHostMalloc((void **)&(s->host_target_batch_lens), (size_t)host_max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        G2_CHECK(hipHostMalloc((void **)&(s->host_query_batch_offsets), (size_t)host_max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        G2_CHECK(hipHostMalloc((void **)&(s->host_target_batch_offsets), (size_t)host_max_n_alns * sizeof(uint32_t), hipHostMallocDefault));
+        G2_CHECK(hipMalloc((void **)&(s->query_batch_lens), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+        G2_CHECK(hipMalloc((void **)&(s->target_batch_lens), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+        G2_CHECK(hipMalloc((void **)&(s->query_batch_offsets), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+        G2_CHECK(hipMalloc((void **)&(s->target_batch_offsets), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+        s->host_res = gasal_res_new_host(host_max_n_alns, params);
+        if (params->start_pos == WITH_TB)
+            G2_CHECK(hipHostMalloc((void **)&(s->host_res->cigar), gpu_max_query_batch_bytes, hipHostMallocDefault));
+        s->device_cpy = gasal_res_new_device_cpy(gpu_max_n_alns, params);
+        s->device_res = gasal_res_new_device(s->device_cpy);
+        if (params->secondBest) {
+            s->host_res_second = gasal_res_new_host(host_max_n_alns, params);
+            s->device_cpy_second = gasal_res_new_device_cpy(host_max_n_alns, params);
+            s->device_res_second = gasal_res_new_device(s->device_cpy_second);
+        } else {
+            s->host_res_second = NULL;
+            s->device_cpy_second = NULL;
+            s->device_res_second = NULL;
+        }
+        s->packed_tb_matrix_size = 0;   // direction words are sized per batch (impl)
+        s->packed_tb_matrices = NULL;
+        G2_CHECK(hipStreamCreate(&(s->str)));
+        s->is_free = 1;
+        s->host_max_query_batch_bytes = host_max_query_batch_bytes;
+        s->host_max_target_batch_bytes = host_max_target_batch_bytes;
+        s->host_max_n_alns = host_max_n_alns;
+        s->gpu_max_query_batch_bytes = gpu_max_query_batch_bytes;
+        s->gpu_max_target_batch_bytes = gpu_max_target_batch_bytes;
+        s->gpu_max_n_alns = gpu_max_n_alns;
+        s->current_n_alns = 0;
+        s->impl = new Impl();
+    }
+}
+
+static void impl_free(Impl *m) {
+    if (!m) return;
+    void *ptrs[] = {m->keys, m->keys_alt, m->perm, m->perm_alt, m->sort_tmp, m->bnd, m->tb, m->eh,
+                    m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (m->tm_init)
+        for (int i = 0; i < 6; i++) (void)hipEventDestroy(m->tm.ev[i]);
+    delete m;
+}
+
+// src/ctors.cpp:130-190
+extern "C" void gasal_destroy_streams(gasal_gpu_storage_v *gpu_storage_vec, Parameters *params) {
+    for (int i = 0; i < gpu_storage_vec->n; i++) {
+        gasal_gpu_storage_t *s = &(gpu_storage_vec->a[i]);
+        if (s->str) (void)hipStreamSynchronize(s->str);
+        gasal_host_batch_destroy(s->extensible_host_unpacked_query_batch);
+        gasal_host_batch_destroy(s->extensible_host_unpacked_target_batch);
+        gasal_res_destroy_host(s->host_res);
+        gasal_res_destroy_device(s->device_res, s->device_cpy);
+        if (params->secondBest) {
+            gasal_res_destroy_host(s->host_res_second);
+            gasal_res_destroy_device(s->device_res_second, s->device_cpy_second);
+        }
+        if (s->seed_scores) G2_CHECK(hipFree(s->seed_scores));
+        if (s->host_seed_scores) G2_CHECK(hipHostFree(s->host_seed_scores));
+        if (s->query_op) G2_CHECK(hipFree(s->query_op));
+        if (s->target_op) G2_CHECK(hipFree(s->target_op));
+        if (s->host_query_op) G2_CHECK(hipHostFree(s->host_query_op));
+        if (s->host_target_op) G2_CHECK(hipHostFree(s->host_target_op));
+        if (s->host_query_batch_offsets) G2_CHECK(hipHostFree(s->host_query_batch_offsets));
+        if (s->host_target_batch_offsets) G2_CHECK(hipHostFree(s->host_target_batch_offsets));
+        if (s->host_query_batch_lens) G2_CHECK(hipHostFree(s->host_query_batch_lens));
+        if (s->host_target_batch_lens) G2_CHECK(hipHostFree(s->host_target_batch_lens));
+        if (s->unpacked_query_batch) G2_CHECK(hipFree(s->unpacked_query_batch));
+        if (s->unpacked_target_batch) G2_CHECK(hipFree(s->unpacked_target_batch));
+        if (!params->isPacked) {
+            if (s->packed_query_batch) G2_CHECK(hipFree(s->packed_query_batch));
+            if (s->packed_target_batch) G2_CHECK(hipFree(s->packed_target_batch));
+        }
+        if (s->query_batch_offsets) G2_CHECK(hipFree(s->query_batch_offsets));
+        if (s->target_batch_offsets) G2_CHECK(hipFree(s->target_batch_offsets));
+        if (s->query_batch_lens) G2_CHECK(hipFree(s->query_batch_lens));
+        if (s->target_batch_lens) G2_CHECK(hipFree(s->target_batch_lens));
+        impl_free((Impl *)s->impl);
+        s->impl = NULL;
+        if (s->str) G2_CHECK(hipStreamDestroy(s->str));
+        s->str = NULL;
+    }
+}
+
+// src/ctors.cpp:193-196
+extern "C" void gasal_destroy_gpu_storage_v(gasal_gpu_storage_v *gpu_storage_vec) {
+    if (gpu_storage_vec->a != NULL) free(gpu_storage_vec->a);
+    gpu_storage_vec->a = NULL;
+}
+
+// src/ctors.cpp:202-231 (deprecated in the reference)
+extern "C" void gasal_gpu_mem_alloc(gasal_gpu_storage_t *gpu_storage, int gpu_max_query_batch_bytes,
+                                    int gpu_max_target_batch_bytes, int gpu_max_n_alns, Parameters *params) {
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->unpacked_query_batch), (size_t)gpu_max_query_batch_bytes + 16));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->unpacked_target_batch), (size_t)gpu_max_target_batch_bytes + 16));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->packed_query_batch), (size_t)(gpu_max_query_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->packed_target_batch), (size_t)(gpu_max_target_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->query_batch_lens), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->target_batch_lens), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->query_batch_offsets), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+    G2_CHECK(hipMalloc((void **)&(gpu_storage->target_batch_offsets), (size_t)gpu_max_n_alns * sizeof(uint32_t)));
+    if (gpu_storage->device_cpy == NULL) gpu_storage->device_cpy = gasal_res_new_device_cpy((uint32_t)gpu_max_n_alns, params);
+    gpu_storage->device_res = gasal_res_new_device(gpu_storage->device_cpy);
+    gpu_storage->gpu_max_query_batch_bytes = (uint32_t)gpu_max_query_batch_bytes;
+    gpu_storage->gpu_max_target_batch_bytes = (uint32_t)gpu_max_target_batch_bytes;
+    gpu_storage->gpu_max_n_alns = (uint32_t)gpu_max_n_alns;
+}
+
+// src/ctors.cpp:234-252 (deprecated in the reference)
+extern "C" void gasal_gpu_mem_free(gasal_gpu_storage_t *gpu_storage, Parameters *params) {
+    if (gpu_storage->unpacked_query_batch) G2_CHECK(hipFree(gpu_storage->unpacked_query_batch));
+    if (gpu_storage->unpacked_target_batch) G2_CHECK(hipFree(gpu_storage->unpacked_target_batch));
+    if (gpu_storage->packed_query_batch) G2_CHECK(hipFree(gpu_storage->packed_query_batch));
+    if (gpu_storage->packed_target_batch) G2_CHECK(hipFree(gpu_storage->packed_target_batch));
+    if (gpu_storage->query_batch_offsets) G2_CHECK(hipFree(gpu_storage->query_batch_offsets));
+    if (gpu_storage->target_batch_offsets) G2_CHECK(hipFree(gpu_storage->target_batch_offsets));
+    if (gpu_storage->query_batch_lens) G2_CHECK(hipFree(gpu_storage->query_batch_lens));
+    if (gpu_storage->target_batch_lens) G2_CHECK(hipFree(gpu_storage->target_batch_lens));
+    gasal_res_destroy_device(gpu_storage->device_res, gpu_storage->device_cpy);
+    if (params->secondBest) gasal_res_destroy_device(gpu_storage->device_res_second, gpu_storage->device_cpy_second);
+}
+
+// ------------------------------------------------------------ launcher -----
+extern "C" void gasal_copy_subst_scores(gasal_subst_scores *subst) {                 // src/gasal_align.cu:329-339
+    g_scores.gapo = subst->gap_open;
+    g_scores.gape = subst->gap_extend;
+    g_scores.gapoe = subst->gap_open + subst->gap_extend;
+    g_scores.match = subst->match;
+    g_scores.mismatch = subst->mismatch;
+}
+
+namespace {
+
+// Validation of src/gasal_align.cu:32-67.  Returns false after reporting.
+bool validate(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n) {
+    if (n <= 0) { report("[GASAL ERROR:] actual_n_alns <= 0\n"); gasal2_fail(EXIT_FAILURE); return false; }
+    if (qb <= 0) { report("[GASAL ERROR:] actual_query_batch_bytes <= 0\n"); gasal2_fail(EXIT_FAILURE); return false; }
+    if (tb <= 0) { report("[GASAL ERROR:] actual_target_batch_bytes <= 0\n"); gasal2_fail(EXIT_FAILURE); return false; }
+    if (qb % 8) { report("[GASAL ERROR:] actual_query_batch_bytes=%d is not a multiple of 8\n", qb); gasal2_fail(EXIT_FAILURE); return false; }
+    if (tb % 8) { report("[GASAL ERROR:] actual_target_batch_bytes=%d is not a multiple of 8\n", tb); gasal2_fail(EXIT_FAILURE); return false; }
+    if (qb > s->host_max_query_batch_bytes) {
+        report("[GASAL ERROR:] actual_query_batch_bytes(%d) > host_max_query_batch_bytes(%d)\n", qb, s->host_max_query_batch_bytes);
+        gasal2_fail(EXIT_FAILURE); return false;
+    }
+    if (tb > s->host_max_target_batch_bytes) {
+        report("[GASAL ERROR:] actual_target_batch_bytes(%d) > host_max_target_batch_bytes(%d)\n", tb, s->host_max_target_batch_bytes);
+        gasal2_fail(EXIT_FAILURE); return false;
+    }
+    if (n > s->host_max_n_alns) {
+        report("[GASAL ERROR:] actual_n_alns(%d) > host_max_n_alns(%d)\n", n, s->host_max_n_alns);
+        gasal2_fail(EXIT_FAILURE); return false;
+    }
+    return true;
+}
+
+// Device-buffer growth of src/gasal_align.cu:70-145.
+void grow_device(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    if (s->gpu_max_query_batch_bytes < qb) {
+        uint32_t i = 2;
+        while (s->gpu_max_query_batch_bytes * i < qb) i++;
+        fprintf(stderr, "[GASAL WARNING:] actual_query_batch_bytes(%d) > Allocated GPU memory (gpu_max_query_batch_bytes=%d). Therefore, allocating %d bytes on GPU (gpu_max_query_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
+                qb, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes * i, s->gpu_max_query_batch_bytes * i);
+        s->gpu_max_query_batch_bytes *= i;
+        bool aliased = (void *)s->packed_query_batch == (void *)s->unpacked_query_batch;
+        if (s->unpacked_query_batch) G2_CHECK(hipFree(s->unpacked_query_batch));
+        if (!aliased && s->packed_query_batch) G2_CHECK(hipFree(s->packed_query_batch));
+        G2_CHECK(hipMalloc((void **)&(s->unpacked_query_batch), (size_t)s->gpu_max_query_batch_bytes + 16));
+        if (params->isPacked) s->packed_query_batch = (uint32_t *)s->unpacked_query_batch;
+        else G2_CHECK(hipMalloc((void **)&(s->packed_query_batch), (size_t)(s->gpu_max_query_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+        if (params->start_pos == WITH_TB) {
+            fprintf(stderr, "[GASAL WARNING:] actual_query_batch_bytes(%d) > Allocated HOST memory for CIGAR (gpu_max_query_batch_bytes=%d). Therefore, allocating %d bytes on the host (gpu_max_query_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
+                    qb, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes);
+            if (s->host_res->cigar != NULL) G2_CHECK(hipHostFree(s->host_res->cigar));
+            G2_CHECK(hipHostMalloc((void **)&(s->host_res->cigar), s->gpu_max_query_batch_bytes, hipHostMallocDefault));
+        }
+    }
+    if (s->gpu_max_target_batch_bytes < tb) {
+        uint32_t i = 2;
+        while (s->gpu_max_target_batch_bytes * i < tb) i++;
+        fprintf(stderr, "[GASAL WARNING:] actual_target_batch_bytes(%d) > Allocated GPU memory (gpu_max_target_batch_bytes=%d). Therefore, allocating %d bytes on GPU (gpu_max_target_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
+                tb, s->gpu_max_target_batch_bytes, s->gpu_max_target_batch_bytes * i, s->gpu_max_target_batch_bytes * i);
+        s->gpu_max_target_batch_bytes *= i;
+        bool aliased = (void *)s->packed_target_batch == (void *)s->unpacked_target_batch;
+        if (s->unpacked_target_batch) G2_CHECK(hipFree(s->unpacked_target_batch));
+        if (!aliased && s->packed_target_batch) G2_CHECK(hipFree(s->packed_target_batch));
+        G2_CHECK(hipMalloc((void **)&(s->unpacked_target_batch), (size_t)s->gpu_max_target_batch_bytes + 16));
+        if (params->isPacked) s->packed_target_batch = (uint32_t *)s->unpacked_target_batch;
+        else G2_CHECK(hipMalloc((void **)&(s->packed_target_batch), (size_t)(s->gpu_max_target_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+    }
+    if (s->gpu_max_n_alns < n) {
+        uint32_t i = 2;
+        while (s->gpu_max_n_alns * i < n) i++;
+        fprintf(stderr, "[GASAL WARNING:] actual_n_alns(%d) > gpu_max_n_alns(%d). Therefore, allocating memory for %d alignments on  GPU (gpu_max_n_alns=%d). Performance may be lost if this is repeated many times.\n",
+                n, s->gpu_max_n_alns, s->gpu_max_n_alns * i, s->gpu_max_n_alns * i);
+        s->gpu_max_n_alns *= i;
+        uint32_t **arrs[] = {&s->query_batch_offsets, &s->target_batch_offsets, &s->query_batch_lens, &s->target_batch_lens};
+        for (auto a : arrs) {
+            if (*a) G2_CHECK(hipFree(*a));
+            G2_CHECK(hipMalloc((void **)a, (size_t)s->gpu_max_n_alns * sizeof(uint32_t)));
+        }
+        if (s->seed_scores) G2_CHECK(hipFree(s->seed_scores));
+        G2_CHECK(hipMalloc((void **)&(s->seed_scores), (size_t)s->gpu_max_n_alns * sizeof(uint32_t)));
+        if (s->query_op) G2_CHECK(hipFree(s->query_op));
+        if (s->target_op) G2_CHECK(hipFree(s->target_op));
+        G2_CHECK(hipMalloc((void **)&(s->query_op), s->gpu_max_n_alns));
+        G2_CHECK(hipMalloc((void **)&(s->target_op), s->gpu_max_n_alns));
+        gasal_res_destroy_device(s->device_res, s->device_cpy);
+        s->device_cpy = gasal_res_new_device_cpy(s->gpu_max_n_alns, params);
+        s->device_res = gasal_res_new_device(s->device_cpy);
+        if (params->secondBest) {
+            gasal_res_destroy_device(s->device_res_second, s->device_cpy_second);
+            s->device_cpy_second = gasal_res_new_device_cpy(s->gpu_max_n_alns, params);
+            s->device_res_second = gasal_res_new_device(s->device_cpy_second);
+        }
+    }
+}
+
+template <typename T>
+void ensure(T *&p, size_t &cap_bytes, size_t need_bytes) {
+    if (cap_bytes >= need_bytes && p) return;
+    if (p) G2_CHECK(hipFree(p));
+    p = nullptr;
+    size_t b = need_bytes + need_bytes / 8 + 256;
+    G2_CHECK(hipMalloc((void **)&p, b));
+    cap_bytes = b;
+}
+
+G2Opts opts_of(const Parameters *p) {
+    G2Opts o;
+    o.algo = p->algo;
+    o.start = p->start_pos;
+    o.second = p->secondBest ? 1 : 0;
+    o.head = p->semiglobal_skipping_head;
+    o.tail = p->semiglobal_skipping_tail;
+    o.k_band = p->k_band;
+    o.is_packed = p->isPacked ? 1 : 0;
+    o.is_rc = p->isReverseComplement ? 1 : 0;
+    return o;
+}
+
+// Sizes the workspace for this batch and fills the device view.
+void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const Parameters *params, G2Dev &d) {
+    Impl *m = impl_of(s);
+    uint32_t mq = 0, mt = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        mq = std::max(mq, s->host_query_batch_lens[k]);
+        mt = std::max(mt, s->host_target_batch_lens[k]);
+    }
+    m->last_max_q = mq; m->last_max_t = mt; m->last_n = n;
+    const G2Opts op = opts_of(params);
+    const uint32_t qw = g2_regs(mq), tw = g2_regs(mt);
+    const uint32_t groups = (n + 63) / 64;
+    size_t b = 0;
+    size_t ncap = (size_t)n * sizeof(uint32_t);
+    if (m->n_cap < n || !m->keys) {
+        for (uint32_t **pp : {&m->keys, &m->keys_alt, &m->perm, &m->perm_alt}) {
+            if (*pp) G2_CHECK(hipFree(*pp));
+            G2_CHECK(hipMalloc((void **)pp, ncap + 256));
+        }
+        m->n_cap = n;
+    }
+    ensure(m->sort_tmp, m->sort_tmp_bytes, g2_sort_tmp_bytes(n));
+    const uint32_t bnd_cols = 8 * qw + 8;
+    ensure(m->bnd, m->bnd_bytes, (size_t)groups * bnd_cols * 64 * 4);
+    const bool tb_on = op.start == WITH_TB;
+    const bool tb_dp = tb_on && (op.algo == LOCAL || op.algo == GLOBAL);
+    if (tb_dp) ensure(m->tb, m->tb_bytes, (size_t)groups * tw * (8 * qw) * 64 * 4);
+    const uint32_t eh_cols = mq + 2;
+    if (op.algo == KSW) ensure(m->eh, m->eh_bytes, (size_t)groups * eh_cols * 64 * 8);
+    const uint32_t cap = g2_cigar_capacity(mq, mt);
+    if (tb_on) {
+        ensure(m->nops, m->nops_cap, (size_t)n * 4);
+        ensure(m->cigar, m->cigar_bytes, (size_t)n * cap);
+        ensure(m->cigar_q, m->cigar_q_bytes, (size_t)qb + 16);
+    }
+    ensure(m->scratch_i32, m->scratch_i32_cap, (size_t)n * 4 * 8);
+    (void)b;
+    m->last_cigar_cap = cap;
+    m->last_tb = tb_on;
+    s->packed_tb_matrices = (uint4 *)m->tb;
+    s->packed_tb_matrix_size = m->tb_bytes / 16;
+
+    memset(&d, 0, sizeof(d));
+    d.uq = s->unpacked_query_batch; d.ut = s->unpacked_target_batch;
+    d.pq = s->packed_query_batch; d.pt = s->packed_target_batch;
+    d.qoff = s->query_batch_offsets; d.toff = s->target_batch_offsets;
+    d.qlen = s->query_batch_lens; d.tlen = s->target_batch_lens;
+    d.qop = s->query_op; d.top = s->target_op;
+    d.seeds = s->seed_scores;
+    d.q_bytes = qb; d.t_bytes = tb; d.n = n;
+    d.max_qw = qw; d.max_tw = tw;
+    int32_t *scr = m->scratch_i32;
+    auto pick = [&](int32_t *p, int idx) { return p ? p : scr + (size_t)idx * n; };
+    d.score = pick(s->device_cpy->aln_score, 0);
+    d.qend = pick(s->device_cpy->query_batch_end, 1);
+    d.tend = pick(s->device_cpy->target_batch_end, 2);
+    d.qstart = pick(s->device_cpy->query_batch_start, 3);
+    d.tstart = pick(s->device_cpy->target_batch_start, 4);
+    if (params->secondBest && s->device_cpy_second) {
+        d.s_score = pick(s->device_cpy_second->aln_score, 5);
+        d.s_qend = pick(s->device_cpy_second->query_batch_end, 6);
+        d.s_tend = pick(s->device_cpy_second->target_batch_end, 7);
+    } else {
+        d.s_score = scr + (size_t)5 * n; d.s_qend = scr + (size_t)6 * n; d.s_tend = scr + (size_t)7 * n;
+    }
+    d.n_ops = m->nops;
+    d.cigar = m->cigar;
+    d.cigar_cap = cap;
+    d.cigar_qlayout = tb_on ? m->cigar_q : nullptr;
+    d.keys = m->keys; d.keys_alt = m->keys_alt; d.perm = m->perm; d.perm_alt = m->perm_alt;
+    d.sort_tmp = m->sort_tmp; d.sort_tmp_bytes = m->sort_tmp_bytes;
+    d.bnd = m->bnd; d.bnd_cols = bnd_cols;
+    d.tb = m->tb; d.tb_tw = tw; d.tb_qw8 = 8 * qw;
+    d.eh = m->eh; d.eh_cols = eh_cols;
+    d.aux = m->aux;
+}
+
+G2Scores scores_now() { return g_scores; }
+
+void upload_async(gasal_gpu_storage_t *s, uint32_t n, Parameters *params) {
+    for (host_batch_t *cur = s->extensible_host_unpacked_query_batch; cur != NULL; cur = cur->next)
+        if (cur->data_size) G2_CHECK(hipMemcpyAsync(&(s->unpacked_query_batch[cur->offset]), cur->data, cur->data_size, hipMemcpyHostToDevice, s->str));
+    for (host_batch_t *cur = s->extensible_host_unpacked_target_batch; cur != NULL; cur = cur->next)
+        if (cur->data_size) G2_CHECK(hipMemcpyAsync(&(s->unpacked_target_batch[cur->offset]), cur->data, cur->data_size, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(s->query_batch_lens, s->host_query_batch_lens, (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(s->target_batch_lens, s->host_target_batch_lens, (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(s->query_batch_offsets, s->host_query_batch_offsets, (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(s->target_batch_offsets, s->host_target_batch_offsets, (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    if (params->algo == KSW) {
+        if (s->seed_scores == NULL) fprintf(stderr, "seed_scores == NULL\n");
+        if (s->host_seed_scores == NULL) fprintf(stderr, "host_seed_scores == NULL\n");
+        if (s->seed_scores == NULL || s->host_seed_scores == NULL) { report("[GASAL ERROR:] KSW seed scores missing\n"); gasal2_fail(EXIT_FAILURE); return; }
+        G2_CHECK(hipMemcpyAsync(s->seed_scores, s->host_seed_scores, (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    }
+    if (params->isReverseComplement) {
+        G2_CHECK(hipMemcpyAsync(s->query_op, s->host_query_op, n, hipMemcpyHostToDevice, s->str));
+        G2_CHECK(hipMemcpyAsync(s->target_op, s->host_target_op, n, hipMemcpyHostToDevice, s->str));
+    }
+}
+
+void run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    G2Dev d;
+    prepare(s, qb, tb, n, params, d);
+    Impl *m = impl_of(s);
+    G2Scores sc = scores_now();
+    hipError_t e = g2_run(sc, opts_of(params), d, s->str, &m->tm);
+    if (e != hipSuccess) {
+        report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
+        gasal2_fail(EXIT_FAILURE);
+    }
+}
+
+void download_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t n, Parameters *params) {
+    gasal_res_t *h = s->host_res, *dc = s->device_cpy;
+    if (h->aln_score && dc->aln_score) G2_CHECK(hipMemcpyAsync(h->aln_score, dc->aln_score, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    if (h->query_batch_start && dc->query_batch_start) G2_CHECK(hipMemcpyAsync(h->query_batch_start, dc->query_batch_start, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    if (h->target_batch_start && dc->target_batch_start) G2_CHECK(hipMemcpyAsync(h->target_batch_start, dc->target_batch_start, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    if (h->query_batch_end && dc->query_batch_end) G2_CHECK(hipMemcpyAsync(h->query_batch_end, dc->query_batch_end, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    if (h->target_batch_end && dc->target_batch_end) G2_CHECK(hipMemcpyAsync(h->target_batch_end, dc->target_batch_end, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    if (params->start_pos == WITH_TB) {
+        Impl *m = impl_of(s);
+        if (h->cigar && m->cigar_q) G2_CHECK(hipMemcpyAsync(h->cigar, m->cigar_q, qb, hipMemcpyDeviceToHost, s->str));
+        if (h->n_cigar_ops && m->nops) G2_CHECK(hipMemcpyAsync(h->n_cigar_ops, m->nops, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    }
+    if (params->secondBest && s->host_res_second && s->device_cpy_second) {
+        gasal_res_t *h2 = s->host_res_second, *d2 = s->device_cpy_second;
+        if (h2->aln_score && d2->aln_score) G2_CHECK(hipMemcpyAsync(h2->aln_score, d2->aln_score, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+        if (h2->query_batch_end && d2->query_batch_end) G2_CHECK(hipMemcpyAsync(h2->query_batch_end, d2->query_batch_end, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+        if (h2->target_batch_end && d2->target_batch_end) G2_CHECK(hipMemcpyAsync(h2->target_batch_end, d2->target_batch_end, (size_t)n * 4, hipMemcpyDeviceToHost, s->str));
+    }
+}
+
+}  // namespace
+
+// src/gasal_align.cu:29-307
+extern "C" void gasal_aln_async(gasal_gpu_storage_t *gpu_storage, const uint32_t actual_query_batch_bytes,
+                                const uint32_t actual_target_batch_bytes, const uint32_t actual_n_alns,
+                                Parameters *params) {
+    if (!validate(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns)) return;
+    grow_device(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params);
+    upload_async(gpu_storage, actual_n_alns, params);
+    run_async(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params);
+    download_async(gpu_storage, actual_query_batch_bytes, actual_n_alns, params);
+    gpu_storage->is_free = 0;
+}
+
+// src/gasal_align.cu:310-326
+extern "C" int gasal_is_aln_async_done(gasal_gpu_storage_t *gpu_storage) {
+    if (gpu_storage->is_free == 1) return -2;
+    hipError_t err = hipStreamQuery(gpu_storage->str);
+    if (err != hipSuccess) {
+        if (err == hipErrorNotReady) return -1;
+        report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(err), (int)err, __LINE__, __FILE__);
+        gasal2_fail(EXIT_FAILURE);
+        return -3;
+    }
+    gasal_host_batch_reset(gpu_storage);
+    gpu_storage->is_free = 1;
+    gpu_storage->current_n_alns = 0;
+    return 0;
+}
+
+extern "C" void gasal2_upload(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    if (!validate(s, qb, tb, n)) return;
+    grow_device(s, qb, tb, n, params);
+    upload_async(s, n, params);
+    G2_CHECK(hipStreamSynchronize(s->str));
+}
+
+extern "C" void gasal2_run_resident(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    if (!validate(s, qb, tb, n)) return;
+    run_async(s, qb, tb, n, params);
+}
+
+extern "C" void gasal2_download(gasal_gpu_storage_t *s, uint32_t qb, uint32_t n, Parameters *params) {
+    download_async(s, qb, n, params);
+    G2_CHECK(hipStreamSynchronize(s->str));
+}
+
+extern "C" void gasal2_enable_timing(gasal_gpu_storage_t *s, int32_t on) {
+    Impl *m = impl_of(s);
+    if (on && !m->tm_init) {
+        for (int i = 0; i < 6; i++) G2_CHECK(hipEventCreate(&m->tm.ev[i]));
+        m->tm_init = true;
+    }
+    m->tm.enabled = on && m->tm_init;
+}
+
+extern "C" float gasal2_stage_ms(gasal_gpu_storage_t *s, int32_t stage) {
+    Impl *m = impl_of(s);
+    if (!m->tm_init || !m->tm.enabled) return -1.0f;
+    static const int a[5] = {0, 0, 1, 2, 3}, b[5] = {5, 1, 2, 3, 4};
+    if (stage < 0 || stage > 4) return -1.0f;
+    float ms = -1.0f;
+    if (hipEventSynchronize(m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
+    if (hipEventElapsedTime(&ms, m->tm.ev[a[stage]], m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+extern "C" uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint32_t cap) {
+    Impl *m = impl_of(s);
+    if (!m->last_tb || k >= m->last_n || !m->nops) return 0;
+    uint32_t nops = 0;
+    G2_CHECK(hipMemcpy(&nops, m->nops + k, 4, hipMemcpyDeviceToHost));
+    uint32_t nb = std::min(nops, std::min(cap, m->last_cigar_cap));
+    if (nb && out) G2_CHECK(hipMemcpy(out, m->cigar + (size_t)k * m->last_cigar_cap, nb, hipMemcpyDeviceToHost));
+    return nops;
+}

@@ -1,0 +1,95 @@
+/*
+ * gasal2_ext.h -- C-ABI additions of the MI355X library.
+ *
+ * The reference API (gasal_align.h, ctors.h, host_batch.h, interfaces.h) takes
+ * a C++ `Parameters*`; these helpers let a C / ctypes / cgo / JNI host build one
+ * and reach the library's run-time switches.  Each entry states the reference
+ * interface it extends or that it has no counterpart.
+ */
+#ifndef GASAL2_EXT_H
+#define GASAL2_EXT_H
+
+#include "gasal.h"
+
+#ifdef __cplusplus
+class Parameters;
+extern "C" {
+#else
+typedef struct Parameters Parameters;
+#endif
+
+/* Plain-data view of the algorithm fields of class Parameters
+ * (src/args_parser.h:30-48); field meanings and enum values are the reference's. */
+typedef struct {
+    int32_t sa, sb, gapo, gape;
+    int32_t start_pos;               /* comp_start  */
+    int32_t print_out, n_threads;
+    int32_t k_band;
+    int32_t secondBest;              /* Bool        */
+    int32_t isPacked;
+    int32_t isReverseComplement;
+    int32_t semiglobal_skipping_head;/* data_source */
+    int32_t semiglobal_skipping_tail;/* data_source */
+    int32_t algo;                    /* algo_type   */
+} gasal2_params_t;
+
+/* Parameters(int argc, char **argv) with argc = 0 (src/args_parser.cpp:8-39):
+ * reference defaults.  No reference counterpart as a C function. */
+Parameters *gasal2_parameters_new(void);
+void gasal2_parameters_free(Parameters *p);
+void gasal2_parameters_get(const Parameters *p, gasal2_params_t *out);
+void gasal2_parameters_set(Parameters *p, const gasal2_params_t *in);
+
+/* Run-time values of the reference's compile-time N_CODE and N_PENALTY
+ * (Makefile:1-4, src/gasal_kernels.h:37-56).  has_penalty = 0 means
+ * "N_PENALTY undefined".  Process-wide, like gasal_copy_subst_scores. */
+void gasal2_set_n_code(int32_t n_code);
+int32_t gasal2_get_n_code(void);
+void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty);
+
+/* Error path (src/gasal.h:15-22; src/gasal_align.cu:32-67 exit on misuse).
+ * mode 0 (default): print and exit(EXIT_FAILURE) like the reference.
+ * mode 1: print, record the message and return from the failing API call
+ * without side effects, so a test harness can observe it. */
+void gasal2_set_error_mode(int32_t mode);
+int32_t gasal2_error_count(void);
+const char *gasal2_last_error(void);
+
+/* Split form of gasal_aln_async (src/gasal_align.cu:29-307) for callers whose
+ * inputs are already resident in device memory (and for measuring the device
+ * path alone):
+ *   gasal2_upload   : the H2D half (src/gasal_align.cu:152-175,207-235), blocking;
+ *   gasal2_run_resident : pack, reverse/complement, DP, traceback, all on
+ *                     gpu_storage->str, results left in device memory;
+ *   gasal2_download : the D2H half (src/gasal_align.cu:266-304), blocking.
+ * gasal_aln_async == upload + run_resident + download, asynchronously. */
+void gasal2_upload(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t target_batch_bytes, uint32_t n_alns,
+                   Parameters *params);
+void gasal2_run_resident(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t target_batch_bytes,
+                         uint32_t n_alns, Parameters *params);
+void gasal2_download(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t n_alns, Parameters *params);
+
+/* Device time in milliseconds of the last run_resident's stages, measured with
+ * HIP events on gpu_storage->str: stage 0 = whole run, 1 = pack, 2 =
+ * reverse/complement, 3 = DP kernel(s), 4 = traceback.  Events are recorded only
+ * after gasal2_enable_timing(s, 1).  No reference counterpart. */
+void gasal2_enable_timing(gasal_gpu_storage_t *s, int32_t on);
+float gasal2_stage_ms(gasal_gpu_storage_t *s, int32_t stage);
+
+/* Full CIGAR of pair k of the last finished WITH_TB batch (reverse order, one
+ * byte per run: op | count << 2, op 0 M, 1 X, 2 D, 3 I; reference format
+ * README.md:203-211).  host_res->cigar holds the same bytes at
+ * host_query_batch_offsets[k], clipped to the pair's padded query length, as in
+ * the reference; this accessor has no such limit.  Returns n_ops, copies at most
+ * cap bytes. */
+uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint32_t cap);
+
+/* Kernel dispatch policy for LOCAL (no reference counterpart; results are
+ * identical under every policy):
+ *   0 auto, 1 one lane per pair, 2 one wavefront of lanes per pair. */
+void gasal2_set_dispatch(int32_t policy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
