@@ -1,0 +1,176 @@
+"""Benchmark of the batched alignment hot path on one MI355X.
+
+Workload (default): the full-feature batch -- 1,000,000 pairs, query length
+uniform 100-300 bp, target = query + 0-100 bp random flanks at 5% divergence,
+0.5% N bases with N_PENALTY 1, reverse-complemented queries on half the pairs --
+aligned LOCAL WITH_TB with second best (score, ends, starts, second best, CIGAR),
+int32 scores, 4-bit packed on the device.  One step = one pass of the device
+pipeline (pack, reverse/complement, shape sort, DP, traceback) over the whole
+batch with the inputs already resident in HBM.
+
+Prints one JSON line.  `value` is GCUPS = sum over pairs of Lq*Lt (unpadded
+cells) / step time.  `roofline` prices the DP kernel against the integer-VALU
+peak (N_CU * 64 lanes * clock) at 16 algorithmic int ops per cell; `traffic`
+comes from rocprofv3 counter runs when given.  `cpu_baseline` times the CPU
+oracle (an OpenMP restatement of the reference kernels) on a bounded prefix of
+the same batch on this host and reports the GPU's agreement with it there.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+OPS_TB = 16      # algorithmic int ops per cell, local WITH_TB
+OPS_SCORE = 11   # score-only
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--mode", default="local_tb_second",
+                    choices=["local_tb_second", "local_score", "global", "semi_global"])
+    ap.add_argument("--cpu-pairs", type=int, default=20000, help="oracle sample for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per DP launch")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); pairs are independent and would be "
+              "sharded across GPUs with no exchange -- see DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+
+    import gasal2_amd as g
+
+    t0 = time.time()
+    batch = g.synth_batch(2, 2, args.pairs)
+    cells = int(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+    log(f"generated {batch.n} pairs, {cells / 1e9:.2f} G cells in {time.time() - t0:.1f}s")
+
+    mode = args.mode
+    if mode == "local_tb_second":
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True)
+        ops = OPS_TB
+    elif mode == "local_score":
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START, isReverseComplement=True)
+        ops = OPS_SCORE
+    elif mode == "global":
+        p = g.Parameters(algo=g.GLOBAL, start_pos=g.WITH_START, isReverseComplement=True)
+        ops = OPS_SCORE
+    else:
+        p = g.Parameters(algo=g.SEMI_GLOBAL, start_pos=g.WITH_START, isReverseComplement=True,
+                         semiglobal_skipping_head=g.TARGET, semiglobal_skipping_tail=g.TARGET)
+        ops = OPS_SCORE
+
+    dev = g.device_info()
+    peak_tops = dev["n_cu"] * 64 * dev["clock_mhz"] * 1e6 / 1e12
+    a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=(1, 4, 6, 1), n_penalty=1)
+    t0 = time.time()
+    a.load(batch)
+    log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
+
+    for _ in range(args.warmup):
+        a.run_resident(timing=True)
+        a.sync()
+    dp_ms, tb_ms, total_ms = [], [], []
+    a.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a.run_resident(timing=True)
+        a.sync()
+        dp_ms.append(a.stage_ms(3))
+        tb_ms.append(a.stage_ms(4))
+        total_ms.append(a.stage_ms(0))
+    a.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    gcups = cells / (wall / args.steps) / 1e9
+    dp_avg = float(np.mean(dp_ms))
+    achieved_tops = cells * ops / (dp_avg * 1e-3) / 1e12
+    log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); DP kernel {dp_avg:.2f} ms; traceback {np.mean(tb_ms):.2f} ms")
+
+    # PCIe-inclusive rate of one batch through gasal_aln_async (host pages -> results on host)
+    t0 = time.perf_counter()
+    res = a.run(batch, per_pair=False)
+    pcie_s = time.perf_counter() - t0
+    log(f"gasal_aln_async incl. H2D/D2H: {pcie_s * 1e3:.1f} ms ({cells / pcie_s / 1e9:.1f} GCUPS)")
+
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if args.cpu_pairs > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        m = min(args.cpu_pairs, batch.n)
+        sub = batch.subset(np.arange(m))
+        sub_cells = int(np.dot(sub.qlen.astype(np.int64), sub.tlen.astype(np.int64)))
+        t0 = time.perf_counter()
+        ref = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
+                      head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, is_rc=True,
+                      n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
+        cpu_s = time.perf_counter() - t0
+        keys = [k for k in ("score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend", "s_tend")
+                if res.get(k) is not None]
+        if mode != "local_tb_second":
+            keys = [k for k in keys if not k.startswith("s_")]
+        if mode == "global":
+            keys = ["score"]
+        if mode == "local_score":
+            keys = [k for k in keys if k in ("score", "qend", "tend")]
+        ok = np.ones(m, bool)
+        for k in keys:
+            ok &= res[k][:m] == ref[k]
+        if p.start_pos == g.WITH_TB:
+            ok &= res["n_ops"][:m] == ref["n_ops"]
+        cpu = {"value": round(sub_cells / cpu_s / 1e9, 4), "unit": "GCUPS", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"first {m} pairs of the same batch, same mode, oracle OpenMP schedule(dynamic) "
+                         f"({cpu_s:.1f}s)",
+               "gpu_agreement_on_sample": float(ok.mean())}
+        log(f"cpu oracle: {cpu}")
+
+    line = {
+        "metric": "GCUPS + % int-VALU roofline",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
+                               "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries",
+                   "mode": mode, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
+                   "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
+        "roofline": {"bound": "valu", "kernel": "DP (one lane per pair)", "ops_per_cell": ops,
+                     "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3), "unit": "TOP/s",
+                     "frac": round(achieved_tops / peak_tops, 4), "traffic": traffic,
+                     "dp_kernel_ms": round(dp_avg, 3)},
+        "cpu_baseline": cpu,
+        "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 3),
+    }
+    print(json.dumps(line), flush=True)
+    a.close()
+
+
+if __name__ == "__main__":
+    main()

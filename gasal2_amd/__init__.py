@@ -162,10 +162,12 @@ _SIGS = {
     "gasal2_upload": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "gasal2_run_resident": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
     "gasal2_download": (None, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal2_synchronize": (None, [C.POINTER(gasal_gpu_storage_t)]),
     "gasal2_enable_timing": (None, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
     "gasal2_stage_ms": (C.c_float, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
     "gasal2_get_cigar": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32]),
     "gasal2_set_dispatch": (None, [C.c_int32]),
+    "gasal2_device_info": (C.c_int32, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
 }
 
 
@@ -182,6 +184,14 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def device_info():
+    n_cu, clk = C.c_int32(0), C.c_int32(0)
+    name = C.create_string_buffer(256)
+    if lib().gasal2_device_info(C.byref(n_cu), C.byref(clk), name, 256) != 0:
+        raise RuntimeError("no HIP device")
+    return {"n_cu": n_cu.value, "clock_mhz": clk.value / 1000.0, "name": name.value.decode(errors="replace")}
 
 
 def exported_symbols():
@@ -459,6 +469,36 @@ class Aligner:
             if r == -2:
                 raise RuntimeError("nothing launched")
             time.sleep(0.0005)
+        return self._collect(n, qb)
+
+    # ---- resident-input form (gasal2_upload / gasal2_run_resident / gasal2_download)
+    def load(self, batch, per_pair=None):
+        """Fills the host batch and copies it to device memory once."""
+        s, p, L = self.s, self.params, self._L
+        n = batch.n
+        qb, tb = self._fill(batch, per_pair if per_pair is not None else n <= 2000)
+        s.current_n_alns = n
+        if p.isReverseComplement:
+            C.memmove(s.host_query_op, batch.qop.ctypes.data, n)
+            C.memmove(s.host_target_op, batch.top.ctypes.data, n)
+        if p.algo == algo_type.KSW:
+            C.memmove(s.host_seed_scores, batch.seeds.ctypes.data, 4 * n)
+        L.gasal2_upload(C.byref(s), qb, tb, n, p.native())
+        self._loaded = (qb, tb, n)
+        return self._loaded
+
+    def run_resident(self, timing=False):
+        qb, tb, n = self._loaded
+        if timing:
+            self._L.gasal2_enable_timing(C.byref(self.s), 1)
+        self._L.gasal2_run_resident(C.byref(self.s), qb, tb, n, self.params.native())
+
+    def sync(self):
+        self._L.gasal2_synchronize(C.byref(self.s))
+
+    def download(self):
+        qb, tb, n = self._loaded
+        self._L.gasal2_download(C.byref(self.s), qb, n, self.params.native())
         return self._collect(n, qb)
 
     def _collect(self, n, qb):

@@ -112,6 +112,20 @@ extern "C" void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty) {
 }
 extern "C" void gasal2_set_dispatch(int32_t policy) { g2_set_dispatch(policy); }
 
+extern "C" int32_t gasal2_device_info(int32_t *n_cu, int32_t *clock_khz, char *name, int32_t name_cap) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    if (name && name_cap > 0) {
+        strncpy(name, prop.name, (size_t)name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------------ Parameters ---
 // src/args_parser.cpp:8-39 (defaults), :41-44, :46-52, :54-75, :77-95, :97-240, :242-250
 Parameters::Parameters(int argc_, char **argv_) {
@@ -1028,6 +1042,8 @@ extern "C" void gasal2_download(gasal_gpu_storage_t *s, uint32_t qb, uint32_t n,
     download_async(s, qb, n, params);
     G2_CHECK(hipStreamSynchronize(s->str));
 }
+
+extern "C" void gasal2_synchronize(gasal_gpu_storage_t *s) { G2_CHECK(hipStreamSynchronize(s->str)); }
 
 extern "C" void gasal2_enable_timing(gasal_gpu_storage_t *s, int32_t on) {
     Impl *m = impl_of(s);

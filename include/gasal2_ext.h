@@ -68,6 +68,8 @@ void gasal2_upload(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t 
 void gasal2_run_resident(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t target_batch_bytes,
                          uint32_t n_alns, Parameters *params);
 void gasal2_download(gasal_gpu_storage_t *s, uint32_t query_batch_bytes, uint32_t n_alns, Parameters *params);
+/* Blocks until everything enqueued on gpu_storage->str has finished. */
+void gasal2_synchronize(gasal_gpu_storage_t *s);
 
 /* Device time in milliseconds of the last run_resident's stages, measured with
  * HIP events on gpu_storage->str: stage 0 = whole run, 1 = pack, 2 =
@@ -88,6 +90,12 @@ uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint
  * identical under every policy):
  *   0 auto, 1 one lane per pair, 2 one wavefront of lanes per pair. */
 void gasal2_set_dispatch(int32_t policy);
+
+/* Properties of the current device used to state the integer-VALU peak:
+ * compute units, peak engine clock (kHz) and the name.  Returns 0 on success.
+ * No reference counterpart (gasal_set_device prints the name only,
+ * src/interfaces.cpp:98-128). */
+int32_t gasal2_device_info(int32_t *n_cu, int32_t *clock_khz, char *name, int32_t name_cap);
 
 #ifdef __cplusplus
 }

