@@ -50,6 +50,11 @@ struct G2Dev {
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
     uint32_t *aux;                   // small device counters (work queue etc.)
+    // LOCAL fast path (see k_local_fast): 0 = off, 1 = launched beside the
+    // generic kernel, the nibble census in aux[0] picks which one runs.
+    int32_t fast;
+    int32_t enc_s, enc_cb;           // position encoding: h << enc_s | ~strip << (3+enc_cb) | ~col << 3 | 7-row
+    uint32_t enc_cm, enc_sm;         // column / strip field masks
 };
 
 struct G2Timing {
@@ -64,15 +69,16 @@ size_t g2_sort_tmp_bytes(uint32_t n);
 // first launch error.
 hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t stream, G2Timing *tm);
 
-// Dispatch policy for LOCAL (0 auto, 1 lane per pair, 2 wavefront per pair).
+// Dispatch policy for LOCAL (0 auto, 1 reference-order kernel only).
 void g2_set_dispatch(int policy);
+int g2_get_dispatch();
 
 static inline uint32_t g2_regs(uint32_t len) { return (len >> 3) + ((len & 7) ? 1u : 0u); }
 
 // CIGAR slot size that always holds a pair's CIGAR (see get_tb walk bounds).
 static inline uint32_t g2_cigar_capacity(uint32_t max_qlen, uint32_t max_tlen) {
     uint32_t q8 = 8 * g2_regs(max_qlen), t8 = 8 * g2_regs(max_tlen);
-    return q8 + t8 + (q8 + t8) / 32 + 16;
+    return (q8 + t8 + (q8 + t8) / 32 + 16 + 15) & ~15u;   // 16-byte aligned slots
 }
 
 #endif

@@ -51,6 +51,18 @@ struct LaneCtx {
     uint32_t *bd;     // boundary row: element c at bd[c * WAVE]
 };
 
+// Direction words of a wave slot are laid out [strip][query word][lane][8
+// columns]: a lane writes one 32-byte chunk per 8x8 tile (two 16-byte stores,
+// contiguous across the wave) and a traceback walk stays inside one chunk for
+// eight columns.
+__device__ __forceinline__ uint32_t *tb_chunk(const G2Dev &d, uint32_t grp, uint32_t lane, int32_t strip, int32_t word) {
+    return d.tb + ((((size_t)grp * d.tb_tw + (size_t)strip) * (d.tb_qw8 >> 3) + (size_t)word) * WAVE + lane) * 8;
+}
+__device__ __forceinline__ void tb_store(uint32_t *p, const uint32_t *w) {
+    reinterpret_cast<uint4 *>(p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    reinterpret_cast<uint4 *>(p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
 __device__ __forceinline__ bool lane_setup(const G2Dev &d, LaneCtx &x) {
     x.lane = threadIdx.x & (WAVE - 1);
     x.grp = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
@@ -140,13 +152,40 @@ __device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
         for (uint32_t i = 0; i < regs; i++) w[i] = nib_complement(w[i]);
 }
 
+// 16 threads per pair.  When the last word holds no nibble equal to N_CODE
+// (always so for a byte-sized N_CODE such as 'N'), the reference's reverse is a
+// plain reversal of the padded word sequence: thread t swaps words (i,
+// regs-1-i) for i = t, t+16, ...; otherwise the first thread runs the
+// reference's sequential loop.
+__device__ __forceinline__ void rc_side(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code, uint32_t t) {
+    const uint32_t regs = (len >> 3) + ((len & 7) ? 1 : 0);
+    if (regs == 0 || op == 0) return;
+    bool plain = true;
+    if (op & 1u) {
+        const uint32_t last = w[regs - 1];
+        for (int jj = 0; jj < 32; jj += 4) plain &= (((last >> jj) & 15u) != (uint32_t)n_code);
+    }
+    if (!plain) {
+        if (t == 0) rc_one(w, len, op, n_code);
+        return;
+    }
+    const uint32_t to_swap = (regs >> 1) + (regs & 1);
+    for (uint32_t i = t; i < to_swap; i += 16) {
+        const uint32_t a = w[i], b = w[regs - 1 - i];
+        uint32_t na = (op & 1u) ? nib_reverse(b) : a;
+        uint32_t nb = (op & 1u) ? nib_reverse(a) : b;
+        if (op & 2u) { na = nib_complement(na); nb = nib_complement(nb); }
+        w[i] = na;
+        if (regs - 1 - i != i) w[regs - 1 - i] = nb;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, t = threadIdx.x & 15u;
     if (k >= d.n) return;
-    uint32_t qo = d.qop[k], to = d.top[k];
-    if (qo == 0 && to == 0) return;
-    if (qo) rc_one(d.pq + (d.qoff[k] >> 3), d.qlen[k], qo, n_code);
-    if (to) rc_one(d.pt + (d.toff[k] >> 3), d.tlen[k], to, n_code);
+    const uint32_t qo = d.qop[k], to = d.top[k];
+    if (qo) rc_side(d.pq + (d.qoff[k] >> 3), d.qlen[k], qo, n_code, t);
+    if (to) rc_side(d.pt + (d.toff[k] >> 3), d.tlen[k], to, n_code, t);
 }
 
 // ------------------------------------------------------------------ sort ----
@@ -162,8 +201,11 @@ __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
 
 // ---------------------------------------------------------------- LOCAL -----
 // gasal_local_kernel<LOCAL,S,B> (src/kernels/local_kernel_template.h:71-519).
+__device__ __forceinline__ bool census_fast(uint32_t m);
+
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
+    if (d.fast && census_fast(d.aux[0])) return;   // k_local_fast handles this batch
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     constexpr bool TB = START == S_WITH_TB;
@@ -171,8 +213,6 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
     const int32_t gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     uint32_t *__restrict__ bd = x.bd;
-    uint32_t *__restrict__ tbp = nullptr;
-    if (TB) tbp = d.tb + (size_t)x.grp * d.tb_tw * d.tb_qw8 * WAVE + x.lane;
 
     for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;   // initHD (:118-120)
 
@@ -199,6 +239,7 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
 #pragma unroll
                 for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
             }
+            uint32_t dirw[8];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t ridx = j * 8 + kk;
@@ -237,8 +278,9 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
                     prev_sec = max(sec, prev_sec);
                 }
                 prev_maxHH = max(maxHH, prev_maxHH);
-                if (TB) tbp[((size_t)i * d.tb_qw8 + ridx) * WAVE] = dir;
+                if (TB) dirw[kk] = dir;
             }
+            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -299,6 +341,215 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
     }
 }
 
+// ------------------------------------------------------- LOCAL, fast path ----
+// Same recurrence, order and outputs as k_local, with three changes that cut
+// the integer-VALU work per cell:
+//  * substitution: per 8-row strip each row gets a 32-bit table of eight
+//    signed 4-bit scores indexed by the query base's class (nibble & 7), so a
+//    cell's score is one v_bfe_i32 (valid when every base nibble present in the
+//    batch has a distinct class and all scores fit in 4 bits; the nibble census
+//    in aux[0] and the host check decide, otherwise k_local runs);
+//  * max / second best: cells are ranked by h << S | ~strip | ~column | 7-row,
+//    so "first maximal cell in (strip, column, row) order" becomes a plain
+//    integer max (no per-cell position selects);
+//  * direction bits (WITH_TB): the four comparisons of the reference's nibble
+//    are taken as sign bits of differences and shifted into the column word
+//    with v_alignbit (no SGPR masks); bit order per row from the top:
+//    h!=M, h!=F_in, F restarted, E restarted.  k_get_tb decodes them.
+__device__ __forceinline__ bool census_fast(uint32_t m) { return ((m & (m >> 8)) & 0xFFu) == 0u; }
+
+__device__ __forceinline__ uint32_t score_lut(const G2Scores &c, int32_t nval, uint32_t census, uint32_t tnib) {
+    uint32_t lut = 0;
+#pragma unroll
+    for (uint32_t cls = 0; cls < 8; cls++) {
+        const uint32_t q = ((census >> cls) & 1u) ? cls : cls + 8u;   // the nibble this class stands for
+        int32_t s = (q == tnib) ? c.match : -c.mismatch;
+        s = (q == c.n_value || tnib == c.n_value) ? nval : s;
+        lut |= ((uint32_t)s & 15u) << (4 * cls);
+    }
+    return lut;
+}
+
+__global__ __launch_bounds__(256) void k_census(const uint32_t *__restrict__ pq, uint32_t nq, const uint32_t *__restrict__ pt,
+                                               uint32_t nt, uint32_t *__restrict__ out) {
+    uint32_t m = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq + nt; i += stride) {
+        uint32_t w = i < nq ? pq[i] : pt[i - nq];
+#pragma unroll
+        for (int k = 0; k < 32; k += 4) m |= 1u << ((w >> k) & 15u);
+    }
+    // wave reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
+    if ((threadIdx.x & 63) == 0 && m) atomicOr(out, m);
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
+}
+
+template <int START, bool SECOND>
+__global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
+    const uint32_t census = d.aux[0];
+    if (!census_fast(census)) return;
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    constexpr bool TB = START == S_WITH_TB;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    const int32_t S = d.enc_s;
+    uint32_t *__restrict__ bd = x.bd;
+
+    for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;
+
+    int32_t best = 0;       // encoded first maximal cell
+    int32_t sbest = 0;      // encoded first maximal second-best candidate
+    int32_t pm = 0;         // running max of h in processing order (the reference's maxHH)
+    for (int32_t i = 0; i < TW; i++) {
+        int32_t h[9], f[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; }
+        const uint32_t gpac = x.T[i];
+        uint32_t lut[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
+        // column tag: ~strip and ~column fields, pre-shifted by 3
+        int32_t tagc = (int32_t)((((~(uint32_t)i) & d.enc_sm) << d.enc_cb | d.enc_cm) << 3);
+        uint32_t nxt[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
+        for (int32_t j = 0; j < QW; j++) {
+            const uint32_t rpac = x.Q[j];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
+            if (j + 1 < QW) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+            }
+            uint32_t dirw[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t ridx = j * 8 + kk;
+                // 4 * (query nibble & 7): the nibble of column kk sits at bits 31-4kk .. 28-4kk
+                const uint32_t qsh = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                int32_t e = (int32_t)cur[kk] >> 16;
+                // p walks down the previous column: H(row-1, col-1) for each row
+                int32_t p = h[0];
+                h[0] = sx16(cur[kk]);
+                uint32_t dir = 0;
+                int32_t colbest = 0, colsec = 0;
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t s = __builtin_amdgcn_sbfe(lut[m], qsh, 4);
+                    const int32_t tmp = p + s;
+                    const int32_t hm = max(max(max(tmp, f[m]), e), 0);
+                    const int32_t t = tmp - gapoe;
+                    const int32_t fx = f[m] - gape, ex = e - gape;
+                    if (TB) {
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+                    }
+                    f[m] = max(t, fx);
+                    e = max(t, ex);
+                    const int32_t en = (hm << S) | (8 - m);
+                    colbest = m == 1 ? en : max(colbest, en);
+                    if (SECOND) {
+                        const int32_t cand = hm < pm ? en : 0;
+                        colsec = m == 1 ? cand : max(colsec, cand);
+                        pm = max(pm, hm);
+                    }
+                    p = h[m];
+                    h[m] = hm;
+                }
+                bd[ridx * WAVE] = pk16(h[8], e);
+                best = max(best, colbest | tagc);
+                if (SECOND) sbest = max(sbest, colsec | tagc);
+                tagc -= 8;
+                if (TB) dirw[kk] = dir;
+            }
+            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
+        }
+    }
+    const uint32_t pr = x.pair;
+    const int32_t vbest = best >> S;
+    const uint32_t cm = d.enc_cm, sm = d.enc_sm;
+    auto decode = [&](int32_t en, int32_t &row, int32_t &col) {
+        const uint32_t u = (uint32_t)en;
+        const uint32_t strip = (~(u >> (3 + d.enc_cb))) & sm;
+        col = (int32_t)((~(u >> 3)) & cm);
+        row = (int32_t)(strip * 8 + (7 - (u & 7u)));
+    };
+    int32_t maxXY_x = 0, maxXY_y = 0;
+    if (vbest > 0) decode(best, maxXY_y, maxXY_x);
+    d.score[pr] = vbest;
+    d.qend[pr] = maxXY_x;
+    d.tend[pr] = maxXY_y;
+    if (SECOND) {
+        const int32_t vsec = sbest >> S;
+        int32_t sy = 0, sx = 0;
+        if (vsec > 0) decode(sbest, sy, sx);
+        d.s_score[pr] = vsec;
+        d.s_tend[pr] = sy;
+        d.s_qend[pr] = vbest > 0 ? 8 * QW - 1 : 0;   // compares with the best (src/kernels/local_kernel_template.h:162)
+    }
+    if (START == S_WITH_START) {                                                           // :441-511
+        int32_t maxHH = 0, prev_maxHH = 0;
+        const int32_t fwd_score = vbest;
+        const int32_t rend_reg = ((maxXY_x >> 3) + 1) < QW ? ((maxXY_x >> 3) + 1) : QW;
+        const int32_t gend_reg = ((maxXY_y >> 3) + 1) < TW ? ((maxXY_y >> 3) + 1) : TW;
+        const uint32_t *Qe = x.Q + (rend_reg - 1), *Te = x.T + (gend_reg - 1);
+        maxXY_x = 0; maxXY_y = 0;
+        for (int32_t cc = 0; cc < 8 * rend_reg; cc++) bd[cc * WAVE] = 0u;
+        int32_t gidx = ((gend_reg << 3) + 8) - 1;
+        bool go = maxHH < fwd_score;
+        for (int32_t i = 0; i < gend_reg && go; i++) {
+            int32_t h[9], f[9], p[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
+            const uint32_t gpac = *(Te - i);
+            uint32_t gb[9];
+#pragma unroll
+            for (int m = 1; m < 9; m++) gb[m] = (gpac >> (4 * (m - 1))) & 15u;
+            gidx -= 8;
+            int32_t ridx = (rend_reg << 3) - 1;
+            int32_t gi = 0;
+            for (int32_t j = 0; j < rend_reg && go; j++) {
+                const uint32_t rpac = *(Qe - j);
+                for (int kk = 0; kk < 8 && go; kk++) {
+                    const uint32_t rbase = (rpac >> (4 * kk)) & 15u;
+                    uint32_t hd = bd[gi * WAVE];
+                    h[0] = sx16(hd);
+                    int32_t e = sx16(hd >> 16);
+#pragma unroll
+                    for (int m = 1; m < 9; m++) {
+                        const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                        const int32_t tmp = p[m] + s;
+                        int32_t hm = max(max(tmp, f[m]), max(e, 0));
+                        f[m] = max(tmp - gapoe, f[m] - gape);
+                        e = max(tmp - gapoe, e - gape);
+                        maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
+                        maxHH = max(maxHH, hm);
+                        p[m] = h[m - 1];
+                        h[m] = hm;
+                    }
+                    bd[gi * WAVE] = pk16(h[8], e);
+                    maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
+                    prev_maxHH = max(maxHH, prev_maxHH);
+                    ridx--; gi++;
+                    go = maxHH < fwd_score;
+                }
+            }
+        }
+        d.qstart[pr] = maxXY_x;
+        d.tstart[pr] = maxXY_y;
+    }
+}
+
 // --------------------------------------------------------------- GLOBAL -----
 // gasal_global_kernel<S> (src/kernels/global.h:30-303).
 template <bool TB>
@@ -308,8 +559,6 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
     const int32_t gapo = c.gapo, gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     uint32_t *__restrict__ bd = x.bd;
-    uint32_t *__restrict__ tbp = nullptr;
-    if (TB) tbp = d.tb + (size_t)x.grp * d.tb_tw * d.tb_qw8 * WAVE + x.lane;
     bd[0] = pk16(0, MINUS_INF);                                                            // :57-60
     for (int32_t cc = 1; cc < 8 * QW; cc++) bd[cc * WAVE] = pk16(-(gapo + gape * cc), MINUS_INF);
     const int32_t rowsel = 8 - ((TW << 3) - (int32_t)x.tlen);                              // :299
@@ -335,6 +584,7 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
             uint32_t cur[8];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) cur[kk] = bd[(j * 8 + kk) * WAVE];
+            uint32_t dirw[8];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t ridx = j * 8 + kk;
@@ -360,12 +610,13 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                     h[m] = hm;
                 }
                 bd[ridx * WAVE] = pk16(h[8], e);
-                if (TB) tbp[((size_t)i * d.tb_qw8 + ridx) * WAVE] = dir;
+                if (TB) dirw[kk] = dir;
                 if (ridx == lastcol && i == TW - 1) {                                      // :98-103
 #pragma unroll
                     for (int m = 1; m < 9; m++) score = (m == rowsel) ? h[m] : score;
                 }
             }
+            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
         }
     }
     d.score[x.pair] = score;
@@ -700,8 +951,15 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const uint32_t pr = d.perm[slot];
     const uint32_t read_len = d.qlen[pr], ref_len = d.tlen[pr];
     const int32_t QW = dregs(read_len), TW = dregs(ref_len);
-    const uint32_t *__restrict__ tbp = d.tb + (size_t)grp * d.tb_tw * d.tb_qw8 * WAVE + lane;
-    const int64_t ncells = (int64_t)TW * 8 * QW * 8;
+    // LOCAL words written by k_local_fast carry the four comparisons as
+    // "not" bits; the match/mismatch label comes from the bases.
+    const bool fastfmt = LOCALT && d.fast && census_fast(d.aux[0]);
+    const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
+    const int32_t nval = c.has_np ? -c.np : 0;
+    uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
+    int32_t ch_key = -1;
+    uint32_t qword = 0, tword = 0;  // cached sequence words (fast format)
+    int32_t q_key = -1, t_key = -1;
     uint8_t *__restrict__ cg = d.cigar + (size_t)pr * d.cigar_cap;
     int i, j, total_score = 0, curr_score = 0;
     if (LOCALT) { i = d.tend[pr]; j = d.qend[pr]; total_score = d.score[pr]; }
@@ -710,19 +968,37 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int read_len_8 = 8 * QW;
     int n_ops = 0, op_select = 3, op_shift = 0, count = 0, offset = 0;
     uint32_t op_to_fill;
-    int64_t prev_word = -1;
-    uint32_t word = 0;
     while (i >= 0 && j >= 0) {
-        const int64_t cell = ((((int64_t)(i >> 3) * read_len_8) << 3) + ((int64_t)j << 3) + (i & 7));
+        // cell (i, j) in flat (strip, padded column, row) order; a column index
+        // of 8QW (GLOBAL start at Lq == 8QW) is column 0 of the next strip
+        // (src/kernels/get_tb.h:50)
+        const int32_t strip = (i >> 3) + (j >= read_len_8 ? 1 : 0);
+        const int32_t col = j >= read_len_8 ? j - read_len_8 : j;
         uint32_t cell_op = 0;
-        if (cell < ncells) {
-            const int64_t wi = cell >> 3;   // word = strip * 8QW + column
-            if (wi != prev_word) {
-                const int64_t strip = wi / read_len_8, col = wi - strip * read_len_8;
-                word = tbp[((size_t)strip * d.tb_qw8 + (size_t)col) * WAVE];
-                prev_word = wi;
+        if (strip < TW) {
+            const int32_t key = strip * QW + (col >> 3);
+            if (key != ch_key) {
+                const uint4 *src = reinterpret_cast<const uint4 *>(tb_chunk(d, grp, lane, strip, col >> 3));
+                const uint4 a = src[0], b = src[1];
+                ch[0] = a.x; ch[1] = a.y; ch[2] = a.z; ch[3] = a.w;
+                ch[4] = b.x; ch[5] = b.y; ch[6] = b.z; ch[7] = b.w;
+                ch_key = key;
             }
-            cell_op = (word >> (28 - 4 * (cell & 7))) & 15u;
+            const int c7 = col & 7;
+            const uint32_t word = c7 < 4 ? (c7 < 2 ? (c7 == 0 ? ch[0] : ch[1]) : (c7 == 2 ? ch[2] : ch[3]))
+                                         : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
+            cell_op = (word >> (28 - 4 * (i & 7))) & 15u;
+            if (fastfmt) {
+                const uint32_t nA = (cell_op >> 3) & 1u, nB = (cell_op >> 2) & 1u;
+                const uint32_t nC3 = (cell_op >> 1) & 1u, nC2 = cell_op & 1u;
+                if ((j >> 3) != q_key) { q_key = j >> 3; qword = Qw[q_key]; }
+                if ((i >> 3) != t_key) { t_key = i >> 3; tword = Tw[t_key]; }
+                const uint32_t qb = (qword >> (28 - 4 * (j & 7))) & 15u;
+                const uint32_t tb = (tword >> (28 - 4 * (i & 7))) & 15u;
+                const uint32_t mx = sub_local(c, nval, qb, tb) < 0 ? 1u : 0u;
+                const uint32_t src = nA == 0 ? mx : (nB == 0 ? 3u : 2u);
+                cell_op = src | ((nC3 ^ 1u) << 3) | ((nC2 ^ 1u) << 2);
+            }
         }
         const uint32_t op = (cell_op >> op_shift) & (uint32_t)op_select;
         op_to_fill = op == 0 || op_select == 3 ? op : (uint32_t)op_shift;
@@ -755,15 +1031,17 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
 
 // Copies each pair's CIGAR to the reference's layout (bytes at the pair's query
 // offset, src/gasal_align.h:52), clipped to its padded query length.
+// 16 threads per pair, 8-byte copies (slots are 8-byte aligned and the copy
+// stays inside the pair's padded query slot).
 __global__ __launch_bounds__(256) void k_cigar_qlayout(G2Dev d) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, t = threadIdx.x & 15u;
     if (k >= d.n) return;
-    uint32_t nops = d.n_ops[k];
-    uint32_t slot = 8u * (uint32_t)dregs(d.qlen[k]);
-    uint32_t nb = nops < slot ? nops : slot;
-    const uint8_t *src = d.cigar + (size_t)k * d.cigar_cap;
-    uint8_t *dst = d.cigar_qlayout + d.qoff[k];
-    for (uint32_t b = 0; b < nb; b++) dst[b] = src[b];
+    const uint32_t nops = d.n_ops[k];
+    const uint32_t slot = 8u * (uint32_t)dregs(d.qlen[k]);
+    const uint32_t nb = nops < slot ? nops : slot;
+    const uint2 *src = reinterpret_cast<const uint2 *>(d.cigar + (size_t)k * d.cigar_cap);
+    uint2 *dst = reinterpret_cast<uint2 *>(d.cigar_qlayout + d.qoff[k]);
+    for (uint32_t w = t; w < (nb + 7) / 8; w += 16) dst[w] = src[w];
 }
 
 // WITH_TB for algorithms without a traceback kernel: the reference copies the
@@ -789,6 +1067,7 @@ int g_dispatch = 0;
 }  // namespace
 
 void g2_set_dispatch(int policy) { g_dispatch = policy; }
+int g2_get_dispatch() { return g_dispatch; }
 
 size_t g2_sort_tmp_bytes(uint32_t n) {
     size_t bytes = 0;
@@ -843,7 +1122,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const uint32_t nb256 = (d.n + 255) / 256;
     // 2. reverse / complement (src/gasal_align.cu:232-246)
     if (op.is_rc) {
-        k_rc<<<nb256, 256, 0, s>>>(d, sc.n_code);
+        k_rc<<<(d.n + 15) / 16, 256, 0, s>>>(d, sc.n_code);
         G2_LAUNCH_CHECK();
     }
     if (timing) (void)hipEventRecord(tm->ev[2], s);
@@ -867,6 +1146,23 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const bool second = op.second != 0;
     switch (op.algo) {
         case A_LOCAL:
+            if (d.fast) {
+                const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
+                const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
+                k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
+                uint32_t cb = (qw + tw + 255) / 256;
+                k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);
+                if (op.start == S_WITH_TB) {
+                    if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITH_TB, false><<<grid, blk, 0, s>>>(d, sc);
+                } else if (op.start == S_WITH_START) {
+                    if (second) k_local_fast<S_WITH_START, true><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITH_START, false><<<grid, blk, 0, s>>>(d, sc);
+                } else {
+                    if (second) k_local_fast<S_WITHOUT_START, true><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITHOUT_START, false><<<grid, blk, 0, s>>>(d, sc);
+                }
+            }
             if (op.start == S_WITH_TB) {
                 if (second) k_local<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
                 else k_local<S_WITH_TB, false><<<grid, blk, 0, s>>>(d, sc);
@@ -916,7 +1212,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
         else k_no_tb<<<nb256, 256, 0, s>>>(d);
         G2_LAUNCH_CHECK();
         if (d.cigar_qlayout) {
-            k_cigar_qlayout<<<nb256, 256, 0, s>>>(d);
+            k_cigar_qlayout<<<(d.n + 15) / 16, 256, 0, s>>>(d);
             G2_LAUNCH_CHECK();
         }
     }

@@ -938,7 +938,30 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     d.bnd = m->bnd; d.bnd_cols = bnd_cols;
     d.tb = m->tb; d.tb_tw = tw; d.tb_qw8 = 8 * qw;
     d.eh = m->eh; d.eh_cols = eh_cols;
+    if (!m->aux) G2_CHECK(hipMalloc((void **)&m->aux, 256));
     d.aux = m->aux;
+
+    // LOCAL fast path eligibility: 4-bit signed substitution scores and a
+    // position encoding h << S | ~strip | ~column | row that fits 31 bits.
+    d.fast = 0;
+    if (op.algo == LOCAL && g2_get_dispatch() != 1) {
+        auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
+        const G2Scores &sc = g_scores;
+        const int32_t nval = sc.has_np ? -sc.np : 0;
+        const bool s4 = sc.match >= -8 && sc.match <= 7 && -sc.mismatch >= -8 && -sc.mismatch <= 7 && nval >= -8 &&
+                        nval <= 7;
+        const int cb = std::max(1, bl(8ull * qw - 1)), sbits = std::max(1, bl(tw > 0 ? tw - 1 : 0));
+        const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * qw, 8ull * tw);
+        const int hb = bl(maxh) + 1;
+        const int S = 3 + cb + sbits;
+        if (s4 && S + hb <= 31 && qw > 0 && tw > 0) {
+            d.fast = 1;
+            d.enc_s = S;
+            d.enc_cb = cb;
+            d.enc_cm = (1u << cb) - 1u;
+            d.enc_sm = (1u << sbits) - 1u;
+        }
+    }
 }
 
 G2Scores scores_now() { return g_scores; }

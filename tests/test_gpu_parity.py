@@ -51,6 +51,52 @@ def test_ksw(second):
     assert_parity(got, ref, g.KSW, g.WITHOUT_START, second, b)
 
 
+@pytest.mark.parametrize("start,second", [(g.WITH_TB, True), (g.WITH_START, False), (g.WITHOUT_START, True)])
+def test_local_reference_order_kernel(start, second):
+    """The reference-order kernel (used when the fast path does not apply)
+    forced by dispatch policy 1."""
+    g.lib().gasal2_set_dispatch(1)
+    try:
+        b = g.synth_batch(2, 900, 500)
+        got, ref = run_both(b, g.LOCAL, start, second, is_rc=True, n_penalty=1)
+        assert_parity(got, ref, g.LOCAL, start, second, b)
+    finally:
+        g.lib().gasal2_set_dispatch(0)
+
+
+def test_local_nibble_class_collision_falls_back():
+    """'I' (nibble 9) shares a class with 'A' (nibble 1): the census routes the
+    batch to the reference-order kernel; results stay exact."""
+    b0 = g.synth_batch(1, 901, 300)
+    qs = [b0.query(i) for i in range(b0.n)]
+    ts = [b0.target(i) for i in range(b0.n)]
+    qs[5] = qs[5][:20] + b"IIA" + qs[5][23:]
+    b = g.make_batch(qs, ts)
+    got, ref = run_both(b, g.LOCAL, g.WITH_TB, True)
+    assert_parity(got, ref, g.LOCAL, g.WITH_TB, True, b)
+
+
+@pytest.mark.parametrize("scores", [(1, 9, 6, 1), (8, 4, 6, 1), (2, 3, 5, 2)])
+def test_local_other_scores(scores):
+    b = g.synth_batch(2, 902, 400)
+    got, ref = run_both(b, g.LOCAL, g.WITH_TB, True, is_rc=True, scores=scores, n_penalty=1)
+    assert_parity(got, ref, g.LOCAL, g.WITH_TB, True, b)
+
+
+@pytest.mark.parametrize("n_code", [0x4E, 0x0E])
+def test_reverse_complement_both_sides(n_code):
+    """Random per-pair ops on query and target; N_CODE 0x0E makes the reverse
+    shift by the padding count (src/kernels/pack_rc_seqs.h:112-165)."""
+    import numpy as np
+    b0 = g.synth_batch(1, 903, 400)
+    rng = np.random.default_rng(5)
+    qs = [b0.query(i) for i in range(b0.n)]
+    ts = [b0.target(i) for i in range(b0.n)]
+    b = g.make_batch(qs, ts, qop=rng.integers(0, 4, b0.n), top=rng.integers(0, 4, b0.n), n_code=n_code)
+    got, ref = run_both(b, g.LOCAL, g.WITH_TB, True, is_rc=True, n_code=n_code)
+    assert_parity(got, ref, g.LOCAL, g.WITH_TB, True, b)
+
+
 def test_local_tiny_edge_cases():
     qs = [b"A", b"ACGT", b"ACGTACGT", b"ACGTACGTA", b"N", b"NNNNNNNNN", b"TTTT", b"ACGTNACGT" * 5]
     ts = [b"A", b"ACGTTT", b"ACGTACGT", b"TACGTACGT", b"A", b"ACGTACGTAC", b"GGGG", b"ACGTACGT" * 6]
