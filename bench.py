@@ -40,9 +40,18 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
-    ap.add_argument("--mode", default="local_tb_second",
-                    choices=["local_tb_second", "local_score", "global", "semi_global"])
-    ap.add_argument("--cpu-pairs", type=int, default=20000, help="oracle sample for cpu_baseline (0 = skip)")
+    ap.add_argument("--mode", default=None,
+                    choices=["local_tb_second", "local_score", "global", "semi_global"],
+                    help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
+    ap.add_argument("--workload", default="full", choices=["full", "uneven"],
+                    help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp")
+    ap.add_argument("--dispatch", type=int, default=0,
+                    help="kernel dispatch policy (0 auto, 2 one lane per pair only, 3 one wave per pair)")
+    ap.add_argument("--compare-lane-only", action="store_true",
+                    help="also time dispatch policy 2 (one lane per pair only) on the same batch")
+    ap.add_argument("--cpu-pairs", type=int, default=-1,
+                    help="oracle sample for cpu_baseline: -1 = sized to about --cpu-seconds, 0 = skip")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per DP launch")
     args = ap.parse_args()
@@ -55,11 +64,19 @@ def main():
     import gasal2_amd as g
 
     t0 = time.time()
-    batch = g.synth_batch(2, 2, args.pairs)
+    if args.workload == "full":
+        batch = g.synth_batch(2, 2, args.pairs)
+        mode = args.mode or "local_tb_second"
+        workload = ("full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
+                    "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries")
+    else:
+        batch = g.synth_batch(3, 3, args.pairs if args.pairs != 1_000_000 else 500_000)
+        mode = args.mode or "local_score"
+        workload = ("uneven-length batch: 500k pairs, Lq log-normal (median 400, clipped 50-5000 bp), "
+                    "Lt = Lq * U[0.8,1.2], 10% divergence; one launch, per-pair lane/wave switch")
     cells = int(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
     log(f"generated {batch.n} pairs, {cells / 1e9:.2f} G cells in {time.time() - t0:.1f}s")
 
-    mode = args.mode
     if mode == "local_tb_second":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True)
         ops = OPS_TB
@@ -81,30 +98,45 @@ def main():
     a.load(batch)
     log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
 
-    for _ in range(args.warmup):
-        a.run_resident(timing=True)
+    L = g.lib()
+
+    def timed_steps(policy):
+        L.gasal2_set_dispatch(policy)
+        for _ in range(args.warmup):
+            a.run_resident(timing=True)
+            a.sync()
+        dp, tb = [], []
         a.sync()
-    dp_ms, tb_ms, total_ms = [], [], []
-    a.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        a.run_resident(timing=True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a.run_resident(timing=True)
+            a.sync()
+            dp.append(a.stage_ms(3))
+            tb.append(a.stage_ms(4))
         a.sync()
-        dp_ms.append(a.stage_ms(3))
-        tb_ms.append(a.stage_ms(4))
-        total_ms.append(a.stage_ms(0))
-    a.sync()
-    wall = time.perf_counter() - t0
+        return time.perf_counter() - t0, dp, tb
+
+    lane_only = None
+    if args.compare_lane_only:
+        w2, dp2, _ = timed_steps(2)
+        lane_only = {"gcups": round(cells / (w2 / args.steps) / 1e9, 3), "ms_per_step": round(1e3 * w2 / args.steps, 3),
+                     "dp_kernel_ms": round(float(np.mean(dp2)), 3)}
+        log(f"one lane per pair only: {lane_only}")
+    wall, dp_ms, tb_ms = timed_steps(args.dispatch)
     ms_per_step = 1e3 * wall / args.steps
     gcups = cells / (wall / args.steps) / 1e9
     dp_avg = float(np.mean(dp_ms))
     achieved_tops = cells * ops / (dp_avg * 1e-3) / 1e12
     log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); DP kernel {dp_avg:.2f} ms; traceback {np.mean(tb_ms):.2f} ms")
 
-    # PCIe-inclusive rate of one batch through gasal_aln_async (host pages -> results on host)
+    # PCIe-inclusive rate of one batch through gasal_aln_async: host batch
+    # already filled, timed from the call to results on the host
+    qb, tb, n = a.fill(batch, per_pair=False)
     t0 = time.perf_counter()
-    res = a.run(batch, per_pair=False)
+    a.align_filled(qb, tb, n, poll_s=0)
     pcie_s = time.perf_counter() - t0
+    res = a._collect(n, qb)
+    L.gasal2_set_dispatch(0)
     log(f"gasal_aln_async incl. H2D/D2H: {pcie_s * 1e3:.1f} ms ({cells / pcie_s / 1e9:.1f} GCUPS)")
 
     traffic = None
@@ -113,17 +145,26 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
 
     cpu = None
-    if args.cpu_pairs > 0:
+    if args.cpu_pairs != 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
-        m = min(args.cpu_pairs, batch.n)
-        sub = batch.subset(np.arange(m))
+
+        def oracle_on(m):
+            sub = batch.subset(np.arange(m))
+            t0 = time.perf_counter()
+            r = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
+                        head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, is_rc=True,
+                        n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
+            return sub, r, time.perf_counter() - t0
+
+        if args.cpu_pairs > 0:
+            m = min(args.cpu_pairs, batch.n)
+        else:   # pilot on 2000 pairs, then a prefix of about --cpu-seconds
+            m0 = min(2000, batch.n)
+            _, _, s0 = oracle_on(m0)
+            m = int(min(batch.n, max(m0, m0 * args.cpu_seconds / max(s0, 1e-3))))
+        sub, ref, cpu_s = oracle_on(m)
         sub_cells = int(np.dot(sub.qlen.astype(np.int64), sub.tlen.astype(np.int64)))
-        t0 = time.perf_counter()
-        ref = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
-                      head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, is_rc=True,
-                      n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
-        cpu_s = time.perf_counter() - t0
         keys = [k for k in ("score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend", "s_tend")
                 if res.get(k) is not None]
         if mode != "local_tb_second":
@@ -157,9 +198,8 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
-                               "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries",
-                   "mode": mode, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
+        "config": {"workload": workload,
+                   "mode": mode, "dispatch": args.dispatch, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
         "roofline": {"bound": "valu", "kernel": "DP (one lane per pair)", "ops_per_cell": ops,
                      "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3), "unit": "TOP/s",
@@ -168,6 +208,8 @@ def main():
         "cpu_baseline": cpu,
         "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 3),
     }
+    if lane_only:
+        line["lane_only"] = lane_only
     print(json.dumps(line), flush=True)
     a.close()
 

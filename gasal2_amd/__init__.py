@@ -167,6 +167,7 @@ _SIGS = {
     "gasal2_stage_ms": (C.c_float, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
     "gasal2_get_cigar": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32]),
     "gasal2_set_dispatch": (None, [C.c_int32]),
+    "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_device_info": (C.c_int32, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
 }
 
@@ -446,8 +447,11 @@ class Aligner:
             C.memmove(dst_len, lens.ctypes.data, 4 * n)
         return len(batch.q), len(batch.t)
 
-    def run(self, batch, per_pair=None, timing=False):
-        s, p, L = self.s, self.params, self._L
+    def fill(self, batch, per_pair=None):
+        """Host half of a submission: fills the storage's host batch, lengths,
+        offsets and per-pair ops (what a caller does with
+        gasal_host_batch_fill before gasal_aln_async)."""
+        s, p = self.s, self.params
         n = batch.n
         if per_pair is None:
             per_pair = n <= 2000
@@ -458,6 +462,12 @@ class Aligner:
             C.memmove(s.host_target_op, batch.top.ctypes.data, n)
         if p.algo == algo_type.KSW:
             C.memmove(s.host_seed_scores, batch.seeds.ctypes.data, 4 * n)
+        return qb, tb, n
+
+    def align_filled(self, qb, tb, n, timing=False, poll_s=0.0005):
+        """gasal_aln_async on a filled host batch, then gasal_is_aln_async_done
+        until the results are on the host."""
+        s, p, L = self.s, self.params, self._L
         if timing:
             L.gasal2_enable_timing(C.byref(s), 1)
         gasal_aln_async(s, qb, tb, n, p)
@@ -466,9 +476,14 @@ class Aligner:
             r = gasal_is_aln_async_done(s)
             if r == 0:
                 break
-            if r == -2:
-                raise RuntimeError("nothing launched")
-            time.sleep(0.0005)
+            if r != -1:
+                raise RuntimeError(f"gasal_is_aln_async_done returned {r}")
+            if poll_s:
+                time.sleep(poll_s)
+
+    def run(self, batch, per_pair=None, timing=False):
+        qb, tb, n = self.fill(batch, per_pair)
+        self.align_filled(qb, tb, n, timing)
         return self._collect(n, qb)
 
     # ---- resident-input form (gasal2_upload / gasal2_run_resident / gasal2_download)

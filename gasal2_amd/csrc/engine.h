@@ -50,11 +50,19 @@ struct G2Dev {
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
     uint32_t *aux;                   // small device counters (work queue etc.)
-    // LOCAL fast path (see k_local_fast): 0 = off, 1 = launched beside the
-    // generic kernel, the nibble census in aux[0] picks which one runs.
-    int32_t fast;
+    // score-table kernels (k_local_fast, k_global / k_semi LUT variants): 0 =
+    // off, 1 = launched beside the generic kernel, the nibble census in aux[0]
+    // picks which one runs.  enc_ok: the LOCAL lane-per-pair position encoding
+    // fits 31 bits (otherwise k_local takes the pairs the wavefront leaves).
+    int32_t fast, enc_ok;
     int32_t enc_s, enc_cb;           // position encoding: h << enc_s | ~strip << (3+enc_cb) | ~col << 3 | 7-row
     uint32_t enc_cm, enc_sm;         // column / strip field masks
+    // intra-pair wavefront for long pairs (LOCAL score + ends): pairs whose
+    // shape passes wf_min_tw / wf_min_cells are sorted first and taken from an
+    // atomic queue (aux[1], count in aux[2]) by the first wf_blocks blocks.
+    uint32_t wf_blocks, wf_min_tw, wf_min_cells;
+    uint32_t *wf_scratch;            // per wavefront block: per pass 17 x 64 dwords of strip state
+    uint32_t wf_stride;              // dwords per wavefront block
 };
 
 struct G2Timing {

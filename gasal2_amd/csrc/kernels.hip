@@ -63,10 +63,12 @@ __device__ __forceinline__ void tb_store(uint32_t *p, const uint32_t *w) {
     reinterpret_cast<uint4 *>(p)[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-__device__ __forceinline__ bool lane_setup(const G2Dev &d, LaneCtx &x) {
+// goff: blocks before this kernel's lane-per-pair groups; slot0: sorted slots
+// taken by the wavefront path.
+__device__ __forceinline__ bool lane_setup(const G2Dev &d, LaneCtx &x, uint32_t goff = 0, uint32_t slot0 = 0) {
     x.lane = threadIdx.x & (WAVE - 1);
-    x.grp = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    x.slot = x.grp * WAVE + x.lane;
+    x.grp = (blockIdx.x - goff) * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    x.slot = slot0 + x.grp * WAVE + x.lane;
     if (x.slot >= d.n) return false;
     x.pair = d.perm[x.slot];
     x.qlen = d.qlen[x.pair];
@@ -189,14 +191,21 @@ __global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
 }
 
 // ------------------------------------------------------------------ sort ----
+__device__ __forceinline__ bool census_fast(uint32_t m);
+
+// Sort key: long-pair flag (goes to the wavefront kernel), then query words,
+// then target words; sorted descending.  aux[2] counts the long pairs.
 __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
     uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
-    qw = qw > 0xFFFFu ? 0xFFFFu : qw;
+    const bool lng = d.wf_blocks > 0 && census_fast(d.aux[0]) && tw >= d.wf_min_tw &&
+                     (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
+    qw = qw > 0x7FFFu ? 0x7FFFu : qw;
     tw = tw > 0xFFFFu ? 0xFFFFu : tw;
-    d.keys[k] = (qw << 16) | tw;
+    d.keys[k] = (lng ? 0x80000000u : 0u) | (qw << 16) | tw;
     d.perm[k] = k;
+    if (lng) atomicAdd(&d.aux[2], 1u);
 }
 
 // ---------------------------------------------------------------- LOCAL -----
@@ -205,9 +214,10 @@ __device__ __forceinline__ bool census_fast(uint32_t m);
 
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
-    if (d.fast && census_fast(d.aux[0])) return;   // k_local_fast handles this batch
+    if (d.fast && d.enc_ok && census_fast(d.aux[0])) return;   // k_local_fast handles this batch
     LaneCtx x;
-    if (!lane_setup(d, x)) return;
+    // long pairs of a score-only batch sort first and run one wave per pair
+    if (!lane_setup(d, x, 0, d.wf_blocks && census_fast(d.aux[0]) ? d.aux[2] : 0u)) return;
     constexpr bool TB = START == S_WITH_TB;
     const int32_t nval = c.has_np ? -c.np : 0;
     const int32_t gapoe = c.gapoe, gape = c.gape;
@@ -358,13 +368,14 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
 //    h!=M, h!=F_in, F restarted, E restarted.  k_get_tb decodes them.
 __device__ __forceinline__ bool census_fast(uint32_t m) { return ((m & (m >> 8)) & 0xFFu) == 0u; }
 
+// GLOBAL_SEM: DEV_GET_SUB_SCORE_GLOBAL (N is an ordinary base unless N_PENALTY).
+template <bool GLOBAL_SEM = false>
 __device__ __forceinline__ uint32_t score_lut(const G2Scores &c, int32_t nval, uint32_t census, uint32_t tnib) {
     uint32_t lut = 0;
 #pragma unroll
     for (uint32_t cls = 0; cls < 8; cls++) {
         const uint32_t q = ((census >> cls) & 1u) ? cls : cls + 8u;   // the nibble this class stands for
-        int32_t s = (q == tnib) ? c.match : -c.mismatch;
-        s = (q == c.n_value || tnib == c.n_value) ? nval : s;
+        int32_t s = GLOBAL_SEM ? sub_global(c, q, tnib) : sub_local(c, nval, q, tnib);
         lut |= ((uint32_t)s & 15u) << (4 * cls);
     }
     return lut;
@@ -389,12 +400,162 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
     if (k < n) p[k] = 0;
 }
 
+// ------------------------------------------------ LOCAL, one wave per pair ----
+// Intra-pair wavefront for long pairs (score and ends).  Strip s (8 target
+// rows) lives on lane s % 64 in pass s / 64; lane l works on column c at step
+// c + l, so the (H, E) row leaving lane l's strip at column c is handed to lane
+// l + 1 one step later with DPP wave_shr:1, and the query class travels the
+// same way.  Lane 63's bottom row is parked in an LDS ring for lane 0 of the
+// next pass.  Queries longer than WF_W columns are cut into column blocks;
+// between blocks each lane saves its strip state (h, f per row and the
+// diagonal corner) in per-block scratch.  The recurrences and the int16
+// boundary rows are those of k_local_fast.
+//
+// End position: the reference keeps the first strict maximum in (strip,
+// column, row) order (src/kernels/local_kernel_template.h:121-135, 347-371).
+// Within one run (one strip, one column block) columns arrive in order, so a
+// strict compare keeps the first; runs are merged per lane and then over the
+// wave on the key (h, -strip, -column, -row).
+constexpr int WF_W = 1024;
+
+struct WfBest {
+    int32_t v;      // h << 3 | (7 - row in strip); 0 = nothing positive yet
+    int32_t strip, col;
+};
+
+// true when a precedes b in the reference's choice: higher h, then lower
+// strip, then lower column (one (strip, column) cell group is never split
+// over two runs, so the row bits never decide)
+__device__ __forceinline__ bool wf_before(int32_t av, int32_t as, int32_t ac, int32_t bv, int32_t bs, int32_t bc) {
+    if ((av >> 3) != (bv >> 3)) return (av >> 3) > (bv >> 3);
+    if (as != bs) return as < bs;
+    return ac < bc;
+}
+
+__device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
+    __shared__ uint32_t ring[WF_W];
+    const int32_t l = (int32_t)threadIdx.x;
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    uint32_t *vs = d.wf_scratch + (size_t)blockIdx.x * d.wf_stride;
+    // Every value that steers a loop below is wave-uniform, and no branch on the
+    // lane index feeds one: with `if (lane == 0) item = atomicAdd(...)` the
+    // compiler splits the queue loop per lane (lane 0 re-enters the atomic,
+    // lanes 1..63 keep iterating with item 0), which never terminates.  All
+    // lanes take part in the atomic; only lane 0 adds, so lane 0 reads the
+    // counter before its own increment.
+    const uint32_t nlong = __builtin_amdgcn_readfirstlane(d.aux[2]);
+    for (;;) {
+        const uint32_t item = __builtin_amdgcn_readfirstlane(atomicAdd(&d.aux[1], l == 0 ? 1u : 0u));
+        if (item >= nlong) break;
+        const uint32_t pr = __builtin_amdgcn_readfirstlane(d.perm[item]);
+        const uint32_t qlen = __builtin_amdgcn_readfirstlane(d.qlen[pr]);
+        const uint32_t tlen = __builtin_amdgcn_readfirstlane(d.tlen[pr]);
+        const int32_t QW = dregs(qlen), TW = dregs(tlen);
+        const uint32_t *Q = d.pq + (d.qoff[pr] >> 3);
+        const uint32_t *T = d.pt + (d.toff[pr] >> 3);
+        const int32_t C = 8 * QW;
+        const int32_t npass = (TW + 63) / 64;
+        const int32_t nblk = (C + WF_W - 1) / WF_W;
+        WfBest lb = {0, 0x7FFFFFFF, 0x7FFFFFFF};
+        for (int32_t b = 0; b < nblk; b++) {
+            const int32_t c0 = b * WF_W;
+            const int32_t wb = min(WF_W, C - c0);
+            for (int32_t k = 0; k < npass; k++) {
+                const int32_t s = l + 64 * k;
+                const bool act = s < TW;
+                uint32_t *st = vs + (size_t)k * 17 * 64 + l;   // this lane's strip state, [17][64]
+                int32_t h[9], f[9];
+                int32_t ptop = 0;   // H(strip top - 1, column - 1): diagonal of the strip's first row
+#pragma unroll
+                for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; }
+                if (b > 0 && act) {
+#pragma unroll
+                    for (int m = 1; m < 9; m++) { h[m] = (int32_t)st[(m - 1) * 64]; f[m] = (int32_t)st[(m + 7) * 64]; }
+                    ptop = (int32_t)st[16 * 64];
+                }
+                uint32_t lut[9];
+                const uint32_t gpac = act ? T[s] : 0u;
+#pragma unroll
+                for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
+                int32_t rv = 0, rt = 0;   // run best: encoded value, step
+                uint32_t out = 0, qcls = 0;
+                uint32_t qcur = 0, qnext = Q[c0 >> 3];   // query word of lane 0's column, next one prefetched
+                for (int32_t tau = 0; tau < wb + 63; tau++) {
+                    // inputs of lane 0 (column c0 + tau): top boundary and query class
+                    uint32_t top0 = 0, q0 = 0;
+                    if (tau < wb) {
+                        if ((tau & 7) == 0) {
+                            qcur = qnext;
+                            if (tau + 8 < wb) qnext = Q[(c0 + tau + 8) >> 3];
+                        }
+                        if (k > 0) top0 = ring[tau];
+                        q0 = ((qcur >> (28 - 4 * (tau & 7))) & 7u) << 2;
+                    }
+                    const uint32_t tin = (uint32_t)__builtin_amdgcn_update_dpp((int)top0, (int)out, 0x138, 0xF, 0xF, false);
+                    qcls = (uint32_t)__builtin_amdgcn_update_dpp((int)q0, (int)qcls, 0x138, 0xF, 0xF, false);
+                    const int32_t rel = tau - l;
+                    if (act && rel >= 0 && rel < wb) {
+                        int32_t e = (int32_t)tin >> 16;
+                        int32_t p = ptop;
+                        ptop = sx16(tin);
+                        int32_t colbest = 0;
+#pragma unroll
+                        for (int m = 1; m < 9; m++) {
+                            const int32_t sc = __builtin_amdgcn_sbfe(lut[m], qcls, 4);
+                            const int32_t tmp = p + sc;
+                            const int32_t hm = max(max(max(tmp, f[m]), e), 0);
+                            const int32_t t = tmp - gapoe;
+                            f[m] = max(t, f[m] - gape);
+                            e = max(t, e - gape);
+                            const int32_t en = (hm << 3) | (8 - m);
+                            colbest = m == 1 ? en : max(colbest, en);
+                            p = h[m];
+                            h[m] = hm;
+                        }
+                        // strictly higher h only: an equal h in a later column never wins
+                        if (colbest > (rv | 7)) { rv = colbest; rt = tau; }
+                        out = pk16(h[8], e);
+                        if (l == 63 && k + 1 < npass) ring[rel] = out;
+                    }
+                }
+                if (rv >= 8) {   // h > 0 in this run
+                    const int32_t rc = c0 + rt - l;
+                    if (wf_before(rv, s, rc, lb.v, lb.strip, lb.col)) lb = {rv, s, rc};
+                }
+                if (b + 1 < nblk && act) {
+#pragma unroll
+                    for (int m = 1; m < 9; m++) { st[(m - 1) * 64] = (uint32_t)h[m]; st[(m + 7) * 64] = (uint32_t)f[m]; }
+                    st[16 * 64] = (uint32_t)ptop;
+                }
+                __syncthreads();   // ring written by lane 63 is read by lane 0 in the next pass
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const int32_t ov = __shfl_xor(lb.v, off), os = __shfl_xor(lb.strip, off), oc = __shfl_xor(lb.col, off);
+            if (wf_before(ov, os, oc, lb.v, lb.strip, lb.col)) lb = {ov, os, oc};
+        }
+        if (l == 0) {
+            const int32_t v = lb.v >> 3;
+            d.score[pr] = v;
+            d.qend[pr] = v > 0 ? lb.col : 0;
+            d.tend[pr] = v > 0 ? lb.strip * 8 + (7 - (lb.v & 7)) : 0;
+        }
+    }
+}
+
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
     if (!census_fast(census)) return;
+    if (blockIdx.x < d.wf_blocks) {
+        if (START == S_WITHOUT_START && !SECOND) wf_local(d, c, census);
+        return;
+    }
+    if (!d.enc_ok) return;   // position encoding does not fit: k_local takes the short pairs
     LaneCtx x;
-    if (!lane_setup(d, x)) return;
+    if (!lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) return;
     constexpr bool TB = START == S_WITH_TB;
     const int32_t nval = c.has_np ? -c.np : 0;
     const int32_t gapoe = c.gapoe, gape = c.gape;
@@ -552,8 +713,12 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
 
 // --------------------------------------------------------------- GLOBAL -----
 // gasal_global_kernel<S> (src/kernels/global.h:30-303).
-template <bool TB>
+// LUT: substitution from a per-strip 4-bit score table (see k_local_fast),
+// launched beside the generic variant; the nibble census picks one.
+template <bool TB, bool LUT>
 __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
+    const uint32_t census = d.aux[0];
+    if (LUT ? !census_fast(census) : (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const int32_t gapo = c.gapo, gapoe = c.gapoe, gape = c.gape;
@@ -578,7 +743,10 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
         const uint32_t gpac = x.T[i];
         uint32_t gb[9];
 #pragma unroll
-        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        for (int m = 1; m < 9; m++) {
+            gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            if (LUT) gb[m] = score_lut<true>(c, 0, census, gb[m]);
+        }
         for (int32_t j = 0; j < QW; j++) {
             const uint32_t rpac = x.Q[j];
             uint32_t cur[8];
@@ -594,7 +762,7 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                 uint32_t dir = 0;
 #pragma unroll
                 for (int m = 1; m < 9; m++) {
-                    const int32_t s = sub_global(c, rbase, gb[m]);
+                    const int32_t s = LUT ? __builtin_amdgcn_sbfe(gb[m], (rbase & 7u) << 2, 4) : sub_global(c, rbase, gb[m]);
                     const int32_t tmp = p[m] + s;
                     int32_t hm = max(max(tmp, f[m]), e);
                     if (TB) {
@@ -650,14 +818,17 @@ __device__ __forceinline__ void semi_strip_init(const G2Scores &c, int32_t *h, i
         u += 8;
     }
 }
+// gb[m]: the row's target nibble, or (LUT) its 4-bit score table.
+template <bool LUT>
 __device__ __forceinline__ void semi_column(const G2Scores &c, int32_t nval, uint32_t rbase, const uint32_t *gb,
                                             int32_t *h, int32_t *f, int32_t *p, uint32_t &hd) {
     h[0] = sx16(hd);
     int32_t e = sx16(hd >> 16);
     int32_t prev_hm_diff = h[0] - c.gapoe;
+    const uint32_t qsh = (rbase & 7u) << 2;
 #pragma unroll
     for (int m = 1; m < 9; m++) {                                                          // :17-28
-        const int32_t s = sub_local(c, nval, rbase, gb[m]);
+        const int32_t s = LUT ? __builtin_amdgcn_sbfe(gb[m], qsh, 4) : sub_local(c, nval, rbase, gb[m]);
         int32_t cur = h[m] - c.gapoe;
         f[m] = max(cur, f[m] - c.gape);
         cur = p[m] + s;
@@ -672,19 +843,26 @@ __device__ __forceinline__ void semi_column(const G2Scores &c, int32_t nval, uin
 }
 // Reversed copy word k of a sequence (:244-266): base 8k+b at bits 28-4b is the
 // original base len-1-(8k+b); positions past the start are nibble 0.
+// It is the nibble-reversed 8-base window of the forward sequence that starts
+// at base len-8(k+1): one funnel shift of two forward words, then a nibble
+// reverse.
 __device__ __forceinline__ uint32_t rev_word(const uint32_t *W, int32_t len, int32_t k) {
-    uint32_t r = 0;
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        int32_t ii = len - 1 - (8 * k + b);
-        uint32_t nib = ii >= 0 ? ((W[ii >> 3] >> (28 - 4 * (ii & 7))) & 15u) : 0u;
-        r |= nib << (28 - 4 * b);
+    const int32_t st = len - 8 * (k + 1);
+    uint32_t win;
+    if (st >= 0) {
+        const int32_t w0 = st >> 3, off = st & 7;
+        win = off == 0 ? W[w0] : __builtin_amdgcn_alignbit(W[w0], W[w0 + 1], 32 - 4 * off);
+    } else {
+        win = W[0] >> (4 * (-st));   // bases before the start are nibble 0
     }
-    return r;
+    return nib_reverse(win);
 }
 
-template <int START, bool SECOND, int HEAD, int TAIL>
+template <int START, bool SECOND, int HEAD, int TAIL, bool LUT>
 __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
+    // the WITH_START pass reads reversed copies padded with nibble 0
+    const uint32_t census = d.aux[0] | (START == S_WITH_START ? 1u : 0u);
+    if (LUT ? !census_fast(census) : (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const int32_t nval = c.has_np ? -c.np : 0;
@@ -706,14 +884,17 @@ __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
         const uint32_t gpac = x.T[i];
         uint32_t gb[9];
 #pragma unroll
-        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        for (int m = 1; m < 9; m++) {
+            gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            if (LUT) gb[m] = score_lut(c, nval, census, gb[m]);
+        }
         for (int32_t j = 0; j < QW; j++) {
             const uint32_t rpac = x.Q[j];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t ridx = j * 8 + kk;
                 uint32_t hd = bd[ridx * WAVE];
-                semi_column(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
+                semi_column<LUT>(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
                 bd[ridx * WAVE] = hd;
                 if (TT && ridx + 1 == read_len) {                                          // :160-178
 #pragma unroll
@@ -767,14 +948,17 @@ __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
             const uint32_t gpac = i >= 0 ? rev_word(x.T, ref_len, i) : 0u;
             uint32_t gb[9];
 #pragma unroll
-            for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            for (int m = 1; m < 9; m++) {
+            gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            if (LUT) gb[m] = score_lut(c, nval, census, gb[m]);
+        }
             for (int32_t j = 0; j < QW && maxHH < fwd_score; j++) {
                 const uint32_t rpac = rev_word(x.Q, read_len, j);
 #pragma unroll
                 for (int kk = 0; kk < 8; kk++) {
                     const int32_t ridx = j * 8 + kk;
                     uint32_t hd = bd[ridx * WAVE];
-                    semi_column(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
+                    semi_column<LUT>(c, nval, (rpac >> (28 - 4 * kk)) & 15u, gb, h, f, p, hd);
                     bd[ridx * WAVE] = hd;
                     if (TT && ridx + 1 == read_len) {
 #pragma unroll
@@ -953,7 +1137,7 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int32_t QW = dregs(read_len), TW = dregs(ref_len);
     // LOCAL words written by k_local_fast carry the four comparisons as
     // "not" bits; the match/mismatch label comes from the bases.
-    const bool fastfmt = LOCALT && d.fast && census_fast(d.aux[0]);
+    const bool fastfmt = LOCALT && d.fast && d.enc_ok && census_fast(d.aux[0]);
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1082,14 +1266,14 @@ size_t g2_sort_tmp_bytes(uint32_t n) {
         if (e_ != hipSuccess) return e_;         \
     } while (0)
 
-template <int START, bool SECOND>
+template <int START, bool SECOND, bool LUT>
 static void launch_semi_ht(int head, int tail, dim3 g, dim3 b, hipStream_t s, const G2Dev &d, const G2Scores &c) {
 #define G2_SEMI_T(H)                                                                                         \
     switch (tail) {                                                                                          \
-        case D_NONE: k_semi<START, SECOND, H, D_NONE><<<g, b, 0, s>>>(d, c); break;                          \
-        case D_QUERY: k_semi<START, SECOND, H, D_QUERY><<<g, b, 0, s>>>(d, c); break;                        \
-        case D_TARGET: k_semi<START, SECOND, H, D_TARGET><<<g, b, 0, s>>>(d, c); break;                      \
-        default: k_semi<START, SECOND, H, D_BOTH><<<g, b, 0, s>>>(d, c); break;                              \
+        case D_NONE: k_semi<START, SECOND, H, D_NONE, LUT><<<g, b, 0, s>>>(d, c); break;                     \
+        case D_QUERY: k_semi<START, SECOND, H, D_QUERY, LUT><<<g, b, 0, s>>>(d, c); break;                   \
+        case D_TARGET: k_semi<START, SECOND, H, D_TARGET, LUT><<<g, b, 0, s>>>(d, c); break;                 \
+        default: k_semi<START, SECOND, H, D_BOTH, LUT><<<g, b, 0, s>>>(d, c); break;                         \
     }
     switch (head) {
         case D_NONE: G2_SEMI_T(D_NONE); break;
@@ -1126,7 +1310,18 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
         G2_LAUNCH_CHECK();
     }
     if (timing) (void)hipEventRecord(tm->ev[2], s);
-    // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds
+    // nibble census over the packed batch (aux[0]) for the LUT kernels; aux[1..2]
+    // are the long-pair queue head and count
+    if (d.fast) {
+        const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
+        const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
+        k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
+        uint32_t cb = (qw + tw + 255) / 256;
+        k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);
+        G2_LAUNCH_CHECK();
+    }
+    // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds;
+    // long pairs for the one-wave-per-pair path sort ahead of all others
     k_keys<<<nb256, 256, 0, s>>>(d);
     G2_LAUNCH_CHECK();
     {
@@ -1147,11 +1342,6 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     switch (op.algo) {
         case A_LOCAL:
             if (d.fast) {
-                const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
-                const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
-                k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
-                uint32_t cb = (qw + tw + 255) / 256;
-                k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);
                 if (op.start == S_WITH_TB) {
                     if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
                     else k_local_fast<S_WITH_TB, false><<<grid, blk, 0, s>>>(d, sc);
@@ -1160,7 +1350,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
                     else k_local_fast<S_WITH_START, false><<<grid, blk, 0, s>>>(d, sc);
                 } else {
                     if (second) k_local_fast<S_WITHOUT_START, true><<<grid, blk, 0, s>>>(d, sc);
-                    else k_local_fast<S_WITHOUT_START, false><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITHOUT_START, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
                 }
             }
             if (op.start == S_WITH_TB) {
@@ -1175,16 +1365,28 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             }
             break;
         case A_GLOBAL:
-            if (tb) k_global<true><<<grid, blk, 0, s>>>(d, sc);
-            else k_global<false><<<grid, blk, 0, s>>>(d, sc);
+            if (d.fast) {
+                if (tb) k_global<true, true><<<grid, blk, 0, s>>>(d, sc);
+                else k_global<false, true><<<grid, blk, 0, s>>>(d, sc);
+            }
+            if (tb) k_global<true, false><<<grid, blk, 0, s>>>(d, sc);
+            else k_global<false, false><<<grid, blk, 0, s>>>(d, sc);
             break;
         case A_SEMI_GLOBAL:
             if (op.start == S_WITH_START) {
-                if (second) launch_semi_ht<S_WITH_START, true>(op.head, op.tail, grid, blk, s, d, sc);
-                else launch_semi_ht<S_WITH_START, false>(op.head, op.tail, grid, blk, s, d, sc);
+                if (d.fast) {
+                    if (second) launch_semi_ht<S_WITH_START, true, true>(op.head, op.tail, grid, blk, s, d, sc);
+                    else launch_semi_ht<S_WITH_START, false, true>(op.head, op.tail, grid, blk, s, d, sc);
+                }
+                if (second) launch_semi_ht<S_WITH_START, true, false>(op.head, op.tail, grid, blk, s, d, sc);
+                else launch_semi_ht<S_WITH_START, false, false>(op.head, op.tail, grid, blk, s, d, sc);
             } else {
-                if (second) launch_semi_ht<S_WITHOUT_START, true>(op.head, op.tail, grid, blk, s, d, sc);
-                else launch_semi_ht<S_WITHOUT_START, false>(op.head, op.tail, grid, blk, s, d, sc);
+                if (d.fast) {
+                    if (second) launch_semi_ht<S_WITHOUT_START, true, true>(op.head, op.tail, grid, blk, s, d, sc);
+                    else launch_semi_ht<S_WITHOUT_START, false, true>(op.head, op.tail, grid, blk, s, d, sc);
+                }
+                if (second) launch_semi_ht<S_WITHOUT_START, true, false>(op.head, op.tail, grid, blk, s, d, sc);
+                else launch_semi_ht<S_WITHOUT_START, false, false>(op.head, op.tail, grid, blk, s, d, sc);
             }
             break;
         case A_BANDED:

@@ -30,6 +30,8 @@ G2Scores g_scores = {0, 0, 0, 0, 0, (uint32_t)(N_CODE & 0xF), N_CODE,
 
 int g_error_mode = 0;
 int g_error_count = 0;
+// one-wave-per-pair thresholds: target words, cells (64 per query word x target word)
+uint32_t g_wf_min_tw = 48, g_wf_min_cells = 1u << 20, g_wf_max_blocks = 2048;
 std::string g_last_error;
 std::mutex g_err_mu;
 
@@ -53,6 +55,8 @@ struct Impl {
     int32_t *scratch_i32 = nullptr;  // stand-ins for result arrays the mode does not expose
     size_t scratch_i32_cap = 0;
     uint32_t *aux = nullptr;
+    uint32_t *wf = nullptr;          // strip state of the one-wave-per-pair path
+    size_t wf_bytes = 0;
     // last batch
     uint32_t last_n = 0, last_max_q = 0, last_max_t = 0, last_cigar_cap = 0;
     bool last_tb = false;
@@ -111,6 +115,11 @@ extern "C" void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty) {
     g_scores.np = penalty;
 }
 extern "C" void gasal2_set_dispatch(int32_t policy) { g2_set_dispatch(policy); }
+extern "C" void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks) {
+    g_wf_min_tw = min_target_words ? min_target_words : 1u;
+    g_wf_min_cells = min_cells;
+    g_wf_max_blocks = max_blocks ? max_blocks : 1u;
+}
 
 extern "C" int32_t gasal2_device_info(int32_t *n_cu, int32_t *clock_khz, char *name, int32_t name_cap) {
     int dev = 0;
@@ -660,7 +669,7 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 static void impl_free(Impl *m) {
     if (!m) return;
     void *ptrs[] = {m->keys, m->keys_alt, m->perm, m->perm_alt, m->sort_tmp, m->bnd, m->tb, m->eh,
-                    m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux};
+                    m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->tm_init)
@@ -944,22 +953,55 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     // LOCAL fast path eligibility: 4-bit signed substitution scores and a
     // position encoding h << S | ~strip | ~column | row that fits 31 bits.
     d.fast = 0;
-    if (op.algo == LOCAL && g2_get_dispatch() != 1) {
+    d.enc_ok = 0;
+    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL;
+    if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
         auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
+        auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
         const G2Scores &sc = g_scores;
-        const int32_t nval = sc.has_np ? -sc.np : 0;
-        const bool s4 = sc.match >= -8 && sc.match <= 7 && -sc.mismatch >= -8 && -sc.mismatch <= 7 && nval >= -8 &&
-                        nval <= 7;
-        const int cb = std::max(1, bl(8ull * qw - 1)), sbits = std::max(1, bl(tw > 0 ? tw - 1 : 0));
-        const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * qw, 8ull * tw);
-        const int hb = bl(maxh) + 1;
-        const int S = 3 + cb + sbits;
-        if (s4 && S + hb <= 31 && qw > 0 && tw > 0) {
-            d.fast = 1;
-            d.enc_s = S;
-            d.enc_cb = cb;
-            d.enc_cm = (1u << cb) - 1u;
-            d.enc_sm = (1u << sbits) - 1u;
+        // the N score of each algorithm's substitution macro (src/gasal_kernels.h:37-56)
+        const int32_t nval = op.algo == GLOBAL ? (sc.has_np ? -sc.np : 0) : (sc.has_np ? -sc.np : 0);
+        const bool s4 = fits4(sc.match) && fits4(-sc.mismatch) && fits4(nval);
+        if (s4) d.fast = 1;
+        if (s4 && op.algo == LOCAL) {
+            const int cb = std::max(1, bl(8ull * qw - 1)), sbits = std::max(1, bl(tw - 1));
+            const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * qw, 8ull * tw);
+            const int hb = bl(maxh) + 1;
+            const int S = 3 + cb + sbits;
+            if (S + hb <= 31) {
+                d.enc_ok = 1;
+                d.enc_s = S;
+                d.enc_cb = cb;
+                d.enc_cm = (1u << cb) - 1u;
+                d.enc_sm = (1u << sbits) - 1u;
+            }
+        }
+    }
+    // Long LOCAL pairs (score and ends only) run one wave per pair.  The
+    // device applies the same shape test in k_keys and also needs the nibble
+    // census to allow the LUT kernels; the host count only sizes the launch.
+    if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g2_get_dispatch() != 2) {
+        const bool all = g2_get_dispatch() == 3;
+        d.wf_min_tw = all ? 1u : g_wf_min_tw;
+        d.wf_min_cells = all ? 0u : g_wf_min_cells;
+        uint32_t nlong = 0, ltw = 0;
+        for (uint32_t k = 0; k < n; k++) {
+            const uint32_t kq = g2_regs(s->host_query_batch_lens[k]), kt = g2_regs(s->host_target_batch_lens[k]);
+            if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
+                nlong++;
+                ltw = std::max(ltw, kt);
+            }
+        }
+        if (nlong) {
+            d.wf_blocks = std::min<uint32_t>(nlong, g_wf_max_blocks);
+            d.wf_stride = ((ltw + 63) / 64) * 17 * 64;
+            if (8 * qw > 1024) {   // more than one column block: strip state is saved between blocks
+                ensure(m->wf, m->wf_bytes, (size_t)d.wf_blocks * d.wf_stride * 4);
+                d.wf_scratch = m->wf;
+            } else {
+                d.wf_stride = 0;
+                d.wf_scratch = nullptr;
+            }
         }
     }
 }

@@ -86,10 +86,22 @@ float gasal2_stage_ms(gasal_gpu_storage_t *s, int32_t stage);
  * cap bytes. */
 uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint32_t cap);
 
-/* Kernel dispatch policy for LOCAL (no reference counterpart; results are
- * identical under every policy):
- *   0 auto, 1 one lane per pair, 2 one wavefront of lanes per pair. */
+/* Kernel dispatch policy (no reference counterpart; results are identical
+ * under every policy):
+ *   0 auto;
+ *   1 reference-order kernels only (one lane per pair, no score tables);
+ *   2 one lane per pair only (no one-wave-per-pair path);
+ *   3 one wave per pair for every LOCAL score-only pair the path accepts. */
 void gasal2_set_dispatch(int32_t policy);
+
+/* Thresholds of the per-pair switch between one lane per pair and one wave per
+ * pair (LOCAL without start/traceback/second best): a pair runs one wave per
+ * pair when its target has at least min_target_words 8-base words and
+ * 64 * query words * target words >= min_cells; at most max_blocks persistent
+ * one-wave blocks take such pairs from a device queue.  Defaults 48, 2^20,
+ * 2048.  No reference counterpart (the reference has one thread per pair,
+ * src/kernels/local_kernel_template.h:74-75). */
+void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks);
 
 /* Properties of the current device used to state the integer-VALU peak:
  * compute units, peak engine clock (kHz) and the name.  Returns 0 on success.
