@@ -168,6 +168,12 @@ _SIGS = {
     "gasal2_get_cigar": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32]),
     "gasal2_set_dispatch": (None, [C.c_int32]),
     "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gasal2_set_graph_mode": (None, [C.c_int32]),
+    "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "gasal2_tune_clear": (None, []),
+    "gasal2_tune_count": (C.c_int32, []),
+    "gasal2_get_graph_mode": (C.c_int32, []),
+    "gasal2_graph_stats": (None, [C.POINTER(gasal_gpu_storage_t), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gasal2_device_info": (C.c_int32, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),
 }
 

@@ -1,0 +1,117 @@
+"""Autotuner for the launch parameters that do not change results.
+
+What it tunes: the per-pair switch between one lane per pair and one wave per
+pair (LOCAL score-and-ends batches): minimum target words, minimum cells, and
+the number of persistent one-wave blocks (`gasal2_set_wavefront`).  Strip
+height stays 8 rows: the reference truncates the boundary row to int16
+between 8-row strips (src/kernels/local_kernel_template.h:152-154), so any
+other height could change results on overflow.  Cells per lane are fixed by
+the kernels (8 rows x 8 columns per register tile).
+
+How: on a sample batch resident in device memory, every candidate runs once to
+capture its graph, then `repeats` timed replays; the candidate with the lowest
+median wins.  Every candidate's outputs must be identical to the first one's
+(a tuned choice never changes results) -- the tuner raises otherwise.
+
+The winners go into a JSON cache keyed by device, mode and mean-query-length
+bucket; `apply(path)` loads a cache into the library (`gasal2_tune_set`),
+which then picks the entry per batch.
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import Aligner, Parameters, device_info, lib
+
+# mean query length buckets (bp): [lo, hi)
+BUCKETS = [(0, 256), (256, 512), (512, 1024), (1024, 2048), (2048, 4096), (4096, 1 << 31)]
+
+DEFAULT_CANDIDATES = [(tw, cells, blocks)
+                      for tw in (16, 32, 48, 64, 128)
+                      for cells in (1 << 18, 1 << 20, 1 << 22)
+                      for blocks in (512, 1024, 2048)]
+
+DEFAULT_WF = (48, 1 << 20, 2048)
+
+
+def bucket_of(mean_q):
+    for lo, hi in BUCKETS:
+        if lo <= mean_q < hi:
+            return lo, hi
+    return BUCKETS[-1]
+
+
+def tune(batch, params, candidates=None, repeats=3, scores=(1, 4, 6, 1), n_penalty=None, log=None):
+    """Times every candidate on `batch`; returns a dict with the winner, the
+    timings and the bucket the batch falls in."""
+    L = lib()
+    cands = list(candidates or DEFAULT_CANDIDATES)
+    a = Aligner(params, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=scores, n_penalty=n_penalty)
+    timings = []
+    first = None
+    try:
+        a.load(batch)
+        for cand in cands:
+            L.gasal2_set_wavefront(*cand)
+            a.run_resident()
+            a.sync()
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                a.run_resident()
+                a.sync()
+                ts.append(time.perf_counter() - t0)
+            out = a.download()
+            keys = [k for k in ("score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend", "s_tend", "n_ops")
+                    if out.get(k) is not None]
+            if first is None:
+                first = {k: out[k].copy() for k in keys}
+            else:
+                for k in keys:
+                    if not np.array_equal(first[k], out[k]):
+                        raise RuntimeError(f"candidate {cand} changed output {k}: tuning must not change results")
+            ms = 1e3 * float(np.median(ts))
+            timings.append({"candidate": list(cand), "ms": round(ms, 4)})
+            if log:
+                log(f"autotune {cand}: {ms:.3f} ms")
+    finally:
+        L.gasal2_set_wavefront(*DEFAULT_WF)
+        a.close()
+    best = min(timings, key=lambda t: t["ms"])
+    mean_q = float(batch.qlen.mean()) if batch.n else 0.0
+    return {"best": best["candidate"], "best_ms": best["ms"], "timings": timings,
+            "bucket": list(bucket_of(mean_q)), "mean_qlen": mean_q,
+            "mode": [int(params.algo), int(params.start_pos), int(bool(params.secondBest))]}
+
+
+def save(results, path):
+    """Writes tuner results (a list of `tune` outputs) as the cache file."""
+    dev = device_info()
+    entries = [{"algo": r["mode"][0], "start": r["mode"][1], "second": r["mode"][2],
+                "lq_lo": r["bucket"][0], "lq_hi": r["bucket"][1],
+                "min_target_words": r["best"][0], "min_cells": r["best"][1], "max_blocks": r["best"][2],
+                "best_ms": r["best_ms"]} for r in results]
+    with open(path, "w") as f:
+        json.dump({"device": dev.get("name", ""), "n_cu": dev.get("n_cu"), "entries": entries}, f, indent=1)
+
+
+def apply(path):
+    """Loads a cache file into the library; returns the number of entries."""
+    if not os.path.exists(path):
+        return 0
+    with open(path) as f:
+        cache = json.load(f)
+    L = lib()
+    for e in cache.get("entries", []):
+        L.gasal2_tune_set(int(e["algo"]), int(e["start"]), int(e["second"]), int(e["lq_lo"]), int(e["lq_hi"]),
+                          int(e["min_target_words"]), int(e["min_cells"]), int(e["max_blocks"]))
+    return len(cache.get("entries", []))
+
+
+def clear():
+    lib().gasal2_tune_clear()
+
+
+__all__ = ["tune", "save", "apply", "clear", "bucket_of", "BUCKETS", "DEFAULT_CANDIDATES", "Parameters"]

@@ -40,9 +40,8 @@ struct G2Dev {
     uint32_t cigar_cap;
     uint8_t *cigar_qlayout;          // reference layout: CIGAR at query offsets
     // workspace
-    uint32_t *keys, *keys_alt, *perm, *perm_alt;
-    void *sort_tmp;
-    size_t sort_tmp_bytes;
+    uint32_t *keys, *perm;           // shape key and sorted permutation, n each
+    uint32_t *hist;                  // G2_SHAPE_BUCKETS counters / bucket offsets
     uint32_t *bnd;                   // boundary rows: per group bnd_cols*64 dwords
     uint32_t bnd_cols;
     uint32_t *tb;                    // direction words: per group tb_tw*tb_qw8*64
@@ -70,8 +69,8 @@ struct G2Timing {
     hipEvent_t ev[6];   // start, after pack, after rc, after dp, after tb, end
 };
 
-// Bytes of device workspace needed for a batch (host side sizing).
-size_t g2_sort_tmp_bytes(uint32_t n);
+// Shape buckets of the pair ordering (long flag, 8-bit query and target word classes).
+constexpr uint32_t G2_SHAPE_BUCKETS = 1u << 17;
 
 // Enqueue the whole device pipeline on `stream`.  Returns hipSuccess or the
 // first launch error.

@@ -16,7 +16,6 @@
 #include "engine.h"
 
 #include <string.h>
-#include <rocprim/rocprim.hpp>
 
 namespace {
 
@@ -193,19 +192,65 @@ __global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
 // ------------------------------------------------------------------ sort ----
 __device__ __forceinline__ bool census_fast(uint32_t m);
 
-// Sort key: long-pair flag (goes to the wavefront kernel), then query words,
-// then target words; sorted descending.  aux[2] counts the long pairs.
+// Pairs are ordered by a shape bucket, largest first, so that the 64 lanes of
+// a wave share loop bounds: a counting sort over G2_SHAPE_BUCKETS keys
+// (histogram, descending scan, scatter).  It has no memset or library call,
+// so the whole device program can be captured in a graph.  Results never
+// depend on the order (every output is per pair), only the load balance does.
+//
+// Key: long-pair flag (one wave per pair, taken first), then query words,
+// then target words, each as an 8-bit bucket: exact below 128 words, then 16
+// steps per octave (at most 1/16 apart) up to 2^15 words.
+__device__ __forceinline__ uint32_t shape8(uint32_t x) {
+    if (x < 128u) return x;
+    const uint32_t e = 31u - __builtin_clz(x);   // >= 7
+    const uint32_t v = 128u + (e - 7u) * 16u + ((x >> (e - 4u)) & 15u);
+    return v > 255u ? 255u : v;
+}
+
+// aux[2] counts the long pairs.
 __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
-    uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
+    const uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
     const bool lng = d.wf_blocks > 0 && census_fast(d.aux[0]) && tw >= d.wf_min_tw &&
                      (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
-    qw = qw > 0x7FFFu ? 0x7FFFu : qw;
-    tw = tw > 0xFFFFu ? 0xFFFFu : tw;
-    d.keys[k] = (lng ? 0x80000000u : 0u) | (qw << 16) | tw;
-    d.perm[k] = k;
+    const uint32_t key = (lng ? 0x10000u : 0u) | (shape8(qw) << 8) | shape8(tw);
+    d.keys[k] = key;
+    atomicAdd(&d.hist[key], 1u);
     if (lng) atomicAdd(&d.aux[2], 1u);
+}
+
+// Exclusive scan of the histogram in descending key order, in place: hist[b]
+// becomes the first sorted slot of bucket b.  One block of 1024 threads, each
+// owning G2_SHAPE_BUCKETS / 1024 consecutive buckets.
+__global__ __launch_bounds__(1024) void k_scan_desc(uint32_t *hist) {
+    constexpr uint32_t PER = G2_SHAPE_BUCKETS / 1024;
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t hi = G2_SHAPE_BUCKETS - 1 - t * PER;   // this thread's buckets: hi, hi-1, ..., hi-PER+1
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < PER; i++) sum += hist[hi - i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {   // Hillis-Steele inclusive scan
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint32_t i = 0; i < PER; i++) {
+        const uint32_t c = hist[hi - i];
+        hist[hi - i] = run;
+        run += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(G2Dev d) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.n) return;
+    d.perm[atomicAdd(&d.hist[d.keys[k]], 1u)] = k;
 }
 
 // ---------------------------------------------------------------- LOCAL -----
@@ -1253,13 +1298,6 @@ int g_dispatch = 0;
 void g2_set_dispatch(int policy) { g_dispatch = policy; }
 int g2_get_dispatch() { return g_dispatch; }
 
-size_t g2_sort_tmp_bytes(uint32_t n) {
-    size_t bytes = 0;
-    rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
-    (void)rocprim::radix_sort_pairs_desc((void *)nullptr, bytes, kb, vb, n, 0, 32, (hipStream_t)0);
-    return bytes + 256;
-}
-
 #define G2_LAUNCH_CHECK()                        \
     do {                                         \
         hipError_t e_ = hipGetLastError();       \
@@ -1310,30 +1348,23 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
         G2_LAUNCH_CHECK();
     }
     if (timing) (void)hipEventRecord(tm->ev[2], s);
-    // nibble census over the packed batch (aux[0]) for the LUT kernels; aux[1..2]
-    // are the long-pair queue head and count
+    // aux[0] nibble census over the packed batch for the score-table kernels;
+    // aux[1..2] long-pair queue head and count
+    k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
     if (d.fast) {
         const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
         const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
-        k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
         uint32_t cb = (qw + tw + 255) / 256;
         k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);
-        G2_LAUNCH_CHECK();
     }
+    G2_LAUNCH_CHECK();
     // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds;
     // long pairs for the one-wave-per-pair path sort ahead of all others
+    k_zero_u32<<<G2_SHAPE_BUCKETS / 256, 256, 0, s>>>(d.hist, G2_SHAPE_BUCKETS);
     k_keys<<<nb256, 256, 0, s>>>(d);
+    k_scan_desc<<<1, 1024, 0, s>>>(d.hist);
+    k_scatter<<<nb256, 256, 0, s>>>(d);
     G2_LAUNCH_CHECK();
-    {
-        rocprim::double_buffer<uint32_t> kb(d.keys, d.keys_alt), vb(d.perm, d.perm_alt);
-        size_t bytes = d.sort_tmp_bytes;
-        hipError_t e = rocprim::radix_sort_pairs_desc(d.sort_tmp, bytes, kb, vb, d.n, 0, 32, s);
-        if (e != hipSuccess) return e;
-        if (vb.current() != d.perm) {
-            hipError_t e2 = hipMemcpyAsync(d.perm, vb.current(), (size_t)d.n * 4, hipMemcpyDeviceToDevice, s);
-            if (e2 != hipSuccess) return e2;
-        }
-    }
     // 4. DP (dispatch table src/gasal_align.h:7-107)
     const uint32_t groups = (d.n + WAVE - 1) / WAVE;
     const dim3 grid(groups), blk(WAVE);

@@ -32,14 +32,24 @@ int g_error_mode = 0;
 int g_error_count = 0;
 // one-wave-per-pair thresholds: target words, cells (64 per query word x target word)
 uint32_t g_wf_min_tw = 48, g_wf_min_cells = 1u << 20, g_wf_max_blocks = 2048;
+int g_graph_mode = 1;   // capture the device program per batch shape and replay it
+
+// Tuned switch thresholds per (algo, start, second best, mean query length
+// range); filled by the autotuner (gasal2_tune_set), consulted per batch.
+struct TuneEntry {
+    int32_t algo, start, second;
+    uint32_t lq_lo, lq_hi;   // mean query length in [lq_lo, lq_hi)
+    uint32_t min_tw, min_cells, max_blocks;
+};
+std::vector<TuneEntry> g_tune;
+std::mutex g_tune_mu;
 std::string g_last_error;
 std::mutex g_err_mu;
 
 struct Impl {
-    uint32_t *keys = nullptr, *keys_alt = nullptr, *perm = nullptr, *perm_alt = nullptr;
+    uint32_t *keys = nullptr, *perm = nullptr;
     size_t n_cap = 0;
-    void *sort_tmp = nullptr;
-    size_t sort_tmp_bytes = 0;
+    uint32_t *hist = nullptr;
     uint32_t *bnd = nullptr;
     size_t bnd_bytes = 0;
     uint32_t *tb = nullptr;
@@ -62,6 +72,14 @@ struct Impl {
     bool last_tb = false;
     G2Timing tm;
     bool tm_init = false;
+    // captured device program of the last batch shape (see run_async)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    G2Dev g_dev;
+    G2Scores g_sc;
+    G2Opts g_op;
+    bool g_valid = false;
+    uint64_t graph_launches = 0, graph_captures = 0;
 };
 
 Impl *impl_of(gasal_gpu_storage_t *s) {
@@ -115,6 +133,28 @@ extern "C" void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty) {
     g_scores.np = penalty;
 }
 extern "C" void gasal2_set_dispatch(int32_t policy) { g2_set_dispatch(policy); }
+extern "C" void gasal2_tune_set(int32_t algo, int32_t start, int32_t second, uint32_t lq_lo, uint32_t lq_hi,
+                                uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (TuneEntry &e : g_tune)
+        if (e.algo == algo && e.start == start && e.second == (second ? 1 : 0) && e.lq_lo == lq_lo && e.lq_hi == lq_hi) {
+            e.min_tw = min_target_words ? min_target_words : 1u; e.min_cells = min_cells; e.max_blocks = max_blocks ? max_blocks : 1u;
+            return;
+        }
+    g_tune.push_back({algo, start, second ? 1 : 0, lq_lo, lq_hi, min_target_words ? min_target_words : 1u, min_cells,
+                      max_blocks ? max_blocks : 1u});
+}
+extern "C" void gasal2_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+}
+extern "C" int32_t gasal2_tune_count(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int32_t)g_tune.size();
+}
+
+extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
+extern "C" int32_t gasal2_get_graph_mode(void) { return g_graph_mode; }
 extern "C" void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks) {
     g_wf_min_tw = min_target_words ? min_target_words : 1u;
     g_wf_min_cells = min_cells;
@@ -668,12 +708,14 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 
 static void impl_free(Impl *m) {
     if (!m) return;
-    void *ptrs[] = {m->keys, m->keys_alt, m->perm, m->perm_alt, m->sort_tmp, m->bnd, m->tb, m->eh,
+    void *ptrs[] = {m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
                     m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (m->tm_init)
         for (int i = 0; i < 6; i++) (void)hipEventDestroy(m->tm.ev[i]);
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
     delete m;
 }
 
@@ -888,13 +930,13 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     size_t b = 0;
     size_t ncap = (size_t)n * sizeof(uint32_t);
     if (m->n_cap < n || !m->keys) {
-        for (uint32_t **pp : {&m->keys, &m->keys_alt, &m->perm, &m->perm_alt}) {
+        for (uint32_t **pp : {&m->keys, &m->perm}) {
             if (*pp) G2_CHECK(hipFree(*pp));
             G2_CHECK(hipMalloc((void **)pp, ncap + 256));
         }
         m->n_cap = n;
     }
-    ensure(m->sort_tmp, m->sort_tmp_bytes, g2_sort_tmp_bytes(n));
+    if (!m->hist) G2_CHECK(hipMalloc((void **)&m->hist, (size_t)G2_SHAPE_BUCKETS * 4));
     const uint32_t bnd_cols = 8 * qw + 8;
     ensure(m->bnd, m->bnd_bytes, (size_t)groups * bnd_cols * 64 * 4);
     const bool tb_on = op.start == WITH_TB;
@@ -942,8 +984,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     d.cigar = m->cigar;
     d.cigar_cap = cap;
     d.cigar_qlayout = tb_on ? m->cigar_q : nullptr;
-    d.keys = m->keys; d.keys_alt = m->keys_alt; d.perm = m->perm; d.perm_alt = m->perm_alt;
-    d.sort_tmp = m->sort_tmp; d.sort_tmp_bytes = m->sort_tmp_bytes;
+    d.keys = m->keys; d.perm = m->perm; d.hist = m->hist;
     d.bnd = m->bnd; d.bnd_cols = bnd_cols;
     d.tb = m->tb; d.tb_tw = tw; d.tb_qw8 = 8 * qw;
     d.eh = m->eh; d.eh_cols = eh_cols;
@@ -982,8 +1023,20 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     // census to allow the LUT kernels; the host count only sizes the launch.
     if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g2_get_dispatch() != 2) {
         const bool all = g2_get_dispatch() == 3;
-        d.wf_min_tw = all ? 1u : g_wf_min_tw;
-        d.wf_min_cells = all ? 0u : g_wf_min_cells;
+        uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
+        {
+            uint64_t sum_q = 0;
+            for (uint32_t k = 0; k < n; k++) sum_q += s->host_query_batch_lens[k];
+            const uint64_t mean_q = n ? sum_q / n : 0;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            for (const TuneEntry &e : g_tune)
+                if (e.algo == op.algo && e.start == op.start && e.second == op.second && mean_q >= e.lq_lo && mean_q < e.lq_hi) {
+                    min_tw = e.min_tw; min_cells = e.min_cells; max_blocks = e.max_blocks;
+                    break;
+                }
+        }
+        d.wf_min_tw = all ? 1u : min_tw;
+        d.wf_min_cells = all ? 0u : min_cells;
         uint32_t nlong = 0, ltw = 0;
         for (uint32_t k = 0; k < n; k++) {
             const uint32_t kq = g2_regs(s->host_query_batch_lens[k]), kt = g2_regs(s->host_target_batch_lens[k]);
@@ -993,7 +1046,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
             }
         }
         if (nlong) {
-            d.wf_blocks = std::min<uint32_t>(nlong, g_wf_max_blocks);
+            d.wf_blocks = std::min<uint32_t>(nlong, max_blocks);
             d.wf_stride = ((ltw + 63) / 64) * 17 * 64;
             if (8 * qw > 1024) {   // more than one column block: strip state is saved between blocks
                 ensure(m->wf, m->wf_bytes, (size_t)d.wf_blocks * d.wf_stride * 4);
@@ -1029,12 +1082,45 @@ void upload_async(gasal_gpu_storage_t *s, uint32_t n, Parameters *params) {
     }
 }
 
+// Graph mode: the device program (a dozen kernels) is captured once per batch
+// shape -- every kernel argument is in (G2Dev, G2Scores, G2Opts) -- and
+// replayed with one hipGraphLaunch while the shape repeats.  Workspace growth,
+// new n or byte counts, scores or dispatch settings change the arguments and
+// trigger a new capture.  The program holds kernels only (no memset or copy
+// nodes).  Timed runs (per-stage events) launch eagerly.
+hipError_t launch_program(Impl *m, const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t st) {
+    if (!g_graph_mode || m->tm.enabled) return g2_run(sc, op, d, st, &m->tm);
+    const bool same = m->g_valid && memcmp(&m->g_dev, &d, sizeof(d)) == 0 && memcmp(&m->g_sc, &sc, sizeof(sc)) == 0 &&
+                      memcmp(&m->g_op, &op, sizeof(op)) == 0;
+    if (!same) {
+        if (m->gexec) { (void)hipGraphExecDestroy(m->gexec); m->gexec = nullptr; }
+        if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+        m->g_valid = false;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        hipError_t er = g2_run(sc, op, d, st, nullptr);
+        hipGraph_t gr = nullptr;
+        e = hipStreamEndCapture(st, &gr);
+        if (er != hipSuccess) { if (gr) (void)hipGraphDestroy(gr); return er; }
+        if (e != hipSuccess) return e;
+        m->graph = gr;
+        e = hipGraphInstantiate(&m->gexec, gr, nullptr, nullptr, 0);
+        if (e != hipSuccess) return e;
+        m->g_dev = d; m->g_sc = sc; m->g_op = op;
+        m->g_valid = true;
+        m->graph_captures++;
+    }
+    m->graph_launches++;
+    return hipGraphLaunch(m->gexec, st);
+}
+
 void run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
     G2Dev d;
     prepare(s, qb, tb, n, params, d);
     Impl *m = impl_of(s);
     G2Scores sc = scores_now();
-    hipError_t e = g2_run(sc, opts_of(params), d, s->str, &m->tm);
+    G2Opts op = opts_of(params);
+    hipError_t e = launch_program(m, sc, op, d, s->str);
     if (e != hipSuccess) {
         report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
         gasal2_fail(EXIT_FAILURE);
@@ -1128,6 +1214,12 @@ extern "C" float gasal2_stage_ms(gasal_gpu_storage_t *s, int32_t stage) {
     if (hipEventSynchronize(m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
     if (hipEventElapsedTime(&ms, m->tm.ev[a[stage]], m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
     return ms;
+}
+
+extern "C" void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *launches) {
+    Impl *m = impl_of(s);
+    if (captures) *captures = m->graph_captures;
+    if (launches) *launches = m->graph_launches;
 }
 
 extern "C" uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint32_t cap) {

@@ -94,6 +94,28 @@ uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint
  *   3 one wave per pair for every LOCAL score-only pair the path accepts. */
 void gasal2_set_dispatch(int32_t policy);
 
+/* Tuned switch thresholds (written by the autotuner, gasal2_amd/autotune.py):
+ * for batches of mode (algo, start, second best) whose mean query length lies
+ * in [lq_lo, lq_hi), use these gasal2_set_wavefront values instead of the
+ * global ones.  Any setting gives identical results; only speed changes.
+ * No reference counterpart. */
+void gasal2_tune_set(int32_t algo, int32_t start, int32_t second, uint32_t lq_lo, uint32_t lq_hi,
+                     uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks);
+void gasal2_tune_clear(void);
+int32_t gasal2_tune_count(void);
+
+/* Graph mode (default on): each storage captures its device program (pack,
+ * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
+ * time a batch shape is seen and replays it with one launch while the shape
+ * (n, byte counts, workspace, scores, mode) repeats; H2D and D2H copies stay
+ * outside the graph.  Runs with per-stage timing enabled launch eagerly.
+ * gasal2_graph_stats reports captures and graph launches of a storage.  No
+ * reference counterpart (the reference launches each kernel,
+ * src/gasal_align.cu:180-264). */
+void gasal2_set_graph_mode(int32_t on);
+int32_t gasal2_get_graph_mode(void);
+void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *launches);
+
 /* Thresholds of the per-pair switch between one lane per pair and one wave per
  * pair (LOCAL without start/traceback/second best): a pair runs one wave per
  * pair when its target has at least min_target_words 8-base words and

@@ -133,3 +133,20 @@ def test_census_collision_keeps_lane_kernel():
     b = g.make_batch(qs, ts)
     with policy(dispatch=3):
         check(b)
+
+
+def test_autotune_choices_identical_results():
+    """Every switch setting the tuner tries gives the same outputs (the tuner
+    raises otherwise); the tuned table is then used by the library."""
+    from gasal2_amd import autotune
+    b = g.synth_batch(3, 1600, 1500)
+    p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START)
+    cands = [(16, 64 * 16 * 16, 64), (48, 1 << 20, 2048), (1, 0, 256), (1 << 20, 1 << 30, 1)]
+    r = autotune.tune(b, p, candidates=cands, repeats=1)
+    assert len(r["timings"]) == len(cands)
+    autotune.clear()
+    g.lib().gasal2_tune_set(g.LOCAL, g.WITHOUT_START, 0, 0, 1 << 31, *r["best"])
+    try:
+        check(b)
+    finally:
+        autotune.clear()
