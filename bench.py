@@ -53,7 +53,9 @@ def main():
                     help="oracle sample for cpu_baseline: -1 = sized to about --cpu-seconds, 0 = skip")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per DP launch")
+    ap.add_argument("--traffic", default=None,
+                    help="profile summary with a 'bytes_per_launch: N' line (measured HBM bytes per DP launch); "
+                         "default: the committed one for the default workload and mode")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,9 +142,14 @@ def main():
     log(f"gasal_aln_async incl. H2D/D2H: {pcie_s * 1e3:.1f} ms ({cells / pcie_s / 1e9:.1f} GCUPS)")
 
     traffic = None
-    if args.traffic and os.path.exists(args.traffic):
-        with open(args.traffic) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    tpath = args.traffic
+    if tpath is None and args.workload == "full" and mode == "local_tb_second" and args.pairs == 1_000_000:
+        tpath = os.path.join(ROOT, "profiles", "dp_traffic_local_tb_second_1M.md")
+    if tpath and os.path.exists(tpath):
+        with open(tpath) as f:
+            for ln in f:
+                if ln.startswith("bytes_per_launch:"):
+                    traffic = int(ln.split(":")[1])
 
     cpu = None
     if args.cpu_pairs != 0:

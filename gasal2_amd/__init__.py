@@ -172,6 +172,22 @@ _SIGS = {
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_tune_clear": (None, []),
     "gasal2_tune_count": (C.c_int32, []),
+    # learned band (include/gasal2_band.h): device pointers as void*
+    "gasal2_lb_features": (C.c_int32, [C.c_void_p] * 6 + [C.c_uint32] + [C.c_void_p] * 4),
+    "gasal2_lb_dense_fwd": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.c_void_p, C.c_int32, C.c_void_p]),
+    "gasal2_lb_dense_bwd": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_uint32, C.c_uint32,
+                                        C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
+    "gasal2_lb_mse": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    "gasal2_lb_adam": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
+    "gasal2_lb_to_bf16": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "gasal2_lb_band_from_net": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
+                                            C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gasal2_lb_band_peak": (C.c_int32, [C.c_void_p] * 5 + [C.c_uint32, C.c_float, C.c_float] + [C.c_void_p] * 3),
+    "gasal2_lb_path_diag": (C.c_int32, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 3 + [C.c_uint32] + [C.c_void_p] * 3),
     "gasal2_get_graph_mode": (C.c_int32, []),
     "gasal2_graph_stats": (None, [C.POINTER(gasal_gpu_storage_t), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "gasal2_device_info": (C.c_int32, [C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p, C.c_int32]),

@@ -169,7 +169,9 @@ extern "C" int32_t gasal2_device_info(int32_t *n_cu, int32_t *clock_khz, char *n
     if (n_cu) *n_cu = prop.multiProcessorCount;
     if (clock_khz) *clock_khz = prop.clockRate;
     if (name && name_cap > 0) {
-        strncpy(name, prop.name, (size_t)name_cap - 1);
+        // marketing name when the runtime reports one, else the ISA name (gfx950...)
+        const char *src = prop.name[0] ? prop.name : prop.gcnArchName;
+        strncpy(name, src, (size_t)name_cap - 1);
         name[name_cap - 1] = 0;
     }
     return 0;
@@ -991,36 +993,22 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     if (!m->aux) G2_CHECK(hipMalloc((void **)&m->aux, 256));
     d.aux = m->aux;
 
-    // LOCAL fast path eligibility: 4-bit signed substitution scores and a
-    // position encoding h << S | ~strip | ~column | row that fits 31 bits.
+    // Score-table kernels: 4-bit signed substitution scores.
     d.fast = 0;
     d.enc_ok = 0;
+    auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
+    const G2Scores &sc = g_scores;
     const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL;
     if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
-        auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
         auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
-        const G2Scores &sc = g_scores;
-        // the N score of each algorithm's substitution macro (src/gasal_kernels.h:37-56)
-        const int32_t nval = op.algo == GLOBAL ? (sc.has_np ? -sc.np : 0) : (sc.has_np ? -sc.np : 0);
-        const bool s4 = fits4(sc.match) && fits4(-sc.mismatch) && fits4(nval);
-        if (s4) d.fast = 1;
-        if (s4 && op.algo == LOCAL) {
-            const int cb = std::max(1, bl(8ull * qw - 1)), sbits = std::max(1, bl(tw - 1));
-            const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * qw, 8ull * tw);
-            const int hb = bl(maxh) + 1;
-            const int S = 3 + cb + sbits;
-            if (S + hb <= 31) {
-                d.enc_ok = 1;
-                d.enc_s = S;
-                d.enc_cb = cb;
-                d.enc_cm = (1u << cb) - 1u;
-                d.enc_sm = (1u << sbits) - 1u;
-            }
-        }
+        // the N score of the substitution macros (src/gasal_kernels.h:37-56)
+        const int32_t nval = sc.has_np ? -sc.np : 0;
+        if (fits4(sc.match) && fits4(-sc.mismatch) && fits4(nval)) d.fast = 1;
     }
     // Long LOCAL pairs (score and ends only) run one wave per pair.  The
     // device applies the same shape test in k_keys and also needs the nibble
-    // census to allow the LUT kernels; the host count only sizes the launch.
+    // census to allow the score tables; the host count only sizes the launch.
+    uint32_t eq = qw, et = tw;   // largest shape the one-lane-per-pair kernels see
     if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g2_get_dispatch() != 2) {
         const bool all = g2_get_dispatch() == 3;
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
@@ -1037,15 +1025,20 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         }
         d.wf_min_tw = all ? 1u : min_tw;
         d.wf_min_cells = all ? 0u : min_cells;
-        uint32_t nlong = 0, ltw = 0;
+        uint32_t nlong = 0, ltw = 0, sq = 0, st = 0;
         for (uint32_t k = 0; k < n; k++) {
             const uint32_t kq = g2_regs(s->host_query_batch_lens[k]), kt = g2_regs(s->host_target_batch_lens[k]);
             if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
                 nlong++;
                 ltw = std::max(ltw, kt);
+            } else {
+                sq = std::max(sq, kq);
+                st = std::max(st, kt);
             }
         }
         if (nlong) {
+            eq = std::max(sq, 1u);
+            et = std::max(st, 1u);
             d.wf_blocks = std::min<uint32_t>(nlong, max_blocks);
             d.wf_stride = ((ltw + 63) / 64) * 17 * 64;
             if (8 * qw > 1024) {   // more than one column block: strip state is saved between blocks
@@ -1055,6 +1048,22 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 d.wf_stride = 0;
                 d.wf_scratch = nullptr;
             }
+        }
+    }
+    // LOCAL lane-per-pair score-table kernel: position encoding
+    // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
+    // (all pairs, or those the one-wave path leaves).
+    if (d.fast && op.algo == LOCAL) {
+        const int cb = std::max(1, bl(8ull * eq - 1)), sbits = std::max(1, bl(et - 1));
+        const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * eq, 8ull * et);
+        const int hb = bl(maxh) + 1;
+        const int S = 3 + cb + sbits;
+        if (S + hb <= 31) {
+            d.enc_ok = 1;
+            d.enc_s = S;
+            d.enc_cb = cb;
+            d.enc_cm = (1u << cb) - 1u;
+            d.enc_sm = (1u << sbits) - 1u;
         }
     }
 }
