@@ -43,8 +43,11 @@ def main():
     ap.add_argument("--mode", default=None,
                     choices=["local_tb_second", "local_score", "global", "semi_global"],
                     help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
-    ap.add_argument("--workload", default="full", choices=["full", "uneven"],
-                    help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp")
+    ap.add_argument("--workload", default="full", choices=["full", "uneven", "band"],
+                    help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
+                         "band: learned band vs full DP on 1-10 kbp pairs (train seed 5, held-out seed 6)")
+    ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for --workload band")
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--dispatch", type=int, default=0,
                     help="kernel dispatch policy (0 auto, 2 one lane per pair only, 3 one wave per pair)")
     ap.add_argument("--compare-lane-only", action="store_true",
@@ -64,6 +67,9 @@ def main():
         sys.exit(2)
 
     import gasal2_amd as g
+
+    if args.workload == "band":
+        return band_main(args)
 
     t0 = time.time()
     if args.workload == "full":
@@ -219,6 +225,90 @@ def main():
         line["lane_only"] = lane_only
     print(json.dumps(line), flush=True)
     a.close()
+
+
+def band_main(args):
+    """Learned band vs plain full DP (LOCAL WITH_TB, scores 1/4/6/1) on
+    long divergent pairs: a training set (seed 5) labelled by the full DP's
+    traceback paths, a held-out set (seed 6) aligned three ways -- full DP,
+    learned band + fallback, histogram-peak band + fallback -- each compared
+    with the full DP on every output including the CIGAR.  GCUPS-equivalent =
+    full sum of Lq*Lt over the path's wall time (features + network + banded
+    DP + fallbacks for the banded paths)."""
+    import torch
+    import gasal2_amd as g
+    from gasal2_amd import learned_band as LB
+    n = args.band_pairs
+    t0 = time.time()
+    train = g.synth_batch(5, 5, n)
+    held = g.synth_batch(5, 6, n)
+    cells = float(np.dot(held.qlen.astype(np.int64), held.tlen.astype(np.int64)))
+    log(f"generated 2 x {n} pairs in {time.time() - t0:.1f}s; held-out {cells / 1e12:.3f} T cells")
+    # labels
+    ref_tr, _, (dmin, dmax) = LB.full_dp(train, labels=True)
+    feats, anchors, binws = [], [], []
+    for s0 in range(0, n, 16384):
+        sub = train.subset(np.arange(s0, min(n, s0 + 16384)))
+        a = LB._aligner_for(sub, g.WITH_TB, (1, 4, 6, 1), None)
+        try:
+            a.load(sub)
+            a.prep()
+            f, an, bw = LB.features(a, sub.n)
+            torch.cuda.synchronize()
+            feats.append(f.clone())
+            anchors.append(an.cpu().numpy())
+            binws.append(bw.cpu().numpy())
+        finally:
+            a.close()
+    feat = torch.cat(feats)
+    An, Bn = np.concatenate(anchors), np.concatenate(binws)
+    T, w = LB.make_targets(An, Bn, dmin, dmax, ref_tr["score"])
+    net = LB.BandNet(seed=0)
+    t0 = time.perf_counter()
+    hist = LB.train(net, feat, T, w, epochs=args.epochs, batch=1024, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    _, _, Y = net.forward(feat)
+    Yn = Y.cpu().numpy()
+    mul, add = LB.calibrate_learned(Yn, An, Bn, dmin, dmax, ref_tr["score"])
+    half = LB.calibrate_peak(feat.cpu().numpy(), An, Bn, train.qlen, train.tlen, dmin, dmax, ref_tr["score"])
+    log(f"trained in {train_s:.1f}s (loss {hist[0]:.3f} -> {hist[-1]:.3f}); learned widen x{mul:.2f}+{add}; "
+        f"peak half-width {half:.0f}")
+    del ref_tr, feat
+    # held-out
+    ref, full_s, _ = LB.full_dp(held)
+    log(f"full DP: {full_s:.2f}s ({cells / full_s / 1e9:.1f} GCUPS)")
+    out = {}
+    for name, prop in (("learned", LB.learned_proposer(net, mul, add)), ("peak", LB.peak_proposer(half))):
+        res, st = LB.band_align(held, prop)
+        agree, bad = LB.agreement(res, ref)
+        out[name] = {"gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
+                     "pairs_per_s": round(n / st["total_s"], 1), "hit_rate": round(st["hit_rate"], 5),
+                     "final_agreement": agree, "band_cells_fraction": round(st["band_cells_fraction"], 5),
+                     "fallback_pairs": st["fallback_pairs"], "flag_counts": st["flag_counts"],
+                     "propose_s": round(st["propose_s"], 3), "banded_s": round(st["banded_s"], 3),
+                     "fallback_s": round(st["fallback_s"], 3)}
+        log(f"{name}: {out[name]}")
+    line = {
+        "metric": "learned-band vs full-DP pairs/s at 100% agreement",
+        "value": out["learned"]["pairs_per_s"],
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * out["learned"]["seconds"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32 (DP), bf16/fp32 (network)", "data": "synthetic",
+        "config": {"workload": f"long divergent pairs: {n} train (seed 5) / {n} held-out (seed 6), Lq~U{{1000..10000}}, "
+                               "10-15% divergence, Poisson(1/kbp) indel bursts U{5..50}",
+                   "mode": "LOCAL WITH_TB", "scores": [1, 4, 6, 1], "held_out_cells": cells},
+        "full_dp": {"gcups": round(cells / full_s / 1e9, 2), "seconds": round(full_s, 3),
+                    "pairs_per_s": round(n / full_s, 1)},
+        "learned": out["learned"], "peak_band": out["peak"],
+        "speedup_vs_full": round(full_s / out["learned"]["seconds"], 3),
+        "speedup_peak_vs_full": round(full_s / out["peak"]["seconds"], 3),
+        "training": {"seconds": round(train_s, 2), "epochs": args.epochs, "loss_first": hist[0], "loss_last": hist[-1],
+                     "widen_mul": mul, "widen_add": add, "peak_half_width": half},
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -172,6 +172,13 @@ _SIGS = {
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_tune_clear": (None, []),
     "gasal2_tune_count": (C.c_int32, []),
+    "gasal2_prep_resident": (C.c_int32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p]),
+    "gasal2_result_views": (None, [C.POINTER(gasal_gpu_storage_t)] + [C.POINTER(C.c_void_p)] * 5 + [C.POINTER(C.c_uint32)]),
+    "gasal2_device_views": (None, [C.POINTER(gasal_gpu_storage_t)] + [C.POINTER(C.c_void_p)] * 6),
+    "gasal2_run_banded": (C.c_int32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_int32, C.c_float]),
+    "gasal2_run_subset": (C.c_int32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_uint64]),
     # learned band (include/gasal2_band.h): device pointers as void*
     "gasal2_lb_features": (C.c_int32, [C.c_void_p] * 6 + [C.c_uint32] + [C.c_void_p] * 4),
     "gasal2_lb_dense_fwd": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -184,9 +191,10 @@ _SIGS = {
     "gasal2_lb_adam": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_int32, C.c_void_p, C.c_void_p]),
     "gasal2_lb_to_bf16": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
-    "gasal2_lb_band_from_net": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
-                                            C.c_void_p, C.c_void_p, C.c_void_p]),
-    "gasal2_lb_band_peak": (C.c_int32, [C.c_void_p] * 5 + [C.c_uint32, C.c_float, C.c_float] + [C.c_void_p] * 3),
+    "gasal2_lb_band_from_net": (C.c_int32, [C.c_void_p] * 4 + [C.c_uint32, C.c_float, C.c_float, C.c_float]
+                                + [C.c_void_p] * 4),
+    "gasal2_lb_band_peak": (C.c_int32, [C.c_void_p] * 5 + [C.c_uint32, C.c_float, C.c_float, C.c_float]
+                            + [C.c_void_p] * 4),
     "gasal2_lb_path_diag": (C.c_int32, [C.c_void_p, C.c_uint32] + [C.c_void_p] * 3 + [C.c_uint32] + [C.c_void_p] * 3),
     "gasal2_get_graph_mode": (C.c_int32, []),
     "gasal2_graph_stats": (None, [C.POINTER(gasal_gpu_storage_t), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -537,6 +545,52 @@ class Aligner:
         qb, tb, n = self._loaded
         self._L.gasal2_download(C.byref(self.s), qb, n, self.params.native())
         return self._collect(n, qb)
+
+    # ---- learned-band path (gasal2_prep_resident / gasal2_run_banded / gasal2_run_subset)
+    def prep(self):
+        """Packs and reverse-complements the loaded batch on the device."""
+        qb, tb, n = self._loaded
+        r = self._L.gasal2_prep_resident(C.byref(self.s), qb, tb, n, self.params.native())
+        if r != 0:
+            raise RuntimeError(f"gasal2_prep_resident returned {r}")
+        self.sync()
+
+    def device_views(self):
+        """Device pointers (ints) of packed query/target words, offsets, lengths."""
+        ptrs = [C.c_void_p() for _ in range(6)]
+        self._L.gasal2_device_views(C.byref(self.s), *[C.byref(p) for p in ptrs])
+        return [p.value for p in ptrs]
+
+    def result_views(self):
+        """Device pointers (ints) of the last run's score, qend, tend, n_ops,
+        CIGAR slots, and the slot size."""
+        ptrs = [C.c_void_p() for _ in range(5)]
+        cap = C.c_uint32()
+        self._L.gasal2_result_views(C.byref(self.s), *[C.byref(p) for p in ptrs], C.byref(cap))
+        return [p.value for p in ptrs] + [cap.value]
+
+    def run_banded(self, dlo, dhi, edge_abs=24, edge_frac=0.125):
+        """Banded LOCAL inside per-pair bands; returns the per-pair flags."""
+        qb, tb, n = self._loaded
+        lo = np.ascontiguousarray(dlo, dtype=np.int32)
+        hi = np.ascontiguousarray(dhi, dtype=np.int32)
+        flags = np.zeros(n, np.uint8)
+        r = self._L.gasal2_run_banded(C.byref(self.s), qb, tb, n, self.params.native(), lo.ctypes.data,
+                                      hi.ctypes.data, flags.ctypes.data, int(edge_abs), float(edge_frac))
+        if r < 0:
+            raise RuntimeError(f"gasal2_run_banded returned {r}")
+        return flags
+
+    def run_subset(self, idx, max_tb_bytes=0):
+        """Full DP over pairs idx of the last run, results replacing theirs."""
+        qb, tb, n = self._loaded
+        ix = np.ascontiguousarray(idx, dtype=np.uint32)
+        if len(ix) == 0:
+            return
+        r = self._L.gasal2_run_subset(C.byref(self.s), qb, tb, n, self.params.native(), ix.ctypes.data, len(ix),
+                                      int(max_tb_bytes))
+        if r != 0:
+            raise RuntimeError(f"gasal2_run_subset returned {r}")
 
     def _collect(self, n, qb):
         s, p = self.s, self.params

@@ -43,9 +43,11 @@ __device__ __forceinline__ bool kmer_at(const uint32_t *W, int32_t i, uint32_t &
 
 __device__ __forceinline__ uint32_t slot_of(uint32_t code) { return (code * 2654435761u) >> 19; }   // 13 bits
 
-// One block per pair.  Query k-mers (every stride-th position, at most 4096)
-// go into an LDS hash table; every target k-mer is looked up and each hit
-// adds its diagonal d = j - i to a coarse histogram over [-Lq, Lt].  The
+// One block per pair.  Query k-mers (every stride-th position, at most 4096;
+// a repeated k-mer keeps its smallest position) go into an LDS hash table;
+// every target k-mer is looked up and each hit (query i, target j) adds its
+// diagonal d = i - j (query column - target row, the DP's convention) to a
+// coarse histogram over [-Lt, Lq].  The
 // anchor is the centre of the fullest coarse bin (lowest on ties); a second
 // scan bins hits into LB_FINE bins of width binw around it.
 // Features: 64 fine-bin fractions of all hits, log2(Lq+1)/14, log2(Lt+1)/14,
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(256) void k_features(const uint32_t *__restrict__ p
                                                   uint32_t n, float *__restrict__ feat, int32_t *__restrict__ anchor,
                                                   int32_t *__restrict__ binw_out) {
     __shared__ uint32_t keys[LB_SLOTS];
-    __shared__ uint16_t pos[LB_SLOTS];
+    __shared__ uint32_t pos[LB_SLOTS];
     __shared__ uint32_t coarse[LB_COARSE];
     __shared__ uint32_t fine[LB_FINE];
     __shared__ uint32_t total;
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_features(const uint32_t *__restrict__ p
     for (uint32_t pr = blockIdx.x; pr < n; pr += gridDim.x) {
         const int32_t Lq = (int32_t)qlen[pr], Lt = (int32_t)tlen[pr];
         const uint32_t *Q = pq + (qoff[pr] >> 3), *T = pt + (toff[pr] >> 3);
-        for (uint32_t s = t; s < LB_SLOTS; s += 256) keys[s] = LB_EMPTY;
+        for (uint32_t s = t; s < LB_SLOTS; s += 256) { keys[s] = LB_EMPTY; pos[s] = 0xFFFFFFFFu; }
         if (t < LB_COARSE) coarse[t] = 0;
         if (t < LB_FINE) fine[t] = 0;
         if (t == 0) total = 0;
@@ -78,8 +80,10 @@ __global__ __launch_bounds__(256) void k_features(const uint32_t *__restrict__ p
             uint32_t s = slot_of(code);
             for (int probe = 0; probe < LB_SLOTS; probe++) {
                 const uint32_t old = atomicCAS(&keys[s], LB_EMPTY, code);
-                if (old == LB_EMPTY) { pos[s] = (uint16_t)(i / stride); break; }
-                if (old == code) break;   // repeated k-mer: the first insert keeps its position
+                if (old == LB_EMPTY || old == code) {   // repeated k-mer: the smallest position wins
+                    atomicMin(&pos[s], (uint32_t)i);
+                    break;
+                }
                 s = (s + 1) & (LB_SLOTS - 1);
             }
         }
@@ -96,13 +100,13 @@ __global__ __launch_bounds__(256) void k_features(const uint32_t *__restrict__ p
                 for (int probe = 0; probe < LB_SLOTS; probe++) {
                     const uint32_t k = keys[s];
                     if (k == LB_EMPTY) break;
-                    if (k == code) { i = (int32_t)pos[s] * stride; break; }
+                    if (k == code) { i = (int32_t)pos[s]; break; }
                     s = (s + 1) & (LB_SLOTS - 1);
                 }
                 if (i < 0) continue;
-                const int32_t dgl = j - i;
+                const int32_t dgl = i - j;
                 if (pass == 0) {
-                    const int32_t cb = (int32_t)(((int64_t)(dgl + Lq) * LB_COARSE) / span);
+                    const int32_t cb = (int32_t)(((int64_t)(dgl + Lt) * LB_COARSE) / span);
                     atomicAdd(&coarse[cb], 1u);
                     atomicAdd(&total, 1u);
                 } else {
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(256) void k_features(const uint32_t *__restrict__ p
                 for (uint32_t b = 0; b < LB_COARSE; b++)
                     if (coarse[b] > best) { best = coarse[b]; bi = b; }
                 // centre of the coarse bin, in diagonal units
-                s_anchor = (int32_t)(((2 * (int64_t)bi + 1) * span) / (2 * LB_COARSE)) - Lq;
+                s_anchor = (int32_t)(((2 * (int64_t)bi + 1) * span) / (2 * LB_COARSE)) - Lt;
             }
             __syncthreads();
         }
@@ -244,18 +248,35 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ w, ui
     if (i < cnt) o[i] = f2bf(w[i]);
 }
 
+// Share of a pair's k-mer hits in fine bins that meet [lo, hi]; bin b holds
+// diagonals [anchor + (b - 32) * binw, anchor + (b - 31) * binw).
+__device__ __forceinline__ float hit_share(const float *f, int32_t anchor, int32_t bw, int32_t lo, int32_t hi) {
+    float in = 0.0f;
+    for (int b = 0; b < LB_FINE; b++) {
+        const int32_t b0 = anchor + (b - LB_FINE / 2) * bw, b1 = b0 + bw - 1;
+        if (b1 >= lo && b0 <= hi) in += f[b];
+    }
+    return in;
+}
+
 // Band from the network output (centre offset and log half-width in fine-bin
 // units): centre = anchor + binw * y0, half = binw * exp(y1) * mul + add.
-__global__ __launch_bounds__(256) void k_band_from_net(const float *__restrict__ Y, const int32_t *__restrict__ anchor,
+// pre[k] = 16 when less than min_share of the pair's k-mer hits lie in bins
+// the band meets (the band misses the evidence: full DP instead).
+__global__ __launch_bounds__(256) void k_band_from_net(const float *__restrict__ Y, const float *__restrict__ feat,
+                                                       const int32_t *__restrict__ anchor,
                                                        const int32_t *__restrict__ binw, uint32_t n, float mul, float add,
-                                                       int32_t *__restrict__ dlo, int32_t *__restrict__ dhi) {
+                                                       float min_share, int32_t *__restrict__ dlo,
+                                                       int32_t *__restrict__ dhi, uint8_t *__restrict__ pre) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float bw = (float)binw[k];
     const float c = (float)anchor[k] + bw * Y[2 * k];
     const float h = bw * expf(fminf(Y[2 * k + 1], 20.0f)) * mul + add;
-    dlo[k] = (int32_t)floorf(c - h);
-    dhi[k] = (int32_t)ceilf(c + h);
+    const int32_t lo = (int32_t)floorf(c - h), hi = (int32_t)ceilf(c + h);
+    dlo[k] = lo;
+    dhi[k] = hi;
+    pre[k] = hit_share(feat + (size_t)k * LB_NFEAT, anchor[k], binw[k], lo, hi) < min_share ? 16 : 0;
 }
 
 // Non-learned band: centre of the fullest fine bin (lowest on ties), fixed
@@ -263,7 +284,8 @@ __global__ __launch_bounds__(256) void k_band_from_net(const float *__restrict__
 __global__ __launch_bounds__(256) void k_band_peak(const float *__restrict__ feat, const int32_t *__restrict__ anchor,
                                                    const int32_t *__restrict__ binw, const uint32_t *__restrict__ qlen,
                                                    const uint32_t *__restrict__ tlen, uint32_t n, float half, float frac,
-                                                   int32_t *__restrict__ dlo, int32_t *__restrict__ dhi) {
+                                                   float min_share, int32_t *__restrict__ dlo, int32_t *__restrict__ dhi,
+                                                   uint8_t *__restrict__ pre) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float *f = feat + (size_t)k * LB_NFEAT;
@@ -274,8 +296,10 @@ __global__ __launch_bounds__(256) void k_band_peak(const float *__restrict__ fea
     const float c = (float)anchor[k] + (float)binw[k] * ((float)(bi - LB_FINE / 2) + 0.5f);
     const float m = (float)(qlen[k] < tlen[k] ? qlen[k] : tlen[k]);
     const float h = half + frac * m;
-    dlo[k] = (int32_t)floorf(c - h);
-    dhi[k] = (int32_t)ceilf(c + h);
+    const int32_t lo = (int32_t)floorf(c - h), hi = (int32_t)ceilf(c + h);
+    dlo[k] = lo;
+    dhi[k] = hi;
+    pre[k] = hit_share(f, anchor[k], binw[k], lo, hi) < min_share ? 16 : 0;
 }
 
 // Diagonal range (d = query column - target row) of each pair's alignment path,
@@ -357,18 +381,20 @@ extern "C" int32_t gasal2_lb_to_bf16(const float *w, uint16_t *out, uint32_t cnt
     return (int32_t)hipGetLastError();
 }
 
-extern "C" int32_t gasal2_lb_band_from_net(const float *Y, const int32_t *anchor, const int32_t *binw, uint32_t n, float mul,
-                                           float add, int32_t *dlo, int32_t *dhi, void *stream) {
+extern "C" int32_t gasal2_lb_band_from_net(const float *Y, const float *feat, const int32_t *anchor, const int32_t *binw,
+                                           uint32_t n, float mul, float add, float min_share, int32_t *dlo, int32_t *dhi,
+                                           uint8_t *pre, void *stream) {
     if (n == 0) return 0;
-    k_band_from_net<<<nblk(n), 256, 0, (hipStream_t)stream>>>(Y, anchor, binw, n, mul, add, dlo, dhi);
+    k_band_from_net<<<nblk(n), 256, 0, (hipStream_t)stream>>>(Y, feat, anchor, binw, n, mul, add, min_share, dlo, dhi, pre);
     return (int32_t)hipGetLastError();
 }
 
 extern "C" int32_t gasal2_lb_band_peak(const float *feat, const int32_t *anchor, const int32_t *binw, const uint32_t *qlen,
-                                       const uint32_t *tlen, uint32_t n, float half, float frac, int32_t *dlo, int32_t *dhi,
-                                       void *stream) {
+                                       const uint32_t *tlen, uint32_t n, float half, float frac, float min_share,
+                                       int32_t *dlo, int32_t *dhi, uint8_t *pre, void *stream) {
     if (n == 0) return 0;
-    k_band_peak<<<nblk(n), 256, 0, (hipStream_t)stream>>>(feat, anchor, binw, qlen, tlen, n, half, frac, dlo, dhi);
+    k_band_peak<<<nblk(n), 256, 0, (hipStream_t)stream>>>(feat, anchor, binw, qlen, tlen, n, half, frac, min_share, dlo,
+                                                          dhi, pre);
     return (int32_t)hipGetLastError();
 }
 

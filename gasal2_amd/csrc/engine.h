@@ -20,6 +20,9 @@ struct G2Scores {
 // Algorithm switches (Parameters fields, src/args_parser.h:30-48).
 struct G2Opts {
     int32_t algo, start, second, head, tail, k_band, is_packed, is_rc;
+    int32_t banded;      // LOCAL inside per-pair diagonal bands (k_local_band)
+    int32_t skip_prep;   // batch already packed and reverse-complemented on the device
+    int32_t prep_only;   // pack and reverse-complement only
 };
 
 // Device view of one batch plus the workspace it runs in.
@@ -62,6 +65,17 @@ struct G2Dev {
     uint32_t wf_blocks, wf_min_tw, wf_min_cells;
     uint32_t *wf_scratch;            // per wavefront block: per pass 17 x 64 dwords of strip state
     uint32_t wf_stride;              // dwords per wavefront block
+    // banded LOCAL (k_local_band): per-pair diagonal band [band_lo, band_hi]
+    // of d = query column - target row; direction tiles of pair k start at
+    // 32-byte chunk band_tbo[k], band_bw[k] chunks per strip.
+    const int32_t *band_lo, *band_hi;
+    const uint64_t *band_tbo;
+    const uint32_t *band_bw;
+    uint8_t *band_flags;             // per pair: bit0 edge contact, bit1 walk left the band, bit2 edge score
+    int32_t band_edge_abs;           // edge-score flag: max H on edge tiles >= max(abs, frac * score)
+    float band_edge_frac;
+    // pairs of a sub-batch run (gasal2_run_subset): results go to slot sub_idx[k]
+    const uint32_t *sub_idx;
 };
 
 struct G2Timing {
@@ -75,6 +89,10 @@ constexpr uint32_t G2_SHAPE_BUCKETS = 1u << 17;
 // Enqueue the whole device pipeline on `stream`.  Returns hipSuccess or the
 // first launch error.
 hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t stream, G2Timing *tm);
+
+// Copies the results of a sub-batch run (pairs sub.sub_idx[k]) into the main
+// batch's result arrays and CIGAR slots.
+hipError_t g2_scatter_subset(const G2Dev &sub, const G2Dev &main, bool tb, hipStream_t stream);
 
 // Dispatch policy for LOCAL (0 auto, 1 reference-order kernel only).
 void g2_set_dispatch(int policy);

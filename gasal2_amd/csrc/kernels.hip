@@ -756,6 +756,116 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     }
 }
 
+// ----------------------------------------------------- LOCAL, diagonal band ----
+// Learned-band DP: k_local_fast's recurrences restricted, per pair, to the 8x8
+// tiles that meet the diagonal band d = column - row in [band_lo, band_hi]:
+// strip s covers query words [(8s + lo) >> 3, ((8s + 7 + hi) >> 3] clipped to
+// the matrix.  A cell outside the computed tiles counts as H = 0 with no gap
+// (E, F = BAND_NEG): every banded value is then the best over paths inside the
+// tiles, a lower bound of the full-matrix value.  The matrix's own top row and
+// first column keep the reference boundary (H = 0, E = F = 0,
+// src/kernels/local_kernel_template.h:103,118-127).  Best cell: first strict
+// maximum in (strip, column, row) order, as in k_local.
+// Flags (band_flags): bit0 the best cell is on an edge tile (a tile with a
+// neighbour outside the band), bit2 some edge-tile cell has
+// H >= max(edge_abs, edge_frac * score), bit3 not run (score tables refused).
+constexpr int32_t BAND_NEG = -16000;
+
+__device__ __forceinline__ void band_words(int32_t s, int32_t lo, int32_t hi, int32_t QW, int32_t &wlo, int32_t &whi) {
+    wlo = (8 * s + lo) >> 3;            // arithmetic shift: floor
+    whi = ((8 * s + 7 + hi) >> 3) + 1;
+    wlo = wlo < 0 ? 0 : wlo;
+    whi = whi > QW ? QW : whi;
+}
+
+template <bool TB>
+__global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
+    const uint32_t census = d.aux[0];
+    LaneCtx x;
+    if (!lane_setup(d, x)) return;
+    const uint32_t pr = x.pair;
+    if (!census_fast(census)) { d.band_flags[pr] = 8; return; }
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    const int32_t lo = d.band_lo[pr], hi = d.band_hi[pr];
+    uint32_t *__restrict__ bd = x.bd;
+    uint32_t *tbp = TB ? d.tb + d.band_tbo[pr] * 8 : nullptr;
+    const int32_t bw = TB ? (int32_t)d.band_bw[pr] : 0;
+    int32_t rv = 0, rs = 0, rc = 0;   // best: h << 3 | (8 - row in strip), strip, column
+    int32_t emax = 0;                 // best encoded h on edge tiles
+    bool best_edge = false, overflow = false;
+    int32_t plo = 0, phi = 0;         // word range of the previous strip
+    for (int32_t st = 0; st < TW; st++) {
+        int32_t wlo, whi;
+        band_words(st, lo, hi, QW, wlo, whi);
+        if (wlo >= whi) { plo = phi = 0; continue; }
+        int32_t h[9], f[9];
+        const uint32_t gpac = x.T[st];
+        uint32_t lut[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = wlo == 0 ? 0 : BAND_NEG; }
+        // H(strip top - 1, first column - 1), if the previous strip computed it
+        if (st > 0 && wlo > 0 && wlo - 1 >= plo && wlo - 1 < phi) h[0] = sx16(bd[(8 * wlo - 1) * WAVE]);
+        for (int32_t w = wlo; w < whi; w++) {
+            const uint32_t rpac = x.Q[w];
+            const bool top_in = st == 0 || (w >= plo && w < phi);
+            const bool edge = (w == wlo && wlo > 0) || (w == whi - 1 && whi < QW) || !top_in;
+            uint32_t dirw[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const int32_t col = 8 * w + kk;
+                const uint32_t qsh = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                const uint32_t cur = st == 0 ? 0u : (top_in ? bd[col * WAVE] : pk16(0, BAND_NEG));
+                int32_t e = (int32_t)cur >> 16;
+                int32_t p = h[0];
+                h[0] = sx16(cur);
+                uint32_t dir = 0;
+                int32_t colbest = 0;
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t sc = __builtin_amdgcn_sbfe(lut[m], qsh, 4);
+                    const int32_t tmp = p + sc;
+                    const int32_t hm = max(max(max(tmp, f[m]), e), 0);
+                    const int32_t t = tmp - gapoe;
+                    const int32_t fx = f[m] - gape, ex = e - gape;
+                    if (TB) {
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+                    }
+                    f[m] = max(t, fx);
+                    e = max(t, ex);
+                    const int32_t en = (hm << 3) | (8 - m);
+                    colbest = m == 1 ? en : max(colbest, en);
+                    p = h[m];
+                    h[m] = hm;
+                }
+                bd[col * WAVE] = pk16(h[8], e);
+                if (colbest > (rv | 7)) { rv = colbest; rs = st; rc = col; best_edge = edge; }
+                if (edge) emax = max(emax, colbest);
+                if (TB) dirw[kk] = dir;
+            }
+            if (TB) {
+                if (w - wlo < bw) tb_store(tbp + ((size_t)st * bw + (size_t)(w - wlo)) * 8, dirw);
+                else overflow = true;
+            }
+        }
+        plo = wlo;
+        phi = whi;
+    }
+    const int32_t v = rv >> 3;
+    d.score[pr] = v;
+    d.qend[pr] = v > 0 ? rc : 0;
+    d.tend[pr] = v > 0 ? rs * 8 + (7 - (rv & 7)) : 0;
+    const int32_t eh = emax >> 3;
+    const int32_t thr = max(d.band_edge_abs, (int32_t)ceilf(d.band_edge_frac * (float)v));
+    d.band_flags[pr] = (uint8_t)((best_edge && v > 0 ? 1u : 0u) | (eh >= thr && eh > 0 ? 4u : 0u) | (overflow ? 8u : 0u));
+}
+
 // --------------------------------------------------------------- GLOBAL -----
 // gasal_global_kernel<S> (src/kernels/global.h:30-303).
 // LUT: substitution from a per-strip 4-bit score table (see k_local_fast),
@@ -1182,7 +1292,13 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int32_t QW = dregs(read_len), TW = dregs(ref_len);
     // LOCAL words written by k_local_fast carry the four comparisons as
     // "not" bits; the match/mismatch label comes from the bases.
-    const bool fastfmt = LOCALT && d.fast && d.enc_ok && census_fast(d.aux[0]);
+    // banded runs (k_local_band) store band-compressed tiles per pair
+    const bool band = LOCALT && d.band_lo != nullptr;
+    const int32_t blo = band ? d.band_lo[pr] : 0, bhi = band ? d.band_hi[pr] : 0;
+    const int32_t bbw = band ? (int32_t)d.band_bw[pr] : 0;
+    uint32_t bflag = 0;
+    if (band && (d.band_flags[pr] & 8u)) { d.n_ops[pr] = 0; return; }
+    const bool fastfmt = LOCALT && (band || (d.fast && d.enc_ok && census_fast(d.aux[0])));
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1207,7 +1323,18 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
         if (strip < TW) {
             const int32_t key = strip * QW + (col >> 3);
             if (key != ch_key) {
-                const uint4 *src = reinterpret_cast<const uint4 *>(tb_chunk(d, grp, lane, strip, col >> 3));
+                const uint32_t *chunk;
+                if (band) {
+                    int32_t wlo, whi;
+                    band_words(strip, blo, bhi, QW, wlo, whi);
+                    const int32_t wv = col >> 3;
+                    if (wv < wlo || wv >= whi || wv - wlo >= bbw) { bflag |= 2u; break; }   // walk left the band
+                    if ((wv == wlo && wlo > 0) || (wv == whi - 1 && whi < QW)) bflag |= 1u;
+                    chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
+                } else {
+                    chunk = tb_chunk(d, grp, lane, strip, col >> 3);
+                }
+                const uint4 *src = reinterpret_cast<const uint4 *>(chunk);
                 const uint4 a = src[0], b = src[1];
                 ch[0] = a.x; ch[1] = a.y; ch[2] = a.z; ch[3] = a.w;
                 ch[4] = b.x; ch[5] = b.y; ch[6] = b.z; ch[7] = b.w;
@@ -1256,6 +1383,7 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     }
     if (LOCALT) { d.tstart[pr] = i; d.qstart[pr] = j; }
     d.n_ops[pr] = (uint32_t)n_ops;
+    if (band && bflag) d.band_flags[pr] |= (uint8_t)bflag;
 }
 
 // Copies each pair's CIGAR to the reference's layout (bytes at the pair's query
@@ -1291,9 +1419,37 @@ __global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, uint32_t n) {
     if (k < n) p[k] = 0;
 }
 
+// One thread per pair of a sub-batch; CIGAR slots copied as 4-byte words
+// (slot sizes are multiples of 16 bytes).
+__global__ __launch_bounds__(256) void k_sub_scatter(G2Dev sb, G2Dev mn, int tb) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= sb.n) return;
+    const uint32_t dst = sb.sub_idx[k];
+    mn.score[dst] = sb.score[k];
+    mn.qend[dst] = sb.qend[k];
+    mn.tend[dst] = sb.tend[k];
+    mn.qstart[dst] = sb.qstart[k];
+    mn.tstart[dst] = sb.tstart[k];
+    if (tb) {
+        const uint32_t nops = sb.n_ops[k];
+        mn.n_ops[dst] = nops;
+        const uint32_t nb = nops < sb.cigar_cap ? nops : sb.cigar_cap;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(sb.cigar + (size_t)k * sb.cigar_cap);
+        uint32_t *out = reinterpret_cast<uint32_t *>(mn.cigar + (size_t)dst * mn.cigar_cap);
+        const uint32_t nw = (nb + 3) / 4 < mn.cigar_cap / 4 ? (nb + 3) / 4 : mn.cigar_cap / 4;
+        for (uint32_t w = 0; w < nw; w++) out[w] = src[w];
+    }
+}
+
 int g_dispatch = 0;
 
 }  // namespace
+
+hipError_t g2_scatter_subset(const G2Dev &sub, const G2Dev &main, bool tb, hipStream_t stream) {
+    if (sub.n == 0) return hipSuccess;
+    k_sub_scatter<<<(sub.n + 255) / 256, 256, 0, stream>>>(sub, main, tb ? 1 : 0);
+    return hipGetLastError();
+}
 
 void g2_set_dispatch(int policy) { g_dispatch = policy; }
 int g2_get_dispatch() { return g_dispatch; }
@@ -1326,7 +1482,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const bool timing = tm && tm->enabled;
     if (timing) (void)hipEventRecord(tm->ev[0], s);
     // 1. pack (src/gasal_align.cu:190-201)
-    if (!op.is_packed) {
+    if (!op.is_packed && !op.skip_prep) {
         auto packside = [&](const uint8_t *u, uint32_t *p, uint32_t bytes) {
             uint32_t n16 = bytes / 16;
             uint32_t tail = (bytes % 16) ? 1u : 0u;
@@ -1343,10 +1499,11 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     if (timing) (void)hipEventRecord(tm->ev[1], s);
     const uint32_t nb256 = (d.n + 255) / 256;
     // 2. reverse / complement (src/gasal_align.cu:232-246)
-    if (op.is_rc) {
+    if (op.is_rc && !op.skip_prep) {
         k_rc<<<(d.n + 15) / 16, 256, 0, s>>>(d, sc.n_code);
         G2_LAUNCH_CHECK();
     }
+    if (op.prep_only) return hipSuccess;
     if (timing) (void)hipEventRecord(tm->ev[2], s);
     // aux[0] nibble census over the packed batch for the score-table kernels;
     // aux[1..2] long-pair queue head and count
@@ -1370,7 +1527,11 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const dim3 grid(groups), blk(WAVE);
     const bool tb = op.start == S_WITH_TB;
     const bool second = op.second != 0;
-    switch (op.algo) {
+    switch (op.banded ? -1 : op.algo) {
+        case -1:   // per-pair diagonal bands (score tables only; the caller checks d.fast)
+            if (tb) k_local_band<true><<<grid, blk, 0, s>>>(d, sc);
+            else k_local_band<false><<<grid, blk, 0, s>>>(d, sc);
+            break;
         case A_LOCAL:
             if (d.fast) {
                 if (op.start == S_WITH_TB) {

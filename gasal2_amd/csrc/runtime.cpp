@@ -80,6 +80,24 @@ struct Impl {
     G2Opts g_op;
     bool g_valid = false;
     uint64_t graph_launches = 0, graph_captures = 0;
+    // banded runs (gasal2_run_banded) and sub-batches (gasal2_run_subset)
+    int32_t *band_lo = nullptr, *band_hi = nullptr;
+    uint32_t *band_bw = nullptr;
+    uint64_t *band_tbo = nullptr;
+    uint8_t *band_flags = nullptr;
+    size_t band_cap = 0;
+    std::vector<int32_t> h_lo, h_hi;
+    std::vector<uint32_t> h_bw;
+    std::vector<uint64_t> h_tbo;
+    uint32_t *sub_u32 = nullptr;     // qoff, toff, qlen, tlen, idx: 5 x sub_cap
+    int32_t *sub_res = nullptr;      // score, qend, tend, qstart, tstart: 5 x sub_cap
+    uint32_t *sub_nops = nullptr;
+    uint8_t *sub_cigar = nullptr;
+    size_t sub_cap = 0, sub_cigar_bytes = 0;
+    std::vector<uint32_t> h_sub;     // 6 x chunk: qoff, toff, qlen, tlen, idx (+ spare)
+    G2Dev main_dev;                  // device view of the last full or banded run
+    bool main_valid = false;
+    bool prepped = false;            // resident batch packed/reverse-complemented by gasal2_prep_resident
 };
 
 Impl *impl_of(gasal_gpu_storage_t *s) {
@@ -710,7 +728,9 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 
 static void impl_free(Impl *m) {
     if (!m) return;
-    void *ptrs[] = {m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
+    void *ptrs[] = {m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
+                    m->sub_nops, m->sub_cigar,
+                    m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
                     m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -914,18 +934,30 @@ G2Opts opts_of(const Parameters *p) {
     o.k_band = p->k_band;
     o.is_packed = p->isPacked ? 1 : 0;
     o.is_rc = p->isReverseComplement ? 1 : 0;
+    o.banded = 0;
+    o.skip_prep = 0;
+    o.prep_only = 0;
     return o;
 }
 
 // Sizes the workspace for this batch and fills the device view.
-void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const Parameters *params, G2Dev &d) {
+struct PrepExtra {
+    const uint32_t *hq = nullptr, *ht = nullptr;   // host lengths of the pairs run (default: the storage's)
+    bool banded = false;                            // direction tiles sized per band by the caller
+    bool sub = false;                               // sub-batch: result, CIGAR and scratch buffers come from the caller
+};
+
+void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const Parameters *params, G2Dev &d,
+             const PrepExtra &px = PrepExtra()) {
     Impl *m = impl_of(s);
+    const uint32_t *hq = px.hq ? px.hq : s->host_query_batch_lens;
+    const uint32_t *ht = px.ht ? px.ht : s->host_target_batch_lens;
     uint32_t mq = 0, mt = 0;
     for (uint32_t k = 0; k < n; k++) {
-        mq = std::max(mq, s->host_query_batch_lens[k]);
-        mt = std::max(mt, s->host_target_batch_lens[k]);
+        mq = std::max(mq, hq[k]);
+        mt = std::max(mt, ht[k]);
     }
-    m->last_max_q = mq; m->last_max_t = mt; m->last_n = n;
+    if (!px.sub) { m->last_max_q = mq; m->last_max_t = mt; m->last_n = n; }
     const G2Opts op = opts_of(params);
     const uint32_t qw = g2_regs(mq), tw = g2_regs(mt);
     const uint32_t groups = (n + 63) / 64;
@@ -943,19 +975,21 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     ensure(m->bnd, m->bnd_bytes, (size_t)groups * bnd_cols * 64 * 4);
     const bool tb_on = op.start == WITH_TB;
     const bool tb_dp = tb_on && (op.algo == LOCAL || op.algo == GLOBAL);
-    if (tb_dp) ensure(m->tb, m->tb_bytes, (size_t)groups * tw * (8 * qw) * 64 * 4);
+    if (tb_dp && !px.banded) ensure(m->tb, m->tb_bytes, (size_t)groups * tw * (8 * qw) * 64 * 4);
     const uint32_t eh_cols = mq + 2;
     if (op.algo == KSW) ensure(m->eh, m->eh_bytes, (size_t)groups * eh_cols * 64 * 8);
     const uint32_t cap = g2_cigar_capacity(mq, mt);
-    if (tb_on) {
+    if (tb_on && !px.sub) {
         ensure(m->nops, m->nops_cap, (size_t)n * 4);
         ensure(m->cigar, m->cigar_bytes, (size_t)n * cap);
         ensure(m->cigar_q, m->cigar_q_bytes, (size_t)qb + 16);
     }
-    ensure(m->scratch_i32, m->scratch_i32_cap, (size_t)n * 4 * 8);
+    if (!px.sub) ensure(m->scratch_i32, m->scratch_i32_cap, (size_t)n * 4 * 8);
     (void)b;
-    m->last_cigar_cap = cap;
-    m->last_tb = tb_on;
+    if (!px.sub) {
+        m->last_cigar_cap = cap;
+        m->last_tb = tb_on;
+    }
     s->packed_tb_matrices = (uint4 *)m->tb;
     s->packed_tb_matrix_size = m->tb_bytes / 16;
 
@@ -1014,7 +1048,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
         {
             uint64_t sum_q = 0;
-            for (uint32_t k = 0; k < n; k++) sum_q += s->host_query_batch_lens[k];
+            for (uint32_t k = 0; k < n; k++) sum_q += hq[k];
             const uint64_t mean_q = n ? sum_q / n : 0;
             std::lock_guard<std::mutex> lk(g_tune_mu);
             for (const TuneEntry &e : g_tune)
@@ -1027,7 +1061,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         d.wf_min_cells = all ? 0u : min_cells;
         uint32_t nlong = 0, ltw = 0, sq = 0, st = 0;
         for (uint32_t k = 0; k < n; k++) {
-            const uint32_t kq = g2_regs(s->host_query_batch_lens[k]), kt = g2_regs(s->host_target_batch_lens[k]);
+            const uint32_t kq = g2_regs(hq[k]), kt = g2_regs(ht[k]);
             if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
                 nlong++;
                 ltw = std::max(ltw, kt);
@@ -1127,6 +1161,8 @@ void run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Par
     G2Dev d;
     prepare(s, qb, tb, n, params, d);
     Impl *m = impl_of(s);
+    m->main_dev = d;
+    m->main_valid = true;
     G2Scores sc = scores_now();
     G2Opts op = opts_of(params);
     hipError_t e = launch_program(m, sc, op, d, s->str);
@@ -1188,6 +1224,7 @@ extern "C" int gasal_is_aln_async_done(gasal_gpu_storage_t *gpu_storage) {
 
 extern "C" void gasal2_upload(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
     if (!validate(s, qb, tb, n)) return;
+    impl_of(s)->prepped = false;
     grow_device(s, qb, tb, n, params);
     upload_async(s, n, params);
     G2_CHECK(hipStreamSynchronize(s->str));
@@ -1223,6 +1260,218 @@ extern "C" float gasal2_stage_ms(gasal_gpu_storage_t *s, int32_t stage) {
     if (hipEventSynchronize(m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
     if (hipEventElapsedTime(&ms, m->tm.ev[a[stage]], m->tm.ev[b[stage]]) != hipSuccess) return -1.0f;
     return ms;
+}
+
+// ------------------------------------------------ banded runs and fallback ---
+// Packs and reverse-complements a resident batch (gasal2_upload) in place, so
+// that device-side consumers (gasal2_lb_features) can read the packed words
+// (gasal2_device_views) before gasal2_run_banded.
+extern "C" int32_t gasal2_prep_resident(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    if (!validate(s, qb, tb, n)) return -1;
+    Impl *m = impl_of(s);
+    G2Dev d;
+    PrepExtra px;
+    px.banded = true;   // no direction-tile workspace for this call
+    prepare(s, qb, tb, n, params, d, px);
+    G2Opts op = opts_of(params);
+    op.prep_only = 1;
+    hipError_t e = g2_run(scores_now(), op, d, s->str, nullptr);
+    if (e != hipSuccess) {
+        report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
+        gasal2_fail(EXIT_FAILURE);
+        return -1;
+    }
+    m->prepped = true;
+    return 0;
+}
+
+// Device pointers of the resident batch: packed query/target words, offsets
+// and lengths (valid until the storage grows or is destroyed).
+extern "C" void gasal2_device_views(gasal_gpu_storage_t *s, void **pq, void **pt, void **qoff, void **toff, void **qlen,
+                                    void **tlen) {
+    if (pq) *pq = s->packed_query_batch;
+    if (pt) *pt = s->packed_target_batch;
+    if (qoff) *qoff = s->query_batch_offsets;
+    if (toff) *toff = s->target_batch_offsets;
+    if (qlen) *qlen = s->query_batch_lens;
+    if (tlen) *tlen = s->target_batch_lens;
+}
+
+// Device pointers of the last run's results: score, query end, target end,
+// n_ops and per-pair CIGAR slots of *cap bytes (reverse order).
+extern "C" void gasal2_result_views(gasal_gpu_storage_t *s, void **score, void **qend, void **tend, void **nops,
+                                    void **cigar, uint32_t *cap) {
+    Impl *m = impl_of(s);
+    const G2Dev &d = m->main_dev;
+    if (score) *score = m->main_valid ? (void *)d.score : nullptr;
+    if (qend) *qend = m->main_valid ? (void *)d.qend : nullptr;
+    if (tend) *tend = m->main_valid ? (void *)d.tend : nullptr;
+    if (nops) *nops = m->main_valid ? (void *)d.n_ops : nullptr;
+    if (cigar) *cigar = m->main_valid ? (void *)d.cigar : nullptr;
+    if (cap) *cap = m->main_valid ? d.cigar_cap : 0;
+}
+
+// Banded LOCAL over per-pair diagonal bands (learned-band path): the batch
+// must be resident and prepared (gasal2_upload, gasal2_prep_resident).  Results stay in device memory like
+// gasal2_run_resident; flags (host, n bytes) come back synchronously:
+// bit0 best cell on the band edge, bit1 traceback left the band, bit2 a
+// high score reached the band edge, bit3 not run (score tables refused).
+// Any flagged pair should be re-run with gasal2_run_subset.
+extern "C" int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
+                                     const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t edge_abs,
+                                     float edge_frac) {
+    if (!validate(s, qb, tb, n)) return -1;
+    if (params->algo != LOCAL || params->secondBest || params->start_pos == WITH_START) {
+        report("[GASAL ERROR:] banded runs take LOCAL WITHOUT_START or WITH_TB without second best\n");
+        gasal2_fail(EXIT_FAILURE);
+        return -1;
+    }
+    Impl *m = impl_of(s);
+    PrepExtra px;
+    px.banded = true;
+    G2Dev d;
+    prepare(s, qb, tb, n, params, d, px);
+    m->main_dev = d;
+    m->main_valid = true;
+    if (!m->prepped) {
+        report("[GASAL ERROR:] gasal2_run_banded needs gasal2_prep_resident on the uploaded batch first\n");
+        gasal2_fail(EXIT_FAILURE);
+        return -1;
+    }
+    if (!d.fast) {   // substitution scores outside the 4-bit tables: every pair goes to the full DP
+        if (flags) memset(flags, 8, n);
+        return 1;
+    }
+    const bool tbo_on = params->start_pos == WITH_TB;
+    m->h_lo.assign(dlo, dlo + n);
+    m->h_hi.assign(dhi, dhi + n);
+    m->h_bw.resize(n);
+    m->h_tbo.resize(n);
+    uint64_t tot = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const int64_t QW = g2_regs(s->host_query_batch_lens[k]), TW = g2_regs(s->host_target_batch_lens[k]);
+        const int64_t span = (int64_t)dhi[k] - (int64_t)dlo[k];
+        int64_t bw = span < 0 ? 1 : ((span + 14) >> 3) + 1;   // >= words any strip of the band touches
+        bw = std::max<int64_t>(1, std::min<int64_t>(bw, QW));
+        m->h_bw[k] = (uint32_t)bw;
+        m->h_tbo[k] = tot;
+        if (tbo_on) tot += (uint64_t)TW * (uint64_t)bw;
+    }
+    if (m->band_cap < n || !m->band_lo) {
+        for (void **pp : {(void **)&m->band_lo, (void **)&m->band_hi, (void **)&m->band_bw, (void **)&m->band_tbo,
+                          (void **)&m->band_flags})
+            if (*pp) G2_CHECK(hipFree(*pp));
+        G2_CHECK(hipMalloc((void **)&m->band_lo, (size_t)n * 4 + 256));
+        G2_CHECK(hipMalloc((void **)&m->band_hi, (size_t)n * 4 + 256));
+        G2_CHECK(hipMalloc((void **)&m->band_bw, (size_t)n * 4 + 256));
+        G2_CHECK(hipMalloc((void **)&m->band_tbo, (size_t)n * 8 + 256));
+        G2_CHECK(hipMalloc((void **)&m->band_flags, (size_t)n + 256));
+        m->band_cap = n;
+    }
+    G2_CHECK(hipMemcpyAsync(m->band_lo, m->h_lo.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(m->band_hi, m->h_hi.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(m->band_bw, m->h_bw.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->str));
+    G2_CHECK(hipMemcpyAsync(m->band_tbo, m->h_tbo.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->str));
+    if (tbo_on) {
+        ensure(m->tb, m->tb_bytes, (size_t)tot * 32 + 256);
+        d.tb = m->tb;
+    }
+    d.band_lo = m->band_lo; d.band_hi = m->band_hi; d.band_bw = m->band_bw; d.band_tbo = m->band_tbo;
+    d.band_flags = m->band_flags;
+    d.band_edge_abs = edge_abs;
+    d.band_edge_frac = edge_frac;
+    m->main_dev = d;
+    G2Opts op = opts_of(params);
+    op.banded = 1;
+    op.skip_prep = 1;
+    hipError_t e = launch_program(m, scores_now(), op, d, s->str);
+    if (e != hipSuccess) {
+        report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
+        gasal2_fail(EXIT_FAILURE);
+        return -1;
+    }
+    if (flags) G2_CHECK(hipMemcpyAsync(flags, m->band_flags, n, hipMemcpyDeviceToHost, s->str));
+    G2_CHECK(hipStreamSynchronize(s->str));
+    return 0;
+}
+
+// Full DP for pairs idx[0..cnt) of the last run (banded or full) on this
+// storage, results written over theirs in device memory (the batch stays
+// packed and reverse-complemented on the device).  Pairs go in chunks whose
+// direction-tile workspace stays under max_tb_bytes (0: 32 GiB).
+extern "C" int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
+                                     const uint32_t *idx, uint32_t cnt, uint64_t max_tb_bytes) {
+    Impl *m = impl_of(s);
+    if (!m->main_valid || m->main_dev.n != n) {
+        report("[GASAL ERROR:] gasal2_run_subset needs a preceding run of the same batch on this storage\n");
+        gasal2_fail(EXIT_FAILURE);
+        return -1;
+    }
+    if (cnt == 0) return 0;
+    for (uint32_t k = 0; k < cnt; k++)
+        if (idx[k] >= n) { report("[GASAL ERROR:] gasal2_run_subset: pair index %u >= %u\n", idx[k], n); gasal2_fail(EXIT_FAILURE); return -1; }
+    if (!max_tb_bytes) max_tb_bytes = 32ull << 30;
+    const bool tb_on = params->start_pos == WITH_TB;
+    const G2Dev main = m->main_dev;
+    G2Opts op = opts_of(params);
+    op.skip_prep = 1;
+    uint32_t k0 = 0;
+    while (k0 < cnt) {
+        // grow the chunk while the full-matrix tile workspace fits
+        uint32_t k1 = k0, mq = 0, mt = 0;
+        while (k1 < cnt) {
+            const uint32_t q = std::max(mq, g2_regs(s->host_query_batch_lens[idx[k1]]));
+            const uint32_t t = std::max(mt, g2_regs(s->host_target_batch_lens[idx[k1]]));
+            const uint64_t groups = (k1 - k0 + 1 + 63) / 64;
+            if (k1 > k0 && tb_on && groups * t * (8ull * q) * 64 * 4 > max_tb_bytes) break;
+            mq = q; mt = t; k1++;
+        }
+        const uint32_t c = k1 - k0;
+        m->h_sub.resize((size_t)c * 5);
+        uint32_t *hq_off = m->h_sub.data(), *ht_off = hq_off + c, *hq = ht_off + c, *ht = hq + c, *hidx = ht + c;
+        for (uint32_t k = 0; k < c; k++) {
+            const uint32_t p = idx[k0 + k];
+            hq_off[k] = s->host_query_batch_offsets[p];
+            ht_off[k] = s->host_target_batch_offsets[p];
+            hq[k] = s->host_query_batch_lens[p];
+            ht[k] = s->host_target_batch_lens[p];
+            hidx[k] = p;
+        }
+        G2Dev ds;
+        PrepExtra px;
+        px.hq = hq; px.ht = ht; px.sub = true;
+        prepare(s, qb, tb, c, params, ds, px);
+        if (m->sub_cap < c || !m->sub_u32) {
+            for (void **pp : {(void **)&m->sub_u32, (void **)&m->sub_res, (void **)&m->sub_nops})
+                if (*pp) G2_CHECK(hipFree(*pp));
+            G2_CHECK(hipMalloc((void **)&m->sub_u32, (size_t)c * 5 * 4 + 256));
+            G2_CHECK(hipMalloc((void **)&m->sub_res, (size_t)c * 5 * 4 + 256));
+            G2_CHECK(hipMalloc((void **)&m->sub_nops, (size_t)c * 4 + 256));
+            m->sub_cap = c;
+        }
+        G2_CHECK(hipMemcpyAsync(m->sub_u32, m->h_sub.data(), (size_t)c * 5 * 4, hipMemcpyHostToDevice, s->str));
+        ds.qoff = m->sub_u32; ds.toff = m->sub_u32 + c; ds.qlen = m->sub_u32 + 2 * c; ds.tlen = m->sub_u32 + 3 * c;
+        ds.sub_idx = m->sub_u32 + 4 * c;
+        ds.score = m->sub_res; ds.qend = m->sub_res + c; ds.tend = m->sub_res + 2 * c;
+        ds.qstart = m->sub_res + 3 * c; ds.tstart = m->sub_res + 4 * c;
+        ds.s_score = ds.s_qend = ds.s_tend = nullptr;
+        if (tb_on) {
+            ensure(m->sub_cigar, m->sub_cigar_bytes, (size_t)c * ds.cigar_cap);
+            ds.cigar = m->sub_cigar;
+            ds.n_ops = m->sub_nops;
+            ds.cigar_qlayout = main.cigar_qlayout;   // the reference layout sits at the pairs' own query offsets
+        }
+        hipError_t e = g2_run(scores_now(), op, ds, s->str, nullptr);
+        if (e == hipSuccess) e = g2_scatter_subset(ds, main, tb_on, s->str);
+        if (e != hipSuccess) {
+            report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
+            gasal2_fail(EXIT_FAILURE);
+            return -1;
+        }
+        G2_CHECK(hipStreamSynchronize(s->str));   // host staging arrays are reused by the next chunk
+        k0 = k1;
+    }
+    return 0;
 }
 
 extern "C" void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *launches) {

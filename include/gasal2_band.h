@@ -23,7 +23,7 @@ extern "C" {
 
 /* Per pair: LB_NFEAT features (fractions of k=11 hits in 64 diagonal bins of
  * width binw[k] around anchor[k], the fullest of 256 coarse bins over
- * [-Lq, Lt]; log2(Lq+1)/14, log2(Lt+1)/14, (Lt-Lq)/(Lt+Lq), hits/min(Lq,Lt)).
+ * [-Lt, Lq]; a hit at query i, target j has diagonal i - j; log2(Lq+1)/14, log2(Lt+1)/14, (Lt-Lq)/(Lt+Lq), hits/min(Lq,Lt)).
  * pq/pt: packed batches (8 bases per word, base 0 in bits 31:28); qoff/toff in
  * bases (multiples of 8). */
 int32_t gasal2_lb_features(const uint32_t *pq, const uint32_t *pt, const uint32_t *qoff, const uint32_t *toff,
@@ -53,15 +53,17 @@ int32_t gasal2_lb_to_bf16(const float *w, uint16_t *out, uint32_t cnt, void *str
 
 /* Band from the network output Y[n][2] (centre offset, log half-width in fine
  * bins): [floor(c - h), ceil(c + h)], c = anchor + binw*y0,
- * h = binw*exp(y1)*mul + add. */
-int32_t gasal2_lb_band_from_net(const float *Y, const int32_t *anchor, const int32_t *binw, uint32_t n, float mul,
-                                float add, int32_t *dlo, int32_t *dhi, void *stream);
+ * h = binw*exp(y1)*mul + add.  pre[k] = 16 when the fine bins the band meets
+ * hold less than min_share of the pair's k-mer hits (send it to the full DP). */
+int32_t gasal2_lb_band_from_net(const float *Y, const float *feat, const int32_t *anchor, const int32_t *binw,
+                                uint32_t n, float mul, float add, float min_share, int32_t *dlo, int32_t *dhi,
+                                uint8_t *pre, void *stream);
 
 /* Non-learned band: centre of the fullest fine bin, half-width
- * half + frac*min(Lq, Lt). */
+ * half + frac*min(Lq, Lt); pre as above. */
 int32_t gasal2_lb_band_peak(const float *feat, const int32_t *anchor, const int32_t *binw, const uint32_t *qlen,
-                            const uint32_t *tlen, uint32_t n, float half, float frac, int32_t *dlo, int32_t *dhi,
-                            void *stream);
+                            const uint32_t *tlen, uint32_t n, float half, float frac, float min_share, int32_t *dlo,
+                            int32_t *dhi, uint8_t *pre, void *stream);
 
 /* Diagonal range of each pair's traceback path from its CIGAR (per-pair
  * slots of cap bytes, reverse order) and end cell (labels for training). */

@@ -104,6 +104,36 @@ void gasal2_tune_set(int32_t algo, int32_t start, int32_t second, uint32_t lq_lo
 void gasal2_tune_clear(void);
 int32_t gasal2_tune_count(void);
 
+/* Learned-band path (no reference counterpart; see gasal2_band.h).
+ *
+ * gasal2_prep_resident: pack and reverse-complement an uploaded batch
+ * (gasal2_upload) in place; gasal2_device_views then gives the device
+ * pointers of the packed words, offsets and lengths.
+ *
+ * gasal2_run_banded: LOCAL (WITHOUT_START or WITH_TB, no second best) inside
+ * per-pair diagonal bands d = query column - target row in [dlo[k], dhi[k]]
+ * (host arrays), over the 8x8 tiles the band touches.  Results stay in device
+ * memory (gasal2_download fetches them); flags[k] (host) come back set:
+ * bit0 best cell on a band-edge tile, bit1 traceback left the band, bit2 an
+ * edge-tile cell scored >= max(edge_abs, edge_frac * score), bit3 not run.
+ * Returns 0, or 1 when the substitution scores do not fit the 4-bit score
+ * tables (every pair flagged), -1 on error.
+ *
+ * gasal2_run_subset: full DP for pairs idx[0..cnt) of the last run on this
+ * storage, written over their results; chunked so the full-matrix direction
+ * tiles stay under max_tb_bytes (0: 32 GiB). */
+/* Device pointers of the last run's results on this storage (score, query
+ * end, target end, n_ops, CIGAR slots of *cap bytes), for device-side
+ * consumers such as gasal2_lb_path_diag. */
+void gasal2_result_views(gasal_gpu_storage_t *s, void **score, void **qend, void **tend, void **nops, void **cigar,
+                         uint32_t *cap);
+int32_t gasal2_prep_resident(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params);
+void gasal2_device_views(gasal_gpu_storage_t *s, void **pq, void **pt, void **qoff, void **toff, void **qlen, void **tlen);
+int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
+                          const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t edge_abs, float edge_frac);
+int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
+                          const uint32_t *idx, uint32_t cnt, uint64_t max_tb_bytes);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape

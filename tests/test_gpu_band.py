@@ -1,0 +1,184 @@
+"""GPU tests of the learned-band path: k-mer features and the bf16 network's
+kernels against their CPU references, the banded DP (a band covering the
+matrix equals the full DP; narrow bands plus full-DP fallback equal the
+oracle on every pair, CIGARs included), and a small trained pipeline."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import gasal2_amd as g
+import lbref as R
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def long_batch(n, seed):
+    return g.synth_batch(5, seed, n)
+
+
+def loaded(batch, start=g.WITH_TB):
+    p = g.Parameters(algo=g.LOCAL, start_pos=start, secondBest=g.FALSE)
+    a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, n_penalty=1)
+    a.load(batch)
+    a.prep()
+    return a
+
+
+def oracle(batch, start):
+    return O.align(batch, g.LOCAL, start_pos=start, second_best=False, scores=(1, 4, 6, 1), n_penalty=1,
+                   with_cigar=start == g.WITH_TB)
+
+
+def check_all(a, batch, start):
+    got = a.download()
+    ref = oracle(batch, start)
+    keys = ["score", "qend", "tend"] + (["qstart", "tstart"] if start == g.WITH_TB else [])
+    for k in keys:
+        bad = np.nonzero(got[k] != ref[k])[0]
+        assert len(bad) == 0, f"{k}: pairs {bad[:8]} gpu {got[k][bad[:4]]} oracle {ref[k][bad[:4]]}"
+    if start == g.WITH_TB:
+        assert np.array_equal(got["n_ops"], ref["n_ops"])
+        for k in range(batch.n):
+            assert a.cigar(k) == ref["cigars"][k], f"CIGAR of pair {k}"
+
+
+def test_features_match_reference():
+    from gasal2_amd import learned_band as LB
+    b0 = long_batch(10, 2100)
+    rng = np.random.default_rng(1)
+    q = bytes(rng.choice(list(b"ACGT"), 1500).tolist())
+    extra = [(q, q), (q, bytes(rng.choice(list(b"ACGT"), 200).tolist()) + q), (b"ACGTACGTAN" * 40, b"ACGTACGTAN" * 40),
+             (b"ACG", b"ACGTTT"), (q[:5000], q[:20])]
+    qs = [b0.query(i) for i in range(b0.n)] + [x for x, _ in extra]
+    ts = [b0.target(i) for i in range(b0.n)] + [y for _, y in extra]
+    b = g.make_batch(qs, ts)
+    a = loaded(b)
+    try:
+        feat, anchor, binw = LB.features(a, b.n)
+        torch.cuda.synchronize()
+        F, A, W = feat.cpu().numpy(), anchor.cpu().numpy(), binw.cpu().numpy()
+    finally:
+        a.close()
+    for k in range(b.n):
+        f, an, bw = R.features(qs[k], ts[k])
+        assert A[k] == an and W[k] == bw, (k, A[k], an, W[k], bw)
+        np.testing.assert_allclose(F[k], f, rtol=2e-6, atol=1e-7, err_msg=f"pair {k}")
+
+
+def test_dense_layers_match_reference():
+    rng = np.random.default_rng(5)
+    L = g.lib()
+    B, K, N = 37, 68, 256
+    X = rng.normal(0, 1, (B, K)).astype(np.float32)
+    Wf = rng.normal(0, 0.2, (N, K)).astype(np.float32)
+    b = rng.normal(0, 0.1, N).astype(np.float32)
+    dY = rng.normal(0, 1, (B, N)).astype(np.float32)
+    Xd, bd, dYd = (torch.from_numpy(v).cuda() for v in (X, b, dY))
+    Wbf = torch.from_numpy(R.bf16_bits(Wf).view(np.int16)).cuda()
+    Y = torch.empty((B, N), dtype=torch.float32, device="cuda")
+    assert L.gasal2_lb_dense_fwd(_p(Xd), B, K, _p(Wbf), _p(bd), N, _p(Y), 1, None) == 0
+    dZ = torch.empty_like(Y)
+    dX = torch.empty_like(Xd)
+    dW = torch.empty((N, K), dtype=torch.float32, device="cuda")
+    db = torch.empty(N, dtype=torch.float32, device="cuda")
+    assert L.gasal2_lb_dense_bwd(_p(dYd), _p(Y), 1, _p(Xd), B, K, _p(Wbf), N, _p(dZ), _p(dX), _p(dW), _p(db), None) == 0
+    torch.cuda.synchronize()
+    Yr = R.dense_fwd(X, Wf, b, True)
+    np.testing.assert_allclose(Y.cpu().numpy(), Yr, rtol=1e-4, atol=1e-4)
+    # backward on the kernel's own activations (the ReLU mask must agree)
+    Yg = Y.cpu().numpy().astype(np.float64)
+    _, dXr, dWr, dbr = R.dense_bwd(dY.astype(np.float64), Yg, True, X, Wf)
+    np.testing.assert_allclose(dX.cpu().numpy(), dXr, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(dW.cpu().numpy(), dWr, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(db.cpu().numpy(), dbr, rtol=1e-4, atol=1e-4)
+
+
+def test_network_gradients_and_adam_match_reference():
+    from gasal2_amd import learned_band as LB
+    rng = np.random.default_rng(9)
+    net = LB.BandNet(seed=3)
+    X = rng.normal(0, 1, (64, 68)).astype(np.float32)
+    T = rng.normal(0, 1, (64, 2)).astype(np.float32)
+    w = (rng.uniform(0, 1, 64) > 0.2).astype(np.float32)
+    params = net.numpy_params()
+    loss = net.gradients(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda(), torch.from_numpy(w).cuda())
+    torch.cuda.synchronize()
+    lr, grads = R.mlp_grads(params, X.astype(np.float64), T.astype(np.float64), w.astype(np.float64))
+    assert abs(float(loss.item()) - lr) <= 1e-3 * max(1.0, abs(lr))
+    for gg, gr in zip(net.g, grads):
+        np.testing.assert_allclose(gg.cpu().numpy(), gr, rtol=2e-3, atol=2e-4)
+    # one Adam step
+    w0 = [p.copy() for p in params]
+    net.train_step(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda(), torch.from_numpy(w).cuda(), lr=1e-2)
+    torch.cuda.synchronize()
+    for p0, gg, wn in zip(w0, net.g, net.w):
+        ref, _, _ = R.adam(p0, np.zeros_like(p0), np.zeros_like(p0), gg.cpu().numpy().astype(np.float64), 1e-2, 0.9,
+                           0.999, 1e-8, 1)
+        np.testing.assert_allclose(wn.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("start", [g.WITH_TB, g.WITHOUT_START])
+def test_band_covering_the_matrix_equals_full_dp(start):
+    b1 = long_batch(8, 2200)
+    b2 = g.synth_batch(2, 2201, 40)
+    qs = [b1.query(i) for i in range(b1.n)] + [b2.query(i) for i in range(b2.n)]
+    ts = [b1.target(i) for i in range(b1.n)] + [b2.target(i) for i in range(b2.n)]
+    b = g.make_batch(qs, ts)
+    a = loaded(b, start)
+    try:
+        QW = (b.qlen.astype(np.int64) + 7) // 8
+        TW = (b.tlen.astype(np.int64) + 7) // 8
+        flags = a.run_banded(-8 * TW - 8, 8 * QW + 8)
+        assert not flags.any(), flags
+        check_all(a, b, start)
+    finally:
+        a.close()
+
+
+@pytest.mark.parametrize("half", [4, 24, 96, 400])
+def test_narrow_bands_with_fallback_equal_oracle(half):
+    from gasal2_amd import learned_band as LB
+    b = long_batch(14, 2300 + half)
+    a = loaded(b)
+    try:
+        lo, hi, pre = LB.peak_proposer(float(half))(a, b.n)
+        flags = a.run_banded(lo, hi) | pre
+        a.run_subset(np.nonzero(flags)[0])
+        check_all(a, b, g.WITH_TB)
+        if half == 4:
+            assert flags.sum() > 0   # a 9-diagonal band cannot hold these paths
+    finally:
+        a.close()
+
+
+def test_learned_pipeline_small():
+    from gasal2_amd import learned_band as LB
+    train = long_batch(96, 2400)
+    held = long_batch(24, 2401)
+    ref_tr, _, (dmin, dmax) = LB.full_dp(train, n_penalty=1, labels=True)
+    a = loaded(train)
+    try:
+        feat, anchor, binw = LB.features(a, train.n)
+        torch.cuda.synchronize()
+    finally:
+        a.close()
+    An, Bn = anchor.cpu().numpy(), binw.cpu().numpy()
+    T, w = LB.make_targets(An, Bn, dmin, dmax, ref_tr["score"])
+    net = LB.BandNet(seed=1)
+    hist = LB.train(net, feat, T, w, epochs=30, batch=32, lr=3e-3)
+    assert hist[-1] < hist[0]
+    _, _, Y = net.forward(feat)
+    mul, add = LB.calibrate_learned(Y.cpu().numpy(), An, Bn, dmin, dmax, ref_tr["score"])
+    res, stats = LB.band_align(held, LB.learned_proposer(net, mul, add), n_penalty=1)
+    ref = oracle(held, g.WITH_TB)
+    for k in ("score", "qend", "tend", "qstart", "tstart", "n_ops"):
+        assert np.array_equal(res[k], ref[k]), k
+    assert res["cigars"] == ref["cigars"]
