@@ -176,7 +176,7 @@ _SIGS = {
     "gasal2_result_views": (None, [C.POINTER(gasal_gpu_storage_t)] + [C.POINTER(C.c_void_p)] * 5 + [C.POINTER(C.c_uint32)]),
     "gasal2_device_views": (None, [C.POINTER(gasal_gpu_storage_t)] + [C.POINTER(C.c_void_p)] * 6),
     "gasal2_run_banded": (C.c_int32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_void_p, C.c_int32, C.c_float]),
+                                      C.c_void_p, C.c_int32]),
     "gasal2_run_subset": (C.c_int32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                       C.c_uint64]),
     # learned band (include/gasal2_band.h): device pointers as void*
@@ -569,14 +569,14 @@ class Aligner:
         self._L.gasal2_result_views(C.byref(self.s), *[C.byref(p) for p in ptrs], C.byref(cap))
         return [p.value for p in ptrs] + [cap.value]
 
-    def run_banded(self, dlo, dhi, edge_abs=24, edge_frac=0.125):
+    def run_banded(self, dlo, dhi, check_widen=512):
         """Banded LOCAL inside per-pair bands; returns the per-pair flags."""
         qb, tb, n = self._loaded
         lo = np.ascontiguousarray(dlo, dtype=np.int32)
         hi = np.ascontiguousarray(dhi, dtype=np.int32)
         flags = np.zeros(n, np.uint8)
         r = self._L.gasal2_run_banded(C.byref(self.s), qb, tb, n, self.params.native(), lo.ctypes.data,
-                                      hi.ctypes.data, flags.ctypes.data, int(edge_abs), float(edge_frac))
+                                      hi.ctypes.data, flags.ctypes.data, int(check_widen))
         if r < 0:
             raise RuntimeError(f"gasal2_run_banded returned {r}")
         return flags

@@ -65,15 +65,17 @@ struct G2Dev {
     uint32_t wf_blocks, wf_min_tw, wf_min_cells;
     uint32_t *wf_scratch;            // per wavefront block: per pass 17 x 64 dwords of strip state
     uint32_t wf_stride;              // dwords per wavefront block
+    uint32_t *wf_tb;                 // WITH_TB: direction tiles of the long pairs, per pair [strip][word] x 8 dwords
+    const uint64_t *wf_tbo;          // first 32-byte tile of each long pair in wf_tb
     // banded LOCAL (k_local_band): per-pair diagonal band [band_lo, band_hi]
     // of d = query column - target row; direction tiles of pair k start at
     // 32-byte chunk band_tbo[k], band_bw[k] chunks per strip.
     const int32_t *band_lo, *band_hi;
     const uint64_t *band_tbo;
     const uint32_t *band_bw;
-    uint8_t *band_flags;             // per pair: bit0 edge contact, bit1 walk left the band, bit2 edge score
-    int32_t band_edge_abs;           // edge-score flag: max H on edge tiles >= max(abs, frac * score)
-    float band_edge_frac;
+    uint8_t *band_flags;             // per pair: bit0 edge contact, bit1 walk left the band, bit2 check band disagrees
+    int32_t band_widen;              // check band: [lo - widen, hi + widen], score and ends only (0: none)
+    int32_t *band_wres;              // check band results: score, query end, target end (3 x n)
     // pairs of a sub-batch run (gasal2_run_subset): results go to slot sub_idx[k]
     const uint32_t *sub_idx;
 };

@@ -213,8 +213,7 @@ __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
     const uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
-    const bool lng = d.wf_blocks > 0 && census_fast(d.aux[0]) && tw >= d.wf_min_tw &&
-                     (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
+    const bool lng = d.wf_blocks > 0 && tw >= d.wf_min_tw && (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
     const uint32_t key = (lng ? 0x10000u : 0u) | (shape8(qw) << 8) | shape8(tw);
     d.keys[k] = key;
     atomicAdd(&d.hist[key], 1u);
@@ -262,7 +261,7 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
     if (d.fast && d.enc_ok && census_fast(d.aux[0])) return;   // k_local_fast handles this batch
     LaneCtx x;
     // long pairs of a score-only batch sort first and run one wave per pair
-    if (!lane_setup(d, x, 0, d.wf_blocks && census_fast(d.aux[0]) ? d.aux[2] : 0u)) return;
+    if (!lane_setup(d, x, 0, d.wf_blocks ? d.aux[2] : 0u)) return;
     constexpr bool TB = START == S_WITH_TB;
     const int32_t nval = c.has_np ? -c.np : 0;
     const int32_t gapoe = c.gapoe, gape = c.gape;
@@ -477,6 +476,10 @@ __device__ __forceinline__ bool wf_before(int32_t av, int32_t as, int32_t ac, in
     return ac < bc;
 }
 
+// TB: direction words as in k_local_fast; each lane keeps its last eight
+// columns in a window and stores the 32-byte tile when a column closes an 8x8
+// tile, to the pair's own region wf_tb + wf_tbo[pair] laid out [strip][word].
+template <bool TB>
 __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     __shared__ uint32_t ring[WF_W];
     const int32_t l = (int32_t)threadIdx.x;
@@ -490,6 +493,9 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     // lanes take part in the atomic; only lane 0 adds, so lane 0 reads the
     // counter before its own increment.
     const uint32_t nlong = __builtin_amdgcn_readfirstlane(d.aux[2]);
+    // score tables when the batch's nibble classes allow them, else the
+    // substitution rule per cell (the query base travels instead of its class)
+    const bool use_lut = census_fast(census);
     for (;;) {
         const uint32_t item = __builtin_amdgcn_readfirstlane(atomicAdd(&d.aux[1], l == 0 ? 1u : 0u));
         if (item >= nlong) break;
@@ -502,6 +508,7 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
         const int32_t C = 8 * QW;
         const int32_t npass = (TW + 63) / 64;
         const int32_t nblk = (C + WF_W - 1) / WF_W;
+        uint32_t *tbp = TB ? d.wf_tb + d.wf_tbo[pr] * 8 : nullptr;
         WfBest lb = {0, 0x7FFFFFFF, 0x7FFFFFFF};
         for (int32_t b = 0; b < nblk; b++) {
             const int32_t c0 = b * WF_W;
@@ -519,11 +526,15 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                     for (int m = 1; m < 9; m++) { h[m] = (int32_t)st[(m - 1) * 64]; f[m] = (int32_t)st[(m + 7) * 64]; }
                     ptop = (int32_t)st[16 * 64];
                 }
-                uint32_t lut[9];
+                uint32_t lut[9];          // score table per row, or the row's base when !use_lut
                 const uint32_t gpac = act ? T[s] : 0u;
 #pragma unroll
-                for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
+                for (int m = 1; m < 9; m++)
+                    lut[m] = use_lut ? score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u) : (gpac >> (32 - 4 * m)) & 15u;
                 int32_t rv = 0, rt = 0;   // run best: encoded value, step
+                uint32_t win[8];          // direction words of this lane's last eight columns
+#pragma unroll
+                for (int m = 0; m < 8; m++) win[m] = 0;
                 uint32_t out = 0, qcls = 0;
                 uint32_t qcur = 0, qnext = Q[c0 >> 3];   // query word of lane 0's column, next one prefetched
                 for (int32_t tau = 0; tau < wb + 63; tau++) {
@@ -535,7 +546,8 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                             if (tau + 8 < wb) qnext = Q[(c0 + tau + 8) >> 3];
                         }
                         if (k > 0) top0 = ring[tau];
-                        q0 = ((qcur >> (28 - 4 * (tau & 7))) & 7u) << 2;
+                        const uint32_t qn = (qcur >> (28 - 4 * (tau & 7))) & 15u;
+                        q0 = use_lut ? (qn & 7u) << 2 : qn;
                     }
                     const uint32_t tin = (uint32_t)__builtin_amdgcn_update_dpp((int)top0, (int)out, 0x138, 0xF, 0xF, false);
                     qcls = (uint32_t)__builtin_amdgcn_update_dpp((int)q0, (int)qcls, 0x138, 0xF, 0xF, false);
@@ -545,14 +557,22 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                         int32_t p = ptop;
                         ptop = sx16(tin);
                         int32_t colbest = 0;
+                        uint32_t dir = 0;
 #pragma unroll
                         for (int m = 1; m < 9; m++) {
-                            const int32_t sc = __builtin_amdgcn_sbfe(lut[m], qcls, 4);
+                            const int32_t sc = use_lut ? __builtin_amdgcn_sbfe(lut[m], qcls, 4) : sub_local(c, nval, qcls, lut[m]);
                             const int32_t tmp = p + sc;
                             const int32_t hm = max(max(max(tmp, f[m]), e), 0);
                             const int32_t t = tmp - gapoe;
-                            f[m] = max(t, f[m] - gape);
-                            e = max(t, e - gape);
+                            const int32_t fx = f[m] - gape, ex = e - gape;
+                            if (TB) {
+                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
+                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+                            }
+                            f[m] = max(t, fx);
+                            e = max(t, ex);
                             const int32_t en = (hm << 3) | (8 - m);
                             colbest = m == 1 ? en : max(colbest, en);
                             p = h[m];
@@ -560,6 +580,13 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                         }
                         // strictly higher h only: an equal h in a later column never wins
                         if (colbest > (rv | 7)) { rv = colbest; rt = tau; }
+                        if (TB) {
+#pragma unroll
+                            for (int m = 0; m < 7; m++) win[m] = win[m + 1];
+                            win[7] = dir;
+                            const int32_t col = c0 + rel;
+                            if ((col & 7) == 7) tb_store(tbp + ((size_t)s * QW + (size_t)(col >> 3)) * 8, win);
+                        }
                         out = pk16(h[8], e);
                         if (l == 63 && k + 1 < npass) ring[rel] = out;
                     }
@@ -593,11 +620,12 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
-    if (!census_fast(census)) return;
     if (blockIdx.x < d.wf_blocks) {
-        if (START == S_WITHOUT_START && !SECOND) wf_local(d, c, census);
+        if (START == S_WITHOUT_START && !SECOND) wf_local<false>(d, c, census);
+        if (START == S_WITH_TB && !SECOND) wf_local<true>(d, c, census);
         return;
     }
+    if (!census_fast(census)) return;
     if (!d.enc_ok) return;   // position encoding does not fit: k_local takes the short pairs
     LaneCtx x;
     if (!lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) return;
@@ -767,8 +795,9 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
 // src/kernels/local_kernel_template.h:103,118-127).  Best cell: first strict
 // maximum in (strip, column, row) order, as in k_local.
 // Flags (band_flags): bit0 the best cell is on an edge tile (a tile with a
-// neighbour outside the band), bit2 some edge-tile cell has
-// H >= max(edge_abs, edge_frac * score), bit3 not run (score tables refused).
+// neighbour outside the band), bit2 the check band (the band widened by
+// band_widen diagonals each side, run first with CHECK = true, score and ends
+// only) found a different score or end, bit3 not run (score tables refused).
 constexpr int32_t BAND_NEG = -16000;
 
 __device__ __forceinline__ void band_words(int32_t s, int32_t lo, int32_t hi, int32_t QW, int32_t &wlo, int32_t &whi) {
@@ -778,22 +807,21 @@ __device__ __forceinline__ void band_words(int32_t s, int32_t lo, int32_t hi, in
     whi = whi > QW ? QW : whi;
 }
 
-template <bool TB>
+template <bool TB, bool CHECK>
 __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const uint32_t pr = x.pair;
-    if (!census_fast(census)) { d.band_flags[pr] = 8; return; }
+    if (!census_fast(census)) { if (!CHECK) d.band_flags[pr] = 8; return; }
     const int32_t nval = c.has_np ? -c.np : 0;
     const int32_t gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
-    const int32_t lo = d.band_lo[pr], hi = d.band_hi[pr];
+    const int32_t lo = d.band_lo[pr] - (CHECK ? d.band_widen : 0), hi = d.band_hi[pr] + (CHECK ? d.band_widen : 0);
     uint32_t *__restrict__ bd = x.bd;
     uint32_t *tbp = TB ? d.tb + d.band_tbo[pr] * 8 : nullptr;
     const int32_t bw = TB ? (int32_t)d.band_bw[pr] : 0;
     int32_t rv = 0, rs = 0, rc = 0;   // best: h << 3 | (8 - row in strip), strip, column
-    int32_t emax = 0;                 // best encoded h on edge tiles
     bool best_edge = false, overflow = false;
     int32_t plo = 0, phi = 0;         // word range of the previous strip
     for (int32_t st = 0; st < TW; st++) {
@@ -846,7 +874,6 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
                 }
                 bd[col * WAVE] = pk16(h[8], e);
                 if (colbest > (rv | 7)) { rv = colbest; rs = st; rc = col; best_edge = edge; }
-                if (edge) emax = max(emax, colbest);
                 if (TB) dirw[kk] = dir;
             }
             if (TB) {
@@ -858,12 +885,19 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
         phi = whi;
     }
     const int32_t v = rv >> 3;
+    const int32_t qe = v > 0 ? rc : 0, te = v > 0 ? rs * 8 + (7 - (rv & 7)) : 0;
+    if (CHECK) {
+        d.band_wres[pr] = v;
+        d.band_wres[d.n + pr] = qe;
+        d.band_wres[2 * d.n + pr] = te;
+        return;
+    }
     d.score[pr] = v;
-    d.qend[pr] = v > 0 ? rc : 0;
-    d.tend[pr] = v > 0 ? rs * 8 + (7 - (rv & 7)) : 0;
-    const int32_t eh = emax >> 3;
-    const int32_t thr = max(d.band_edge_abs, (int32_t)ceilf(d.band_edge_frac * (float)v));
-    d.band_flags[pr] = (uint8_t)((best_edge && v > 0 ? 1u : 0u) | (eh >= thr && eh > 0 ? 4u : 0u) | (overflow ? 8u : 0u));
+    d.qend[pr] = qe;
+    d.tend[pr] = te;
+    const bool differs = d.band_wres &&
+                         (d.band_wres[pr] != v || d.band_wres[d.n + pr] != qe || d.band_wres[2 * d.n + pr] != te);
+    d.band_flags[pr] = (uint8_t)((best_edge && v > 0 ? 1u : 0u) | (differs ? 4u : 0u) | (overflow ? 8u : 0u));
 }
 
 // --------------------------------------------------------------- GLOBAL -----
@@ -1283,11 +1317,15 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
 template <bool LOCALT>
 __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t grp = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    const uint32_t slot = grp * WAVE + lane;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= d.n) return;
     const uint32_t pr = d.perm[slot];
+    // long pairs (sorted first) ran one wave per pair with per-pair tiles; the
+    // lane path's groups count from the first slot after them
+    const uint32_t nlong = (LOCALT && d.wf_tb) ? d.aux[2] : 0u;
+    const bool wfp = slot < nlong;
+    const uint32_t rslot = slot - nlong;
+    const uint32_t lane = rslot & (WAVE - 1), grp = rslot / WAVE;
     const uint32_t read_len = d.qlen[pr], ref_len = d.tlen[pr];
     const int32_t QW = dregs(read_len), TW = dregs(ref_len);
     // LOCAL words written by k_local_fast carry the four comparisons as
@@ -1298,7 +1336,7 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int32_t bbw = band ? (int32_t)d.band_bw[pr] : 0;
     uint32_t bflag = 0;
     if (band && (d.band_flags[pr] & 8u)) { d.n_ops[pr] = 0; return; }
-    const bool fastfmt = LOCALT && (band || (d.fast && d.enc_ok && census_fast(d.aux[0])));
+    const bool fastfmt = LOCALT && (band || wfp || (d.fast && d.enc_ok && census_fast(d.aux[0])));
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1331,6 +1369,8 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
                     if (wv < wlo || wv >= whi || wv - wlo >= bbw) { bflag |= 2u; break; }   // walk left the band
                     if ((wv == wlo && wlo > 0) || (wv == whi - 1 && whi < QW)) bflag |= 1u;
                     chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
+                } else if (wfp) {
+                    chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)(col >> 3)) * 8;
                 } else {
                     chunk = tb_chunk(d, grp, lane, strip, col >> 3);
                 }
@@ -1529,14 +1569,15 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const bool second = op.second != 0;
     switch (op.banded ? -1 : op.algo) {
         case -1:   // per-pair diagonal bands (score tables only; the caller checks d.fast)
-            if (tb) k_local_band<true><<<grid, blk, 0, s>>>(d, sc);
-            else k_local_band<false><<<grid, blk, 0, s>>>(d, sc);
+            if (d.band_wres) k_local_band<false, true><<<grid, blk, 0, s>>>(d, sc);
+            if (tb) k_local_band<true, false><<<grid, blk, 0, s>>>(d, sc);
+            else k_local_band<false, false><<<grid, blk, 0, s>>>(d, sc);
             break;
         case A_LOCAL:
             if (d.fast) {
                 if (op.start == S_WITH_TB) {
                     if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
-                    else k_local_fast<S_WITH_TB, false><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITH_TB, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
                 } else if (op.start == S_WITH_START) {
                     if (second) k_local_fast<S_WITH_START, true><<<grid, blk, 0, s>>>(d, sc);
                     else k_local_fast<S_WITH_START, false><<<grid, blk, 0, s>>>(d, sc);

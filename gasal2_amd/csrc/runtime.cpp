@@ -86,6 +86,13 @@ struct Impl {
     uint64_t *band_tbo = nullptr;
     uint8_t *band_flags = nullptr;
     size_t band_cap = 0;
+    int32_t *band_wres = nullptr;
+    size_t band_wres_bytes = 0;
+    uint32_t *wf_tb = nullptr;       // direction tiles of one-wave-per-pair pairs
+    size_t wf_tb_bytes = 0;
+    uint64_t *wf_tbo = nullptr;
+    size_t wf_tbo_bytes = 0;
+    std::vector<uint64_t> h_wf_tbo;
     std::vector<int32_t> h_lo, h_hi;
     std::vector<uint32_t> h_bw;
     std::vector<uint64_t> h_tbo;
@@ -728,7 +735,7 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 
 static void impl_free(Impl *m) {
     if (!m) return;
-    void *ptrs[] = {m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
+    void *ptrs[] = {m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
                     m->sub_nops, m->sub_cigar,
                     m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
                     m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
@@ -975,7 +982,6 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     ensure(m->bnd, m->bnd_bytes, (size_t)groups * bnd_cols * 64 * 4);
     const bool tb_on = op.start == WITH_TB;
     const bool tb_dp = tb_on && (op.algo == LOCAL || op.algo == GLOBAL);
-    if (tb_dp && !px.banded) ensure(m->tb, m->tb_bytes, (size_t)groups * tw * (8 * qw) * 64 * 4);
     const uint32_t eh_cols = mq + 2;
     if (op.algo == KSW) ensure(m->eh, m->eh_bytes, (size_t)groups * eh_cols * 64 * 8);
     const uint32_t cap = g2_cigar_capacity(mq, mt);
@@ -1043,7 +1049,9 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     // device applies the same shape test in k_keys and also needs the nibble
     // census to allow the score tables; the host count only sizes the launch.
     uint32_t eq = qw, et = tw;   // largest shape the one-lane-per-pair kernels see
-    if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g2_get_dispatch() != 2) {
+    uint32_t nlong = 0;
+    if (d.fast && op.algo == LOCAL && (op.start == WITHOUT_START || (op.start == WITH_TB && !px.banded)) && !op.second &&
+        g2_get_dispatch() != 2) {
         const bool all = g2_get_dispatch() == 3;
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
         {
@@ -1059,12 +1067,16 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         }
         d.wf_min_tw = all ? 1u : min_tw;
         d.wf_min_cells = all ? 0u : min_cells;
-        uint32_t nlong = 0, ltw = 0, sq = 0, st = 0;
+        uint32_t ltw = 0, sq = 0, st = 0;
+        const bool wtb = op.start == WITH_TB;
+        if (wtb) m->h_wf_tbo.resize(n);
+        uint64_t wtot = 0;   // 32-byte tiles of the long pairs
         for (uint32_t k = 0; k < n; k++) {
             const uint32_t kq = g2_regs(hq[k]), kt = g2_regs(ht[k]);
             if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
                 nlong++;
                 ltw = std::max(ltw, kt);
+                if (wtb) { m->h_wf_tbo[k] = wtot; wtot += (uint64_t)kq * kt; }
             } else {
                 sq = std::max(sq, kq);
                 st = std::max(st, kt);
@@ -1082,7 +1094,24 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 d.wf_stride = 0;
                 d.wf_scratch = nullptr;
             }
+            if (wtb) {   // per-pair direction tiles of the long pairs and their offsets
+                ensure(m->wf_tb, m->wf_tb_bytes, (size_t)wtot * 32 + 256);
+                ensure(m->wf_tbo, m->wf_tbo_bytes, (size_t)n * 8);
+                G2_CHECK(hipMemcpyAsync(m->wf_tbo, m->h_wf_tbo.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->str));
+                d.wf_tb = m->wf_tb;
+                d.wf_tbo = m->wf_tbo;
+            }
         }
+    }
+    // direction tiles of the one-lane-per-pair kernels: groups after the long pairs
+    if (tb_dp && !px.banded) {
+        const uint64_t lgroups = ((uint64_t)(n - nlong) + 63) / 64;
+        ensure(m->tb, m->tb_bytes, (size_t)lgroups * et * (8 * (size_t)eq) * 64 * 4 + 256);
+        d.tb = m->tb;
+        d.tb_tw = et;
+        d.tb_qw8 = 8 * eq;
+        s->packed_tb_matrices = (uint4 *)m->tb;
+        s->packed_tb_matrix_size = m->tb_bytes / 16;
     }
     // LOCAL lane-per-pair score-table kernel: position encoding
     // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
@@ -1314,12 +1343,11 @@ extern "C" void gasal2_result_views(gasal_gpu_storage_t *s, void **score, void *
 // Banded LOCAL over per-pair diagonal bands (learned-band path): the batch
 // must be resident and prepared (gasal2_upload, gasal2_prep_resident).  Results stay in device memory like
 // gasal2_run_resident; flags (host, n bytes) come back synchronously:
-// bit0 best cell on the band edge, bit1 traceback left the band, bit2 a
-// high score reached the band edge, bit3 not run (score tables refused).
+// bit0 best cell on the band edge, bit1 traceback left the band, bit2 the
+// check band (widened by check_widen each side) disagrees, bit3 not run.
 // Any flagged pair should be re-run with gasal2_run_subset.
 extern "C" int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
-                                     const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t edge_abs,
-                                     float edge_frac) {
+                                     const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t check_widen) {
     if (!validate(s, qb, tb, n)) return -1;
     if (params->algo != LOCAL || params->secondBest || params->start_pos == WITH_START) {
         report("[GASAL ERROR:] banded runs take LOCAL WITHOUT_START or WITH_TB without second best\n");
@@ -1378,8 +1406,11 @@ extern "C" int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32
     }
     d.band_lo = m->band_lo; d.band_hi = m->band_hi; d.band_bw = m->band_bw; d.band_tbo = m->band_tbo;
     d.band_flags = m->band_flags;
-    d.band_edge_abs = edge_abs;
-    d.band_edge_frac = edge_frac;
+    if (check_widen > 0) {
+        ensure(m->band_wres, m->band_wres_bytes, (size_t)n * 12);
+        d.band_wres = m->band_wres;
+        d.band_widen = check_widen;
+    }
     m->main_dev = d;
     G2Opts op = opts_of(params);
     op.banded = 1;

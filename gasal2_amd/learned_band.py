@@ -29,7 +29,8 @@ from . import Aligner, Parameters, lib, LOCAL, WITH_TB, WITHOUT_START, FALSE
 NFEAT = 68
 HIDDEN = 256
 MARGIN = 16        # label half-width slack (bases)
-MIN_SHARE = 0.5    # a band must meet at least this share of the pair's k-mer hits
+MIN_SHARE = 0.9    # a band must meet at least this share of the pair's k-mer hits
+CHECK_WIDEN = 512  # check band: the proposed band widened by this many diagonals each side
 
 
 def _torch():
@@ -280,10 +281,11 @@ def calibrate_peak(feat, anchor, binw, qlen, tlen, dmin, dmax, score, coverage=0
     return float(np.quantile(need[ok], coverage)) if ok.any() else 64.0
 
 
-def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WITH_TB, edge=(24, 0.125),
-               chunk=16384, log=None):
+def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WITH_TB, check_widen=CHECK_WIDEN,
+               chunk=65536, log=None):
     """Banded LOCAL with full-DP fallback over the batch, in length-sorted
-    chunks of at most `chunk` pairs (one storage each).  `propose(a, n) ->
+    chunks of at most `chunk` pairs (one storage each; a large chunk keeps
+    enough one-lane-per-pair waves in flight).  `propose(a, n) ->
     (dlo, dhi, pre)` host arrays (pre: pairs sent to the full DP outright),
     called on the prepared chunk.  Returns
     (results over all pairs, stats with timings and flag counts)."""
@@ -308,7 +310,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
             a.prep()
             dlo, dhi, pre = propose(a, sub.n)
             t1 = time.perf_counter()
-            flags = a.run_banded(dlo, dhi, edge_abs=edge[0], edge_frac=edge[1]) | pre
+            flags = a.run_banded(dlo, dhi, check_widen=check_widen) | pre
             t2 = time.perf_counter()
             a.run_subset(np.nonzero(flags)[0])
             a.sync()

@@ -114,8 +114,10 @@ int32_t gasal2_tune_count(void);
  * per-pair diagonal bands d = query column - target row in [dlo[k], dhi[k]]
  * (host arrays), over the 8x8 tiles the band touches.  Results stay in device
  * memory (gasal2_download fetches them); flags[k] (host) come back set:
- * bit0 best cell on a band-edge tile, bit1 traceback left the band, bit2 an
- * edge-tile cell scored >= max(edge_abs, edge_frac * score), bit3 not run.
+ * bit0 best cell on a band-edge tile, bit1 traceback left the band, bit2 the
+ * check band -- the band widened by check_widen diagonals on each side, run
+ * first for score and ends only (0: no check) -- found another score or end,
+ * bit3 not run.
  * Returns 0, or 1 when the substitution scores do not fit the 4-bit score
  * tables (every pair flagged), -1 on error.
  *
@@ -130,7 +132,7 @@ void gasal2_result_views(gasal_gpu_storage_t *s, void **score, void **qend, void
 int32_t gasal2_prep_resident(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params);
 void gasal2_device_views(gasal_gpu_storage_t *s, void **pq, void **pt, void **qoff, void **toff, void **qlen, void **tlen);
 int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
-                          const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t edge_abs, float edge_frac);
+                          const int32_t *dlo, const int32_t *dhi, uint8_t *flags, int32_t check_widen);
 int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
                           const uint32_t *idx, uint32_t cnt, uint64_t max_tb_bytes);
 

@@ -125,9 +125,9 @@ def test_tied_maxima_across_strips_and_blocks():
         check(b)
 
 
-def test_census_collision_keeps_lane_kernel():
-    """Nibble classes collide ('I' vs 'A'): no pair may take the one-wave path
-    (it needs the score tables); the reference-order kernel takes all."""
+def test_census_collision_every_pair_one_wave():
+    """Nibble classes collide ('I' vs 'A'): every pair runs one wave per pair
+    with the substitution rule per cell instead of score tables."""
     qs, ts = related_pairs([(1500, 1600), (900, 800), (2100, 2000)], 1500)
     qs[1] = qs[1][:50] + b"IIA" + qs[1][53:]
     b = g.make_batch(qs, ts)
@@ -150,3 +150,43 @@ def test_autotune_choices_identical_results():
         check(b)
     finally:
         autotune.clear()
+
+
+def check_tb(b, **kw):
+    got, ref = run_both(b, g.LOCAL, g.WITH_TB, False, **kw)
+    assert_parity(got, ref, g.LOCAL, g.WITH_TB, False, b)
+
+
+@pytest.mark.parametrize("cfg,n_penalty,is_rc", [(1, None, False), (2, 1, True)])
+def test_tb_every_pair_one_wave(cfg, n_penalty, is_rc):
+    """WITH_TB: direction tiles per pair, traceback through them."""
+    b = g.synth_batch(cfg, 1700 + cfg, 500)
+    with policy(dispatch=3):
+        check_tb(b, is_rc=is_rc, n_penalty=n_penalty)
+
+
+@pytest.mark.parametrize("scores", [(1, 4, 6, 1), (2, 3, 5, 2)])
+def test_tb_pass_and_block_boundaries(scores):
+    qs, ts = related_pairs(SHAPES, 1800 + scores[0])
+    b = g.make_batch(qs, ts)
+    with policy(dispatch=3):
+        check_tb(b, scores=scores, n_penalty=1)
+
+
+def test_tb_mixed_switch_and_lane_groups():
+    """Long pairs one wave each, the rest one lane each: the traceback finds
+    both tile layouts (lane groups count after the long pairs)."""
+    b = g.synth_batch(3, 1900, 700)
+    with policy(dispatch=0, wf=(16, 64 * 16 * 16, 64)):
+        check_tb(b, n_penalty=1)
+
+
+def test_census_collision_runs_one_wave_without_tables():
+    """Nibble classes collide: the one-wave path scores per cell instead of
+    from tables, in both modes."""
+    qs, ts = related_pairs([(1500, 1600), (900, 800), (2100, 2000), (300, 280)], 1950)
+    qs[1] = qs[1][:50] + b"IIA" + qs[1][53:]
+    b = g.make_batch(qs, ts)
+    with policy(dispatch=0, wf=(16, 64 * 16 * 16, 64)):
+        check(b)
+        check_tb(b)
