@@ -48,6 +48,8 @@ def main():
                          "band: learned band vs full DP on 1-10 kbp pairs (train seed 5, held-out seed 6)")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for --workload band")
     ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--check-widen", type=int, nargs="+", default=[128],
+                    help="check-band widening(s) (diagonals each side) to evaluate; the first one is reported as value")
     ap.add_argument("--dispatch", type=int, default=0,
                     help="kernel dispatch policy (0 auto, 2 one lane per pair only, 3 one wave per pair)")
     ap.add_argument("--compare-lane-only", action="store_true",
@@ -279,10 +281,14 @@ def band_main(args):
     ref, full_s, _ = LB.full_dp(held)
     log(f"full DP: {full_s:.2f}s ({cells / full_s / 1e9:.1f} GCUPS)")
     out = {}
-    for name, prop in (("learned", LB.learned_proposer(net, mul, add)), ("peak", LB.peak_proposer(half))):
-        res, st = LB.band_align(held, prop)
+    runs = []
+    for cw in args.check_widen:
+        runs += [(f"learned@{cw}" if cw != args.check_widen[0] else "learned", LB.learned_proposer(net, mul, add), cw),
+                 (f"peak@{cw}" if cw != args.check_widen[0] else "peak", LB.peak_proposer(half), cw)]
+    for name, prop, cw in runs:
+        res, st = LB.band_align(held, prop, check_widen=cw)
         agree, bad = LB.agreement(res, ref)
-        out[name] = {"gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
+        out[name] = {"check_widen": cw, "gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
                      "pairs_per_s": round(n / st["total_s"], 1), "hit_rate": round(st["hit_rate"], 5),
                      "final_agreement": agree, "band_cells_fraction": round(st["band_cells_fraction"], 5),
                      "fallback_pairs": st["fallback_pairs"], "flag_counts": st["flag_counts"],
@@ -303,6 +309,7 @@ def band_main(args):
         "full_dp": {"gcups": round(cells / full_s / 1e9, 2), "seconds": round(full_s, 3),
                     "pairs_per_s": round(n / full_s, 1)},
         "learned": out["learned"], "peak_band": out["peak"],
+        "other_check_widths": {k: v for k, v in out.items() if "@" in k},
         "speedup_vs_full": round(full_s / out["learned"]["seconds"], 3),
         "speedup_peak_vs_full": round(full_s / out["peak"]["seconds"], 3),
         "training": {"seconds": round(train_s, 2), "epochs": args.epochs, "loss_first": hist[0], "loss_last": hist[-1],

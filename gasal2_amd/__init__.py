@@ -569,7 +569,7 @@ class Aligner:
         self._L.gasal2_result_views(C.byref(self.s), *[C.byref(p) for p in ptrs], C.byref(cap))
         return [p.value for p in ptrs] + [cap.value]
 
-    def run_banded(self, dlo, dhi, check_widen=512):
+    def run_banded(self, dlo, dhi, check_widen=128):
         """Banded LOCAL inside per-pair bands; returns the per-pair flags."""
         qb, tb, n = self._loaded
         lo = np.ascontiguousarray(dlo, dtype=np.int32)

@@ -214,7 +214,14 @@ __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
     if (k >= d.n) return;
     const uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
     const bool lng = d.wf_blocks > 0 && tw >= d.wf_min_tw && (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
-    const uint32_t key = (lng ? 0x10000u : 0u) | (shape8(qw) << 8) | shape8(tw);
+    // banded runs: the band's words per strip stand in for the query words
+    uint32_t cw = qw;
+    if (d.band_lo) {
+        const int64_t span = (int64_t)d.band_hi[k] - d.band_lo[k];
+        const int64_t bw = span < 0 ? 1 : ((span + 14) >> 3) + 1;
+        cw = (uint32_t)(bw < (int64_t)qw ? bw : (int64_t)qw);
+    }
+    const uint32_t key = (lng ? 0x10000u : 0u) | (shape8(cw) << 8) | shape8(tw);
     d.keys[k] = key;
     atomicAdd(&d.hist[key], 1u);
     if (lng) atomicAdd(&d.aux[2], 1u);
@@ -835,8 +842,12 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
         for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
 #pragma unroll
         for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = wlo == 0 ? 0 : BAND_NEG; }
+        // The boundary row is kept band-relative: strip st writes column col at
+        // bd[col - 8 wlo], so the lanes of a wave (different bands) touch at
+        // most two rows per access; the previous strip's value of col sits at
+        // col - 8 plo (>= the write index, read before it is overwritten).
         // H(strip top - 1, first column - 1), if the previous strip computed it
-        if (st > 0 && wlo > 0 && wlo - 1 >= plo && wlo - 1 < phi) h[0] = sx16(bd[(8 * wlo - 1) * WAVE]);
+        if (st > 0 && wlo > 0 && wlo - 1 >= plo && wlo - 1 < phi) h[0] = sx16(bd[(8 * (wlo - plo) - 1) * WAVE]);
         for (int32_t w = wlo; w < whi; w++) {
             const uint32_t rpac = x.Q[w];
             const bool top_in = st == 0 || (w >= plo && w < phi);
@@ -846,7 +857,7 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t col = 8 * w + kk;
                 const uint32_t qsh = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
-                const uint32_t cur = st == 0 ? 0u : (top_in ? bd[col * WAVE] : pk16(0, BAND_NEG));
+                const uint32_t cur = st == 0 ? 0u : (top_in ? bd[(8 * (w - plo) + kk) * WAVE] : pk16(0, BAND_NEG));
                 int32_t e = (int32_t)cur >> 16;
                 int32_t p = h[0];
                 h[0] = sx16(cur);
@@ -872,7 +883,7 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
                     p = h[m];
                     h[m] = hm;
                 }
-                bd[col * WAVE] = pk16(h[8], e);
+                bd[(8 * (w - wlo) + kk) * WAVE] = pk16(h[8], e);
                 if (colbest > (rv | 7)) { rv = colbest; rs = st; rc = col; best_edge = edge; }
                 if (TB) dirw[kk] = dir;
             }

@@ -30,7 +30,7 @@ NFEAT = 68
 HIDDEN = 256
 MARGIN = 16        # label half-width slack (bases)
 MIN_SHARE = 0.9    # a band must meet at least this share of the pair's k-mer hits
-CHECK_WIDEN = 512  # check band: the proposed band widened by this many diagonals each side
+CHECK_WIDEN = 128  # check band: the proposed band widened by this many diagonals each side
 
 
 def _torch():
@@ -185,9 +185,11 @@ def chunks_by_tb(batch, order, budget=24 << 30):
     return out
 
 
-def full_dp(batch, scores=(1, 4, 6, 1), n_penalty=None, labels=False, budget=24 << 30, log=None):
-    """LOCAL WITH_TB full DP over the batch in length-sorted chunks.  Returns
-    (results dict over all pairs, device seconds, [dmin, dmax] if labels)."""
+def full_dp(batch, scores=(1, 4, 6, 1), n_penalty=None, labels=False, budget=96 << 30, log=None):
+    """LOCAL WITH_TB full DP over the batch in length-sorted chunks on one
+    storage (workspace allocated once and reused).  Returns (results dict over
+    all pairs, seconds of the device program incl. its first allocation,
+    [dmin, dmax] if labels)."""
     torch = _torch()
     L = lib()
     n = batch.n
@@ -197,10 +199,13 @@ def full_dp(batch, scores=(1, 4, 6, 1), n_penalty=None, labels=False, budget=24 
     dmin = np.zeros(n, np.int32)
     dmax = np.zeros(n, np.int32)
     secs = 0.0
-    for ch in chunks_by_tb(batch, length_order(batch), budget):
-        sub = batch.subset(ch)
-        a = _aligner_for(sub, WITH_TB, scores, n_penalty)
-        try:
+    chunks = chunks_by_tb(batch, length_order(batch), budget)
+    p = Parameters(algo=LOCAL, start_pos=WITH_TB, secondBest=FALSE)
+    a = Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), max(len(c) for c in chunks), scores=scores,
+                n_penalty=n_penalty)
+    try:
+        for ch in chunks:
+            sub = batch.subset(ch)
             a.load(sub)
             a.sync()
             t0 = time.perf_counter()
@@ -221,10 +226,10 @@ def full_dp(batch, scores=(1, 4, 6, 1), n_penalty=None, labels=False, budget=24 
                 res[k][ch] = r[k]
             for i, k in enumerate(ch):
                 res["cigars"][k] = a.cigar(i)
-        finally:
-            a.close()
-        if log:
-            log(f"full DP chunk of {len(ch)} pairs done")
+            if log:
+                log(f"full DP chunk of {len(ch)} pairs done")
+    finally:
+        a.close()
     return res, secs, (dmin, dmax)
 
 
