@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--mode", default=None,
-                    choices=["local_tb_second", "local_score", "global", "semi_global"],
+                    choices=["local_tb_second", "local_tb", "local_score", "global", "semi_global"],
                     help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
     ap.add_argument("--workload", default="full", choices=["full", "uneven", "band"],
                     help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
@@ -89,6 +89,9 @@ def main():
 
     if mode == "local_tb_second":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True)
+        ops = OPS_TB
+    elif mode == "local_tb":
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, isReverseComplement=True)
         ops = OPS_TB
     elif mode == "local_score":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START, isReverseComplement=True)
@@ -216,7 +219,9 @@ def main():
         "config": {"workload": workload,
                    "mode": mode, "dispatch": args.dispatch, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
-        "roofline": {"bound": "valu", "kernel": "DP (one lane per pair)", "ops_per_cell": ops,
+        "roofline": {"bound": "valu",
+                     "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",
+                     "ops_per_cell": ops,
                      "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3), "unit": "TOP/s",
                      "frac": round(achieved_tops / peak_tops, 4), "traffic": traffic,
                      "dp_kernel_ms": round(dp_avg, 3)},

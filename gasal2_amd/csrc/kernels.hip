@@ -486,7 +486,9 @@ __device__ __forceinline__ bool wf_before(int32_t av, int32_t as, int32_t ac, in
 // TB: direction words as in k_local_fast; each lane keeps its last eight
 // columns in a window and stores the 32-byte tile when a column closes an 8x8
 // tile, to the pair's own region wf_tb + wf_tbo[pair] laid out [strip][word].
-template <bool TB>
+// LUT: score tables (the batch's nibble classes allow them), else the
+// substitution rule per cell with the query base travelling instead of its class.
+template <bool TB, bool LUT>
 __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     __shared__ uint32_t ring[WF_W];
     const int32_t l = (int32_t)threadIdx.x;
@@ -500,9 +502,7 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     // lanes take part in the atomic; only lane 0 adds, so lane 0 reads the
     // counter before its own increment.
     const uint32_t nlong = __builtin_amdgcn_readfirstlane(d.aux[2]);
-    // score tables when the batch's nibble classes allow them, else the
-    // substitution rule per cell (the query base travels instead of its class)
-    const bool use_lut = census_fast(census);
+    constexpr bool use_lut = LUT;
     for (;;) {
         const uint32_t item = __builtin_amdgcn_readfirstlane(atomicAdd(&d.aux[1], l == 0 ? 1u : 0u));
         if (item >= nlong) break;
@@ -628,8 +628,14 @@ template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
     if (blockIdx.x < d.wf_blocks) {
-        if (START == S_WITHOUT_START && !SECOND) wf_local<false>(d, c, census);
-        if (START == S_WITH_TB && !SECOND) wf_local<true>(d, c, census);
+        if (START == S_WITHOUT_START && !SECOND) {
+            if (census_fast(census)) wf_local<false, true>(d, c, census);
+            else wf_local<false, false>(d, c, census);
+        }
+        if (START == S_WITH_TB && !SECOND) {
+            if (census_fast(census)) wf_local<true, true>(d, c, census);
+            else wf_local<true, false>(d, c, census);
+        }
         return;
     }
     if (!census_fast(census)) return;
