@@ -49,6 +49,10 @@ struct G2Dev {
     uint32_t bnd_cols;
     uint32_t *tb;                    // direction words: per group tb_tw*tb_qw8*64
     uint32_t tb_tw, tb_qw8;
+    // per-group tile regions (set when one size for all groups would not fit):
+    // group g starts at dword tb_gbase[g], strips of tb_gqw[g] query words
+    const uint64_t *tb_gbase;
+    const uint32_t *tb_gqw;
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
     uint32_t *aux;                   // small device counters (work queue etc.)

@@ -54,8 +54,25 @@ struct LaneCtx {
 // columns]: a lane writes one 32-byte chunk per 8x8 tile (two 16-byte stores,
 // contiguous across the wave) and a traceback walk stays inside one chunk for
 // eight columns.
-__device__ __forceinline__ uint32_t *tb_chunk(const G2Dev &d, uint32_t grp, uint32_t lane, int32_t strip, int32_t word) {
-    return d.tb + ((((size_t)grp * d.tb_tw + (size_t)strip) * (d.tb_qw8 >> 3) + (size_t)word) * WAVE + lane) * 8;
+// A lane's tiles: its first chunk and the strip stride in query words, taken
+// once per kernel (one region size for all groups, or per-group regions).
+struct TbView {
+    uint32_t *base;
+    uint32_t qw;
+    __device__ __forceinline__ uint32_t *chunk(int32_t strip, int32_t word) const {
+        return base + ((size_t)strip * qw + (size_t)word) * (WAVE * 8);
+    }
+};
+__device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t lane) {
+    TbView v;
+    if (d.tb_gbase) {
+        v.base = d.tb + d.tb_gbase[grp] + (size_t)lane * 8;
+        v.qw = d.tb_gqw[grp];
+    } else {
+        v.qw = d.tb_qw8 >> 3;
+        v.base = d.tb + ((size_t)grp * d.tb_tw * v.qw * WAVE + lane) * 8;
+    }
+    return v;
 }
 __device__ __forceinline__ void tb_store(uint32_t *p, const uint32_t *w) {
     reinterpret_cast<uint4 *>(p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
@@ -274,6 +291,7 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
     const int32_t gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     uint32_t *__restrict__ bd = x.bd;
+    const TbView tbv = TB ? tb_view(d, x.grp, x.lane) : TbView{nullptr, 0};
 
     for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;   // initHD (:118-120)
 
@@ -341,7 +359,7 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
                 prev_maxHH = max(maxHH, prev_maxHH);
                 if (TB) dirw[kk] = dir;
             }
-            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
+            if (TB) tb_store(tbv.chunk(i, j), dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -648,6 +666,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const int32_t QW = x.QW, TW = x.TW;
     const int32_t S = d.enc_s;
     uint32_t *__restrict__ bd = x.bd;
+    const TbView tbv = TB ? tb_view(d, x.grp, x.lane) : TbView{nullptr, 0};
 
     for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;
 
@@ -719,7 +738,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
                 tagc -= 8;
                 if (TB) dirw[kk] = dir;
             }
-            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
+            if (TB) tb_store(tbv.chunk(i, j), dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -930,6 +949,7 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
     const int32_t gapo = c.gapo, gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     uint32_t *__restrict__ bd = x.bd;
+    const TbView tbv = TB ? tb_view(d, x.grp, x.lane) : TbView{nullptr, 0};
     bd[0] = pk16(0, MINUS_INF);                                                            // :57-60
     for (int32_t cc = 1; cc < 8 * QW; cc++) bd[cc * WAVE] = pk16(-(gapo + gape * cc), MINUS_INF);
     const int32_t rowsel = 8 - ((TW << 3) - (int32_t)x.tlen);                              // :299
@@ -990,7 +1010,7 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                     for (int m = 1; m < 9; m++) score = (m == rowsel) ? h[m] : score;
                 }
             }
-            if (TB) tb_store(tb_chunk(d, x.grp, x.lane, i, j), dirw);
+            if (TB) tb_store(tbv.chunk(i, j), dirw);
         }
     }
     d.score[x.pair] = score;
@@ -1389,7 +1409,7 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
                 } else if (wfp) {
                     chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)(col >> 3)) * 8;
                 } else {
-                    chunk = tb_chunk(d, grp, lane, strip, col >> 3);
+                    chunk = tb_view(d, grp, lane).chunk(strip, col >> 3);
                 }
                 const uint4 *src = reinterpret_cast<const uint4 *>(chunk);
                 const uint4 a = src[0], b = src[1];

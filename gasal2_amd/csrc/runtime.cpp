@@ -33,6 +33,7 @@ int g_error_count = 0;
 // one-wave-per-pair thresholds: target words, cells (64 per query word x target word)
 uint32_t g_wf_min_tw = 48, g_wf_min_cells = 1u << 20, g_wf_max_blocks = 2048;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
+uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all groups up to this many bytes
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
 // range); filled by the autotuner (gasal2_tune_set), consulted per batch.
@@ -88,6 +89,11 @@ struct Impl {
     size_t band_cap = 0;
     int32_t *band_wres = nullptr;
     size_t band_wres_bytes = 0;
+    uint64_t *tb_gbase = nullptr;    // per-group tile regions of the lane path
+    uint32_t *tb_gqw = nullptr;
+    size_t tb_gbase_bytes = 0, tb_gqw_bytes = 0;
+    std::vector<uint32_t> h_hist, h_gqw;
+    std::vector<uint64_t> h_gbase;
     uint32_t *wf_tb = nullptr;       // direction tiles of one-wave-per-pair pairs
     size_t wf_tb_bytes = 0;
     uint64_t *wf_tbo = nullptr;
@@ -177,6 +183,8 @@ extern "C" int32_t gasal2_tune_count(void) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     return (int32_t)g_tune.size();
 }
+
+extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
 
 extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
 extern "C" int32_t gasal2_get_graph_mode(void) { return g_graph_mode; }
@@ -735,7 +743,7 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 
 static void impl_free(Impl *m) {
     if (!m) return;
-    void *ptrs[] = {m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
+    void *ptrs[] = {m->tb_gbase, m->tb_gqw, m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
                     m->sub_nops, m->sub_cigar,
                     m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
                     m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
@@ -1106,7 +1114,61 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     // direction tiles of the one-lane-per-pair kernels: groups after the long pairs
     if (tb_dp && !px.banded) {
         const uint64_t lgroups = ((uint64_t)(n - nlong) + 63) / 64;
-        ensure(m->tb, m->tb_bytes, (size_t)lgroups * et * (8 * (size_t)eq) * 64 * 4 + 256);
+        const uint64_t one_size = lgroups * et * (8 * (uint64_t)eq) * 64 * 4;
+        if (one_size <= g_tb_one_size_max) {
+            ensure(m->tb, m->tb_bytes, (size_t)one_size + 256);
+        } else {
+            // Per-group regions.  The device orders the lane-path pairs by
+            // descending shape bucket (k_keys), so the buckets falling in each
+            // 64-slot group are fixed; each group gets the largest bucket
+            // edges among them.
+            auto shape8 = [](uint32_t x) -> uint32_t {
+                if (x < 128u) return x;
+                uint32_t e = 31u - (uint32_t)__builtin_clz(x);
+                uint32_t v = 128u + (e - 7u) * 16u + ((x >> (e - 4u)) & 15u);
+                return v > 255u ? 255u : v;
+            };
+            auto edge8 = [](uint32_t v) -> uint32_t {   // largest x with shape8(x) == v
+                if (v < 128u) return v;
+                if (v == 255u) return 0xFFFFFFFFu;
+                uint32_t e = 7u + (v - 128u) / 16u, mant = (v - 128u) % 16u;
+                return ((17u + mant) << (e - 4u)) - 1u;
+            };
+            m->h_hist.assign(1u << 16, 0u);
+            const bool wfa = d.wf_blocks > 0;
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t kq = g2_regs(hq[k]), kt = g2_regs(ht[k]);
+                if (wfa && kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) continue;
+                m->h_hist[(shape8(kq) << 8) | shape8(kt)]++;
+            }
+            m->h_gbase.assign(lgroups, 0);
+            m->h_gqw.assign(lgroups, 0);
+            std::vector<uint32_t> gtw(lgroups, 0);
+            uint64_t slot = 0;
+            for (int32_t key = (1 << 16) - 1; key >= 0; key--) {
+                uint32_t c = m->h_hist[key];
+                if (!c) continue;
+                const uint32_t qe = std::min(edge8((uint32_t)key >> 8), eq), te = std::min(edge8((uint32_t)key & 255u), et);
+                const uint64_t g0 = slot / 64, g1 = (slot + c - 1) / 64;
+                for (uint64_t g = g0; g <= g1 && g < lgroups; g++) {
+                    m->h_gqw[g] = std::max(m->h_gqw[g], qe);
+                    gtw[g] = std::max(gtw[g], te);
+                }
+                slot += c;
+            }
+            uint64_t tot = 0;   // dwords
+            for (uint64_t g = 0; g < lgroups; g++) {
+                m->h_gbase[g] = tot;
+                tot += (uint64_t)gtw[g] * m->h_gqw[g] * 64 * 8;
+            }
+            ensure(m->tb, m->tb_bytes, (size_t)tot * 4 + 256);
+            ensure(m->tb_gbase, m->tb_gbase_bytes, (size_t)lgroups * 8 + 256);
+            ensure(m->tb_gqw, m->tb_gqw_bytes, (size_t)lgroups * 4 + 256);
+            G2_CHECK(hipMemcpyAsync(m->tb_gbase, m->h_gbase.data(), (size_t)lgroups * 8, hipMemcpyHostToDevice, s->str));
+            G2_CHECK(hipMemcpyAsync(m->tb_gqw, m->h_gqw.data(), (size_t)lgroups * 4, hipMemcpyHostToDevice, s->str));
+            d.tb_gbase = m->tb_gbase;
+            d.tb_gqw = m->tb_gqw;
+        }
         d.tb = m->tb;
         d.tb_tw = et;
         d.tb_qw8 = 8 * eq;

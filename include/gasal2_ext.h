@@ -136,6 +136,13 @@ int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint
 int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params,
                           const uint32_t *idx, uint32_t cnt, uint64_t max_tb_bytes);
 
+/* Direction tiles of the one-lane-per-pair kernels (WITH_TB) use one region
+ * size for every 64-pair group while that totals at most `bytes` (default
+ * 64 GiB); above it each group gets a region sized for its own pairs' shape
+ * bucket (uneven batches).  Results are identical either way.  No reference
+ * counterpart (src/ctors.cpp:48-53 sizes for the maximum lengths). */
+void gasal2_set_tile_budget(uint64_t bytes);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape

@@ -104,3 +104,17 @@ def test_local_tiny_edge_cases():
     for start in (g.WITHOUT_START, g.WITH_START, g.WITH_TB):
         got, ref = run_both(b, g.LOCAL, start, True, n_penalty=1)
         assert_parity(got, ref, g.LOCAL, start, True, b)
+
+
+@pytest.mark.parametrize("algo,second", [(g.LOCAL, True), (g.LOCAL, False), (g.GLOBAL, False)])
+def test_per_group_tile_regions(algo, second):
+    """WITH_TB with per-group direction-tile regions (forced by a zero budget)
+    on a batch of uneven lengths, with and without the one-wave path."""
+    b = g.synth_batch(3, 1010, 600)
+    L = g.lib()
+    L.gasal2_set_tile_budget(0)
+    try:
+        got, ref = run_both(b, algo, g.WITH_TB, second, n_penalty=1)
+        assert_parity(got, ref, algo, g.WITH_TB, second, b)
+    finally:
+        L.gasal2_set_tile_budget(64 << 30)
