@@ -991,12 +991,12 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                     const int32_t s = LUT ? __builtin_amdgcn_sbfe(gb[m], (rbase & 7u) << 2, 4) : sub_global(c, rbase, gb[m]);
                     const int32_t tmp = p[m] + s;
                     int32_t hm = max(max(tmp, f[m]), e);
-                    if (TB) {
-                        const int sh = 28 - ((m - 1) << 2);
-                        const uint32_t m_or_x = tmp >= p[m] ? 0u : 1u;
-                        dir |= hm == tmp ? (m_or_x << sh) : (hm == f[m] ? (3u << sh) : (2u << sh));
-                        dir |= (tmp - gapoe) > (f[m] - gape) ? 0u : (1u << (sh + 3));
-                        dir |= (tmp - gapoe) > (e - gape) ? 0u : (1u << (sh + 2));
+                    if (TB) {   // sign-bit direction format of k_local_fast (k_get_tb decodes it)
+                        const int32_t t = tmp - gapoe;
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - gape - t), 31);
+                        dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(e - gape - t), 31);
                     }
                     f[m] = max(tmp - gapoe, f[m] - gape);
                     e = max(tmp - gapoe, e - gape);
@@ -1373,7 +1373,8 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int32_t bbw = band ? (int32_t)d.band_bw[pr] : 0;
     uint32_t bflag = 0;
     if (band && (d.band_flags[pr] & 8u)) { d.n_ops[pr] = 0; return; }
-    const bool fastfmt = LOCALT && (band || wfp || (d.fast && d.enc_ok && census_fast(d.aux[0])));
+    // GLOBAL words are always in the sign-bit format (k_global)
+    const bool fastfmt = !LOCALT || band || wfp || (d.fast && d.enc_ok && census_fast(d.aux[0]));
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1422,13 +1423,15 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
                                          : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
             cell_op = (word >> (28 - 4 * (i & 7))) & 15u;
             if (fastfmt) {
+                // the cell the tile word belongs to: column col, row 8 strip + (i & 7)
                 const uint32_t nA = (cell_op >> 3) & 1u, nB = (cell_op >> 2) & 1u;
                 const uint32_t nC3 = (cell_op >> 1) & 1u, nC2 = cell_op & 1u;
-                if ((j >> 3) != q_key) { q_key = j >> 3; qword = Qw[q_key]; }
-                if ((i >> 3) != t_key) { t_key = i >> 3; tword = Tw[t_key]; }
-                const uint32_t qb = (qword >> (28 - 4 * (j & 7))) & 15u;
+                if ((col >> 3) != q_key) { q_key = col >> 3; qword = Qw[q_key]; }
+                if (strip != t_key) { t_key = strip; tword = Tw[t_key]; }
+                const uint32_t qb = (qword >> (28 - 4 * (col & 7))) & 15u;
                 const uint32_t tb = (tword >> (28 - 4 * (i & 7))) & 15u;
-                const uint32_t mx = sub_local(c, nval, qb, tb) < 0 ? 1u : 0u;
+                const int32_t sub = LOCALT ? sub_local(c, nval, qb, tb) : sub_global(c, qb, tb);
+                const uint32_t mx = sub < 0 ? 1u : 0u;
                 const uint32_t src = nA == 0 ? mx : (nB == 0 ? 3u : 2u);
                 cell_op = src | ((nC3 ^ 1u) << 3) | ((nC2 ^ 1u) << 2);
             }
