@@ -282,7 +282,8 @@ __device__ __forceinline__ bool census_fast(uint32_t m);
 
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
-    if (d.fast && d.enc_ok && census_fast(d.aux[0])) return;   // k_local_fast handles this batch
+    if (d.fast && (d.enc_ok || (START == S_WITHOUT_START && !SECOND && d.pk16)) && census_fast(d.aux[0]))
+        return;   // k_local_fast handles this batch
     LaneCtx x;
     // long pairs of a score-only batch sort first and run one wave per pair
     if (!lane_setup(d, x, 0, d.wf_blocks ? d.aux[2] : 0u)) return;
@@ -642,6 +643,105 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     }
 }
 
+// ------------------------------------------ LOCAL, packed 16-bit, one lane ----
+// Score and ends with two 8-row strips per lane, one in each 16-bit half of
+// every register (v_pk_add_u16 / v_pk_sub_i16 / v_pk_max_i16): the low half
+// runs strip 2k at column t, the high half strip 2k+1 at column t-1, whose top
+// row is the low half's bottom row of the previous step, kept in a register.
+// The high strip's bottom row goes to the boundary row for the next pair of
+// strips.  Used when every value fits int16 (H <= match * min(8QW, 8TW) <=
+// 32767; E and F never fall below -(mismatch + open + extend * 8)), so the
+// values are those of the int32 kernels.  Ends: per strip the first column
+// whose maximum beats the strip's running best, its first row; strips merged
+// in order (the reference's first strict maximum in (strip, column, row)).
+typedef short pk16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ pk16x2 as_pk(uint32_t u) { return __builtin_bit_cast(pk16x2, u); }
+__device__ __forceinline__ uint32_t as_u32(pk16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ pk16x2 pmax(pk16x2 a, pk16x2 b) { return __builtin_elementwise_max(a, b); }
+// {lo16(lo), lo16(hi)} and {hi16(lo), hi16(hi)}
+__device__ __forceinline__ uint32_t pack_lo(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x05040100u); }
+__device__ __forceinline__ uint32_t pack_hi(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x07060302u); }
+
+__device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census) {
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const pk16x2 gapoe2 = {(short)c.gapoe, (short)c.gapoe}, gape2 = {(short)c.gape, (short)c.gape};
+    const pk16x2 zero2 = {0, 0};
+    const int32_t QW = x.QW, TW = x.TW, C = 8 * QW;
+    uint32_t *__restrict__ bd = x.bd;
+    for (int32_t cc = 0; cc < C; cc++) bd[cc * WAVE] = 0u;
+    int32_t bv = 0, bs = 0, bc = 0, br = 0;   // best: h, strip, column, row in strip
+    for (int32_t s0 = 0; s0 < TW; s0 += 2) {
+        const bool hi_ok = s0 + 1 < TW;
+        uint32_t lutL[9], lutH[9];
+        const uint32_t gL = x.T[s0], gH = hi_ok ? x.T[s0 + 1] : 0u;
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            lutL[m] = score_lut(c, nval, census, (gL >> (32 - 4 * m)) & 15u);
+            lutH[m] = score_lut(c, nval, census, (gH >> (32 - 4 * m)) & 15u);
+        }
+        pk16x2 h[9], f[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = zero2; f[m] = zero2; }
+        uint32_t carry = 0;            // low strip's bottom (H | E << 16) of the previous step
+        uint32_t qshL = 0, qshH = 0;
+        int32_t lv = 0, lc = 0, lr = 0, hv = 0, hc = 0, hr = 0;
+        uint32_t rpac = 0;
+        for (int32_t t = 0; t <= C; t++) {
+            qshH = qshL;
+            uint32_t topL = 0;
+            if (t < C) {
+                const int kk = t & 7;
+                if (kk == 0) rpac = x.Q[t >> 3];
+                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                topL = bd[t * WAVE];
+            }
+            pk16x2 e = as_pk(pack_hi(topL, carry));
+            pk16x2 p = h[0];
+            h[0] = as_pk(pack_lo(topL, carry));
+            pk16x2 colmax = zero2;
+#pragma unroll
+            for (int m = 1; m < 9; m++) {
+                const uint32_t sl = (uint32_t)__builtin_amdgcn_sbfe(lutL[m], qshL, 4);
+                const uint32_t sh = (uint32_t)__builtin_amdgcn_sbfe(lutH[m], qshH, 4);
+                const pk16x2 tmp = p + as_pk(pack_lo(sl, sh));
+                const pk16x2 hm = pmax(pmax(pmax(tmp, f[m]), e), zero2);
+                const pk16x2 tt = tmp - gapoe2;
+                f[m] = pmax(tt, f[m] - gape2);
+                e = pmax(tt, e - gape2);
+                p = h[m];
+                h[m] = hm;
+                colmax = pmax(colmax, hm);
+            }
+            const uint32_t h8 = as_u32(h[8]), e8 = as_u32(e);
+            if (t >= 1) bd[(t - 1) * WAVE] = pack_hi(h8, e8);
+            carry = pack_lo(h8, e8);
+            if (t == 0) {   // the high strip starts at the next step: undo its column -1
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    h[m] = as_pk(as_u32(h[m]) & 0xFFFFu);
+                    f[m] = as_pk(as_u32(f[m]) & 0xFFFFu);
+                }
+            }
+            const int32_t cl = (int32_t)(as_u32(colmax) & 0xFFFFu), ch = (int32_t)(as_u32(colmax) >> 16);
+            if (t < C && cl > lv) {
+                lv = cl; lc = t;
+#pragma unroll
+                for (int m = 8; m >= 1; m--) if ((int32_t)(as_u32(h[m]) & 0xFFFFu) == cl) lr = m - 1;
+            }
+            if (t >= 1 && hi_ok && ch > hv) {
+                hv = ch; hc = t - 1;
+#pragma unroll
+                for (int m = 8; m >= 1; m--) if ((int32_t)(as_u32(h[m]) >> 16) == ch) hr = m - 1;
+            }
+        }
+        if (lv > bv) { bv = lv; bs = s0; bc = lc; br = lr; }
+        if (hv > bv) { bv = hv; bs = s0 + 1; bc = hc; br = hr; }
+    }
+    d.score[x.pair] = bv;
+    d.qend[x.pair] = bv > 0 ? bc : 0;
+    d.tend[x.pair] = bv > 0 ? bs * 8 + br : 0;
+}
+
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
@@ -657,6 +757,11 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         return;
     }
     if (!census_fast(census)) return;
+    if (START == S_WITHOUT_START && !SECOND && d.pk16) {
+        LaneCtx x;
+        if (lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) lane_local_pk(d, c, x, census);
+        return;
+    }
     if (!d.enc_ok) return;   // position encoding does not fit: k_local takes the short pairs
     LaneCtx x;
     if (!lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) return;

@@ -33,7 +33,8 @@ int g_error_count = 0;
 // one-wave-per-pair thresholds: target words, cells (64 per query word x target word)
 uint32_t g_wf_min_tw = 48, g_wf_min_cells = 1u << 20, g_wf_max_blocks = 2048;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
-uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all groups up to this many bytes
+uint64_t g_tb_one_size_max = 64ull << 30;
+int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16   // lane-path tiles: one size for all groups up to this many bytes
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
 // range); filled by the autotuner (gasal2_tune_set), consulted per batch.
@@ -185,6 +186,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 }
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
+extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
 
 extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
 extern "C" int32_t gasal2_get_graph_mode(void) { return g_graph_mode; }
@@ -1174,6 +1176,15 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         d.tb_qw8 = 8 * eq;
         s->packed_tb_matrices = (uint4 *)m->tb;
         s->packed_tb_matrix_size = m->tb_bytes / 16;
+    }
+    // Packed 16-bit LOCAL (score and ends): every value of the pairs the lane
+    // path runs fits int16 -- H <= match * min(8 eq, 8 et); E, F >= -(mismatch,
+    // N penalty) - open - 9 extend -- and the policy allows it.
+    d.pk16 = 0;
+    if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g_pk16_mode) {
+        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
+        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
+        if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
     }
     // LOCAL lane-per-pair score-table kernel: position encoding
     // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs

@@ -143,6 +143,11 @@ int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint
  * counterpart (src/ctors.cpp:48-53 sizes for the maximum lengths). */
 void gasal2_set_tile_budget(uint64_t bytes);
 
+/* Packed 16-bit DP (default on): LOCAL score-and-ends batches whose values fit
+ * int16 run two 8-row strips per lane in the halves of each register; 0 keeps
+ * the int32 kernels.  Results are identical.  No reference counterpart. */
+void gasal2_set_pk16(int32_t on);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape
