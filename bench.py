@@ -47,6 +47,9 @@ def main():
                     help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
                          "band: learned band vs full DP on 1-10 kbp pairs (train seed 5, held-out seed 6)")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for --workload band")
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="also stream the batch in this many sub-batches through one storage and through two "
+                         "alternating storages (H2D of one overlapping the other's compute); 0 = skip")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--check-widen", type=int, nargs="+", default=[128],
                     help="check-band widening(s) (diagonals each side) to evaluate; the first one is reported as value")
@@ -203,6 +206,8 @@ def main():
                "gpu_agreement_on_sample": float(ok.mean())}
         log(f"cpu oracle: {cpu}")
 
+    pipe = pipeline_measure(g, batch, p, args.pipeline) if args.pipeline > 1 else None
+
     line = {
         "metric": "GCUPS + % int-VALU roofline",
         "value": round(gcups, 3),
@@ -230,8 +235,62 @@ def main():
     }
     if lane_only:
         line["lane_only"] = lane_only
+    if pipe:
+        line["pipeline"] = pipe
     print(json.dumps(line), flush=True)
     a.close()
+
+
+def pipeline_measure(g, batch, p, parts):
+    """Whole batch through gasal_aln_async in `parts` sub-batches, host fill to
+    results on the host: one storage (each batch waits for the previous one)
+    vs two storages used alternately (batch k+1's fill and H2D overlap batch
+    k's compute and D2H on the other storage's stream).  Outputs compared."""
+    n = batch.n
+    cuts = np.linspace(0, n, parts + 1).astype(int)
+    subs = [batch.subset(np.arange(cuts[i], cuts[i + 1])) for i in range(parts)]
+    mq, mt, mn = int(batch.qlen.max()), int(batch.tlen.max()), int(max(s.n for s in subs))
+    keys = ("score", "qend", "tend", "qstart", "tstart")
+
+    def run(nst):
+        al = [g.Aligner(p, mq, mt, mn, scores=(1, 4, 6, 1), n_penalty=1) for _ in range(nst)]
+        out = [None] * parts
+        try:
+            for a in al:   # warm: workspace and graph for the sub-batch shape
+                qb, tb, m = a.fill(subs[0], per_pair=False)
+                a.align_filled(qb, tb, m)
+            t0 = time.perf_counter()
+            pending = [None] * nst
+            for k, sb in enumerate(subs):
+                a = al[k % nst]
+                if pending[k % nst] is not None:   # this storage's previous batch: wait, collect
+                    j, qb0, m0 = pending[k % nst]
+                    while g.gasal_is_aln_async_done(a.s) != 0:
+                        pass
+                    out[j] = {kk: a._collect(m0, qb0)[kk].copy() for kk in keys}
+                qb, tb, m = a.fill(sb, per_pair=False)
+                g.gasal_aln_async(a.s, qb, tb, m, a.params)
+                pending[k % nst] = (k, qb, m)
+            for i, a in enumerate(al):
+                if pending[i] is not None:
+                    j, qb0, m0 = pending[i]
+                    while g.gasal_is_aln_async_done(a.s) != 0:
+                        pass
+                    out[j] = {kk: a._collect(m0, qb0)[kk].copy() for kk in keys}
+            return time.perf_counter() - t0, out
+        finally:
+            for a in al:
+                a.close()
+
+    t1, o1 = run(1)
+    t2, o2 = run(2)
+    same = all(np.array_equal(o1[i][k], o2[i][k]) for i in range(parts) for k in keys)
+    cells = float(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+    res = {"parts": parts, "one_storage_s": round(t1, 4), "two_storages_s": round(t2, 4),
+           "gain": round(t1 / t2, 3), "outputs_identical": bool(same),
+           "gcups_one": round(cells / t1 / 1e9, 2), "gcups_two": round(cells / t2 / 1e9, 2)}
+    log(f"pipeline: {res}")
+    return res
 
 
 def band_main(args):
