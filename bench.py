@@ -55,6 +55,8 @@ def main():
                     help="check-band widening(s) (diagonals each side) to evaluate; the first one is reported as value")
     ap.add_argument("--dispatch", type=int, default=0,
                     help="kernel dispatch policy (0 auto, 2 one lane per pair only, 3 one wave per pair)")
+    ap.add_argument("--wf", type=int, nargs=3, default=None, metavar=("MIN_TW", "MIN_CELLS", "MAX_BLOCKS"),
+                    help="one-wave-per-pair switch thresholds (gasal2_set_wavefront)")
     ap.add_argument("--compare-lane-only", action="store_true",
                     help="also time dispatch policy 2 (one lane per pair only) on the same batch")
     ap.add_argument("--cpu-pairs", type=int, default=-1,
@@ -115,6 +117,8 @@ def main():
     log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
 
     L = g.lib()
+    if args.wf:
+        L.gasal2_set_wavefront(*args.wf)
 
     def timed_steps(policy):
         L.gasal2_set_dispatch(policy)
@@ -222,7 +226,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "mode": mode, "dispatch": args.dispatch, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
+                   "mode": mode, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
         "roofline": {"bound": "valu",
                      "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",

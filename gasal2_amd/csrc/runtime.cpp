@@ -32,6 +32,10 @@ int g_error_mode = 0;
 int g_error_count = 0;
 // one-wave-per-pair thresholds: target words, cells (64 per query word x target word)
 uint32_t g_wf_min_tw = 48, g_wf_min_cells = 1u << 20, g_wf_max_blocks = 2048;
+bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for every mode
+// WITH_TB default (measured on the uneven batch: the traceback's per-lane
+// tiles favour sending more pairs to one wave each)
+constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;
 int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16   // lane-path tiles: one size for all groups up to this many bytes
@@ -194,6 +198,7 @@ extern "C" void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cel
     g_wf_min_tw = min_target_words ? min_target_words : 1u;
     g_wf_min_cells = min_cells;
     g_wf_max_blocks = max_blocks ? max_blocks : 1u;
+    g_wf_user = true;
 }
 
 extern "C" int32_t gasal2_device_info(int32_t *n_cu, int32_t *clock_khz, char *name, int32_t name_cap) {
@@ -1064,6 +1069,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         g2_get_dispatch() != 2) {
         const bool all = g2_get_dispatch() == 3;
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
+        if (!g_wf_user && op.start == WITH_TB) { min_tw = WF_TB_MIN_TW; min_cells = WF_TB_MIN_CELLS; }
         {
             uint64_t sum_q = 0;
             for (uint32_t k = 0; k < n; k++) sum_q += hq[k];
