@@ -34,6 +34,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def work_cells(batch, mode, k_band=32):
+    """Cells of the metric: sum of Lq*Lt, or for the tile-banded kernel the
+    nominal band, 64 cells per (strip, query word) inside the band
+    (src/kernels/banded.h:83-85: words [max(0, i - other + 1), min(band + i, QW)))."""
+    if mode != "banded":
+        return int(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+    QW = (batch.qlen.astype(np.int64) + 7) // 8
+    TW = (batch.tlen.astype(np.int64) + 7) // 8
+    band = k_band >> 3
+    other = TW - (QW - band)
+    tot = 0
+    for i in range(int(TW.max())):
+        lo = np.maximum(0, i - other + 1)
+        hi = np.minimum(band + i, QW)
+        tot += int(np.sum(np.where(i < TW, np.clip(hi - lo, 0, None), 0)))
+    return 64 * tot
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -41,9 +59,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--mode", default=None,
-                    choices=["local_tb_second", "local_tb", "local_score", "global", "semi_global"],
+                    choices=["local_tb_second", "local_tb", "local_score", "global", "semi_global", "ksw", "banded"],
                     help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
-    ap.add_argument("--workload", default="full", choices=["full", "uneven", "band"],
+    ap.add_argument("--workload", default="full", choices=["full", "uneven", "seed", "band"],
                     help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
                          "band: learned band vs full DP on 1-10 kbp pairs (train seed 5, held-out seed 6)")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for --workload band")
@@ -84,12 +102,17 @@ def main():
         mode = args.mode or "local_tb_second"
         workload = ("full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
                     "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries")
+    elif args.workload == "seed":
+        batch = g.synth_batch(4, 4, args.pairs if args.pairs != 1_000_000 else 2_000_000)
+        mode = args.mode or "ksw"
+        workload = ("seed extension: 2M pairs Lq, Lt ~ U{100..250}, t = q mutated 3%, h0 ~ U{10..60}; "
+                    "KSW in the reference's form (no band, z-drop 0) or tile-banded LOCAL with k_band 32")
     else:
         batch = g.synth_batch(3, 3, args.pairs if args.pairs != 1_000_000 else 500_000)
         mode = args.mode or "local_score"
         workload = ("uneven-length batch: 500k pairs, Lq log-normal (median 400, clipped 50-5000 bp), "
                     "Lt = Lq * U[0.8,1.2], 10% divergence; one launch, per-pair lane/wave switch")
-    cells = int(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+    cells = work_cells(batch, mode)
     log(f"generated {batch.n} pairs, {cells / 1e9:.2f} G cells in {time.time() - t0:.1f}s")
 
     if mode == "local_tb_second":
@@ -103,6 +126,12 @@ def main():
         ops = OPS_SCORE
     elif mode == "global":
         p = g.Parameters(algo=g.GLOBAL, start_pos=g.WITH_START, isReverseComplement=True)
+        ops = OPS_SCORE
+    elif mode == "ksw":
+        p = g.Parameters(algo=g.KSW, start_pos=g.WITHOUT_START)
+        ops = OPS_SCORE
+    elif mode == "banded":
+        p = g.Parameters(algo=g.BANDED, start_pos=g.WITHOUT_START, k_band=32)
         ops = OPS_SCORE
     else:
         p = g.Parameters(algo=g.SEMI_GLOBAL, start_pos=g.WITH_START, isReverseComplement=True,
@@ -178,8 +207,8 @@ def main():
             sub = batch.subset(np.arange(m))
             t0 = time.perf_counter()
             r = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
-                        head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, is_rc=True,
-                        n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
+                        head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, k_band=p.k_band,
+                        is_rc=bool(p.isReverseComplement), n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
             return sub, r, time.perf_counter() - t0
 
         if args.cpu_pairs > 0:
@@ -189,15 +218,14 @@ def main():
             _, _, s0 = oracle_on(m0)
             m = int(min(batch.n, max(m0, m0 * args.cpu_seconds / max(s0, 1e-3))))
         sub, ref, cpu_s = oracle_on(m)
-        sub_cells = int(np.dot(sub.qlen.astype(np.int64), sub.tlen.astype(np.int64)))
-        keys = [k for k in ("score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend", "s_tend")
-                if res.get(k) is not None]
-        if mode != "local_tb_second":
-            keys = [k for k in keys if not k.startswith("s_")]
-        if mode == "global":
-            keys = ["score"]
-        if mode == "local_score":
-            keys = [k for k in keys if k in ("score", "qend", "tend")]
+        sub_cells = work_cells(sub, mode)
+        # outputs the mode defines (reference src/res.cpp:8-73)
+        keys = ["score"] if p.algo == g.GLOBAL else ["score", "qend", "tend"]
+        if (p.algo == g.LOCAL and p.start_pos in (g.WITH_START, g.WITH_TB)) or \
+                (p.algo == g.SEMI_GLOBAL and p.start_pos == g.WITH_START):
+            keys += ["qstart", "tstart"]
+        if p.secondBest and p.algo in (g.LOCAL, g.SEMI_GLOBAL):
+            keys += ["s_score", "s_qend", "s_tend"]
         ok = np.ones(m, bool)
         for k in keys:
             ok &= res[k][:m] == ref[k]

@@ -55,6 +55,7 @@ struct G2Dev {
     const uint32_t *tb_gqw;
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
+    int32_t ksw16;                   // KSW rows as packed uint16 (H, E): values fit 16 bits
     uint32_t *aux;                   // small device counters (work queue etc.)
     // score-table kernels (k_local_fast, k_global / k_semi LUT variants): 0 =
     // off, 1 = launched beside the generic kernel, the nibble census in aux[0]

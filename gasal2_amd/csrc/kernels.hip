@@ -1375,6 +1375,9 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // ------------------------------------------------------------------ KSW -----
 // gasal_ksw_kernel<B> (src/kernels/ksw_kernel_template.h:46-199).  The per-pair
 // eh[] row lives in device scratch laid out [column][lane] (int2 per entry).
+// EH16: H and E of the eh row packed as two uint16 (every KSW value is >= 0 and
+// the host checked h0 + match * min(Lq, Lt) <= 65535), halving the row's traffic.
+template <bool EH16>
 __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
     LaneCtx x;
     if (!lane_setup(d, x)) return;
@@ -1384,13 +1387,21 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
     const uint32_t h0 = d.seeds[x.pair];
     const int o_del = c.gapo, o_ins = c.gapo, e_del = c.gape, e_ins = c.gape;
     const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
-    int2 *__restrict__ eh = reinterpret_cast<int2 *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
-    for (uint32_t jj = 0; jj <= qlen + 1; jj++) eh[(size_t)jj * WAVE] = make_int2(0, 0);
+    int2 *__restrict__ eh2 = reinterpret_cast<int2 *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
+    uint32_t *__restrict__ eh1 = reinterpret_cast<uint32_t *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
+    auto ld = [&](int k) -> int2 {
+        if (EH16) { const uint32_t u = eh1[(size_t)k * WAVE]; return make_int2((int)(u & 0xFFFFu), (int)(u >> 16)); }
+        return eh2[(size_t)k * WAVE];
+    };
+    auto st = [&](int k, int2 v) {
+        if (EH16) eh1[(size_t)k * WAVE] = (uint32_t)v.x | ((uint32_t)v.y << 16);
+        else eh2[(size_t)k * WAVE] = v;
+    };
+    for (uint32_t jj = 0; jj <= qlen + 1; jj++) st((int)jj, make_int2(0, 0));
     int j;
-    eh[0].x = (int32_t)h0;                                                                 // :78-81
-    eh[WAVE].x = h0 > (uint32_t)oe_ins ? (int32_t)(h0 - (uint32_t)oe_ins) : 0;
-    for (j = 2; j <= (int)qlen && eh[(size_t)(j - 1) * WAVE].x > e_ins; ++j)
-        eh[(size_t)j * WAVE].x = eh[(size_t)(j - 1) * WAVE].x - e_ins;
+    st(0, make_int2((int32_t)h0, 0));                                                      // :78-81
+    st(1, make_int2(h0 > (uint32_t)oe_ins ? (int32_t)(h0 - (uint32_t)oe_ins) : 0, 0));
+    for (j = 2; j <= (int)qlen && ld(j - 1).x > e_ins; ++j) st(j, make_int2(ld(j - 1).x - e_ins, 0));
     int max = (int32_t)h0, max_i = -1, max_j = -1, max_ie = -1, gscore = -1;
     int beg = 0, end = (int)qlen;
     for (uint32_t tt = 0; tt < target_batch_regs; tt++) {
@@ -1407,10 +1418,14 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
                 h1 = 0;
             // j after the tile loops (only the last tile's values matter: :113-152)
             const int T = (int)(qlen >> 3);
-            for (int jj = beg; jj < end; jj++) {
-                const uint32_t rpac = x.Q[jj >> 3];
+            // Columns run from 0 for every lane, cells before the lane's own beg
+            // skipped, so the wave's eh accesses stay on one row (coalesced).
+            uint32_t rpac = 0;
+            for (int jj = 0; jj < end; jj++) {
+                if ((jj & 7) == 0) rpac = x.Q[jj >> 3];
+                if (jj < beg) continue;
                 const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
-                int2 pv = eh[(size_t)jj * WAVE];
+                int2 pv = ld(jj);
                 int h, M = pv.x, e = pv.y;
                 pv.x = h1;
                 const int32_t s = sub_local(c, nval, rbase, gbase);
@@ -1423,7 +1438,7 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
                 t = M - oe_del; t = t > 0 ? t : 0;
                 e -= e_del; e = e > t ? e : t;
                 pv.y = e;
-                eh[(size_t)jj * WAVE] = pv;
+                st(jj, pv);
                 t = M - oe_ins; t = t > 0 ? t : 0;
                 f -= e_ins; f = f > t ? f : t;
             }
@@ -1432,16 +1447,16 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
                 j = be < 8 * T ? 8 * T : be;
                 j = j > 8 * T + 7 ? 8 * T + 7 : j;
             }
-            eh[(size_t)end * WAVE] = make_int2(h1, 0);                                     // :153-154
+            st(end, make_int2(h1, 0));                                                     // :153-154
             if (j == (int)qlen) {
                 max_ie = gscore > h1 ? max_ie : i;
                 gscore = gscore > h1 ? gscore : h1;
             }
             if (m == 0) break;                                                             // :159-160
             if (m > max) { max = m; max_i = i; max_j = mj; }
-            for (j = beg; (j < end) && eh[(size_t)j * WAVE].x == 0 && eh[(size_t)j * WAVE].y == 0; ++j) ;
+            for (j = beg; (j < end) && ld(j).x == 0 && ld(j).y == 0; ++j) ;
             beg = j;
-            for (j = end; (j >= beg) && eh[(size_t)j * WAVE].x == 0 && eh[(size_t)j * WAVE].y == 0; --j) ;
+            for (j = end; (j >= beg) && ld(j).x == 0 && ld(j).y == 0; --j) ;
             end = j + 2 < (int)qlen ? j + 2 : (int)qlen;
         }
     }
@@ -1771,7 +1786,8 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
             break;
         case A_KSW:
-            k_ksw<<<grid, blk, 0, s>>>(d, sc);
+            if (d.ksw16) k_ksw<true><<<grid, blk, 0, s>>>(d, sc);
+            else k_ksw<false><<<grid, blk, 0, s>>>(d, sc);
             break;
         default:
             break;  // UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21)

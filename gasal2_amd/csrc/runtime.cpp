@@ -1045,6 +1045,14 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     d.bnd = m->bnd; d.bnd_cols = bnd_cols;
     d.tb = m->tb; d.tb_tw = tw; d.tb_qw8 = 8 * qw;
     d.eh = m->eh; d.eh_cols = eh_cols;
+    d.ksw16 = 0;
+    if (op.algo == KSW && s->host_seed_scores && !px.sub) {   // every KSW value lies in [0, h0 + match * min(Lq, Lt)]
+        uint64_t hmax = 0;
+        for (uint32_t k = 0; k < n; k++)
+            hmax = std::max<uint64_t>(hmax, (uint64_t)s->host_seed_scores[k] +
+                                                (uint64_t)std::max(g_scores.match, 0) * std::min(hq[k], ht[k]));
+        if (hmax <= 65535 && g_pk16_mode) d.ksw16 = 1;
+    }
     if (!m->aux) G2_CHECK(hipMalloc((void **)&m->aux, 256));
     d.aux = m->aux;
 
