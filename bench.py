@@ -34,10 +34,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def work_cells(batch, mode, k_band=32):
+def work_cells(batch, mode, k_band=32, ksw_w=-1):
     """Cells of the metric: sum of Lq*Lt, or for the tile-banded kernel the
     nominal band, 64 cells per (strip, query word) inside the band
-    (src/kernels/banded.h:83-85: words [max(0, i - other + 1), min(band + i, QW)))."""
+    (src/kernels/banded.h:83-85: words [max(0, i - other + 1), min(band + i, QW))),
+    or for KSW with a band half-width w the cells |j - i| <= w (w clamped per
+    pair to Lq*match - gapo)/gape + 1 as the kernel does)."""
+    if mode == "ksw" and ksw_w >= 0:
+        Lq = batch.qlen.astype(np.int64)
+        Lt = batch.tlen.astype(np.int64)
+        w = np.minimum(ksw_w, np.maximum(1, (Lq * 1 - 6) // 1 + 1))
+        tot = 0
+        for i in range(int(Lt.max())):
+            c = np.minimum(Lq, i + w + 1) - np.maximum(0, i - w)
+            tot += int(np.sum(np.where(i < Lt, np.clip(c, 0, None), 0)))
+        return tot
     if mode != "banded":
         return int(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
     QW = (batch.qlen.astype(np.int64) + 7) // 8
@@ -77,6 +88,8 @@ def main():
                     help="one-wave-per-pair switch thresholds (gasal2_set_wavefront)")
     ap.add_argument("--compare-lane-only", action="store_true",
                     help="also time dispatch policy 2 (one lane per pair only) on the same batch")
+    ap.add_argument("--ksw-ext", type=int, nargs=2, default=None, metavar=("W", "ZDROP"),
+                    help="KSW band half-width and z-drop (seed extension as bwa; default: the reference's none, 0)")
     ap.add_argument("--cpu-pairs", type=int, default=-1,
                     help="oracle sample for cpu_baseline: -1 = sized to about --cpu-seconds, 0 = skip")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -112,7 +125,10 @@ def main():
         mode = args.mode or "local_score"
         workload = ("uneven-length batch: 500k pairs, Lq log-normal (median 400, clipped 50-5000 bp), "
                     "Lt = Lq * U[0.8,1.2], 10% divergence; one launch, per-pair lane/wave switch")
-    cells = work_cells(batch, mode)
+    kw, kz = args.ksw_ext if args.ksw_ext else (-1, 0)
+    if args.ksw_ext:
+        workload += f"; KSW band w={kw}, z-drop {kz}"
+    cells = work_cells(batch, mode, ksw_w=kw)
     log(f"generated {batch.n} pairs, {cells / 1e9:.2f} G cells in {time.time() - t0:.1f}s")
 
     if mode == "local_tb_second":
@@ -146,6 +162,7 @@ def main():
     log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
 
     L = g.lib()
+    g.set_ksw_extension(kw, kz)
     if args.wf:
         L.gasal2_set_wavefront(*args.wf)
 
@@ -208,7 +225,8 @@ def main():
             t0 = time.perf_counter()
             r = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
                         head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, k_band=p.k_band,
-                        is_rc=bool(p.isReverseComplement), n_penalty=1, n_threads=args.cpu_threads, with_cigar=False)
+                        is_rc=bool(p.isReverseComplement), n_penalty=1, n_threads=args.cpu_threads, with_cigar=False,
+                        ksw_w=kw, ksw_zdrop=kz)
             return sub, r, time.perf_counter() - t0
 
         if args.cpu_pairs > 0:
@@ -218,7 +236,7 @@ def main():
             _, _, s0 = oracle_on(m0)
             m = int(min(batch.n, max(m0, m0 * args.cpu_seconds / max(s0, 1e-3))))
         sub, ref, cpu_s = oracle_on(m)
-        sub_cells = work_cells(sub, mode)
+        sub_cells = work_cells(sub, mode, ksw_w=kw)
         # outputs the mode defines (reference src/res.cpp:8-73)
         keys = ["score"] if p.algo == g.GLOBAL else ["score", "qend", "tend"]
         if (p.algo == g.LOCAL and p.start_pos in (g.WITH_START, g.WITH_TB)) or \

@@ -171,6 +171,7 @@ _SIGS = {
     "gasal2_set_graph_mode": (None, [C.c_int32]),
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
     "gasal2_set_pk16": (None, [C.c_int32]),
+    "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_tune_clear": (None, []),
     "gasal2_tune_count": (C.c_int32, []),
@@ -317,6 +318,13 @@ def gasal_set_device(gpu_select=0, is_printing_prop=True):
 
 def set_n_code(n_code):
     lib().gasal2_set_n_code(n_code)
+
+
+def set_ksw_extension(w=-1, zdrop=0):
+    """KSW seed-extension settings: band half-width w (< 0: none) and z-drop
+    (0: off).  The defaults reproduce the reference's KSW kernel, which has
+    neither (src/kernels/ksw_kernel_template.h:27-29)."""
+    lib().gasal2_set_ksw_extension(int(w), int(zdrop))
 
 
 def set_n_penalty(penalty):

@@ -1378,7 +1378,7 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // EH16: H and E of the eh row packed as two uint16 (every KSW value is >= 0 and
 // the host checked h0 + match * min(Lq, Lt) <= 65535), halving the row's traffic.
 template <bool EH16>
-__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
+__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop) {
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const int32_t nval = c.has_np ? -c.np : 0;
@@ -1404,13 +1404,32 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
     for (j = 2; j <= (int)qlen && ld(j - 1).x > e_ins; ++j) st(j, make_int2(ld(j - 1).x - e_ins, 0));
     int max = (int32_t)h0, max_i = -1, max_j = -1, max_ie = -1, gscore = -1;
     int beg = 0, end = (int)qlen;
-    for (uint32_t tt = 0; tt < target_batch_regs; tt++) {
+    if (w >= 0) {   // band extension: w clamped as bwa's ksw_extend2 does (the reference has no band)
+        int mx = (int)((double)((int)qlen * (c.match > 0 ? c.match : 0) - c.gapo) / c.gape + 1.0);
+        mx = mx > 1 ? mx : 1;
+        w = w < mx ? w : mx;
+    }
+    // z-drop (extension; 0 in the reference) ends the whole extension as in
+    // bwa's ksw_extend2; the m == 0 break keeps the reference's tile-local exit
+    bool zstop = false;
+    int jlo = 0;   // wave-uniform lower bound of the active lanes' beg, a multiple of 8
+    for (uint32_t tt = 0; tt < target_batch_regs && !zstop; tt++) {
+        // Every lane still extending is active here (lanes that left the
+        // previous tile on m == 0 rejoin), and beg never decreases: move the
+        // wave-wide column start up by 8 while no lane's beg lies below the
+        // next word.  Bounded: every beg is at most max(Lq, Lt).
+        while (__ballot(beg < jlo + 8) == 0) jlo += 8;
         const uint32_t gpac = x.T[tt];
         for (uint32_t tb = 0; tb < 8; tb++) {
             const int i = (int)(tt * 8 + tb);
             if (i >= (int)tlen) break;                                                     // :98-99
             const uint32_t gbase = (gpac >> (32 - (tb + 1) * 4)) & 0x0Fu;
             int t, f = 0, h1, m = 0, mj = -1;
+            if (w >= 0) {
+                if (beg < i - w) beg = i - w;
+                if (end > i + w + 1) end = i + w + 1;
+                if (end > (int)qlen) end = (int)qlen;
+            }
             if (beg == 0) {
                 h1 = (int32_t)(h0 - (uint32_t)(o_del + e_del * (i + 1)));
                 if (h1 < 0) h1 = 0;
@@ -1418,10 +1437,11 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
                 h1 = 0;
             // j after the tile loops (only the last tile's values matter: :113-152)
             const int T = (int)(qlen >> 3);
-            // Columns run from 0 for every lane, cells before the lane's own beg
-            // skipped, so the wave's eh accesses stay on one row (coalesced).
+            // Columns run from one wave-wide start for every lane, cells before
+            // the lane's own beg skipped, so the wave's eh accesses stay on one
+            // row (coalesced).
             uint32_t rpac = 0;
-            for (int jj = 0; jj < end; jj++) {
+            for (int jj = jlo; jj < end; jj++) {
                 if ((jj & 7) == 0) rpac = x.Q[jj >> 3];
                 if (jj < beg) continue;
                 const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
@@ -1453,7 +1473,16 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c) {
                 gscore = gscore > h1 ? gscore : h1;
             }
             if (m == 0) break;                                                             // :159-160
-            if (m > max) { max = m; max_i = i; max_j = mj; }
+            if (m > max) {
+                max = m; max_i = i; max_j = mj;
+            } else if (zdrop > 0) {                                                        // :164-170
+                if (i - max_i > mj - max_j) {
+                    if (max - m - ((i - max_i) - (mj - max_j)) * e_del > zdrop) zstop = true;
+                } else {
+                    if (max - m - ((mj - max_j) - (i - max_i)) * e_ins > zdrop) zstop = true;
+                }
+                if (zstop) break;
+            }
             for (j = beg; (j < end) && ld(j).x == 0 && ld(j).y == 0; ++j) ;
             beg = j;
             for (j = end; (j >= beg) && ld(j).x == 0 && ld(j).y == 0; --j) ;
@@ -1786,8 +1815,8 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
             break;
         case A_KSW:
-            if (d.ksw16) k_ksw<true><<<grid, blk, 0, s>>>(d, sc);
-            else k_ksw<false><<<grid, blk, 0, s>>>(d, sc);
+            if (d.ksw16) k_ksw<true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+            else k_ksw<false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
             break;
         default:
             break;  // UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21)

@@ -38,7 +38,8 @@ bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for ever
 constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;
-int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16   // lane-path tiles: one size for all groups up to this many bytes
+int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16
+int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)   // lane-path tiles: one size for all groups up to this many bytes
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
 // range); filled by the autotuner (gasal2_tune_set), consulted per batch.
@@ -191,6 +192,10 @@ extern "C" int32_t gasal2_tune_count(void) {
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
+extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
+    g_ksw_w = w;
+    g_ksw_zdrop = zdrop > 0 ? zdrop : 0;
+}
 
 extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
 extern "C" int32_t gasal2_get_graph_mode(void) { return g_graph_mode; }
@@ -959,6 +964,8 @@ G2Opts opts_of(const Parameters *p) {
     o.banded = 0;
     o.skip_prep = 0;
     o.prep_only = 0;
+    o.ksw_w = g_ksw_w;
+    o.ksw_zdrop = g_ksw_zdrop;
     return o;
 }
 

@@ -148,6 +148,16 @@ void gasal2_set_tile_budget(uint64_t bytes);
  * the int32 kernels.  Results are identical.  No reference counterpart. */
 void gasal2_set_pk16(int32_t on);
 
+/* KSW extension (seed extension as bwa's ksw_extend2, which the reference's
+ * KSW kernel transcribes with no band and z-drop 0,
+ * src/kernels/ksw_kernel_template.h:27-29,62): w >= 0 bands row i to query
+ * columns [i - w, i + w] (w first clamped like ksw_extend2), zdrop > 0
+ * enables the z-drop test of :164-170 and, as in ksw_extend2, ends the
+ * pair's extension when it fires (the reference's loop nesting would leave
+ * only the current 8-row tile; its zdrop is 0, so it never fires there).
+ * Defaults -1, 0 reproduce the reference. */
+void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape

@@ -462,12 +462,26 @@ void banded_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint6
 }
 
 /* ------------------------------------------------------------------ KSW ---- */
-/* gasal_ksw_kernel<B>, src/kernels/ksw_kernel_template.h:46-199. */
+/* gasal_ksw_kernel<B>, src/kernels/ksw_kernel_template.h:46-199.
+ * Extension (not in the reference): w >= 0 bands each row to query columns
+ * [i - w, i + w] and zdrop > 0 enables the reference's own z-drop branch
+ * (:164-170, hard-coded off there).  The band follows bwa's ksw_extend2
+ * (ksw.c, which the reference transcribes, :27-29): w is first clamped to
+ * (qlen * max_score - gap_open) / gap_extend + 1 for insertions and
+ * deletions alike, then beg = max(beg, i - w), end = min(end, i + w + 1, qlen)
+ * at the start of row i.  Parity of the extension is unpinned (bwa is not
+ * available here). */
 void ksw_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint64_t tb,
-              uint32_t qlen, uint32_t tlen, uint32_t h0, PairOut &o) {
+              uint32_t qlen, uint32_t tlen, uint32_t h0, int32_t w, int32_t zdrop_ext, PairOut &o) {
     const uint32_t query_batch_regs = (qlen >> 3) + 1;                            /* :56-57 */
     const uint32_t target_batch_regs = (tlen >> 3) + 1;
-    const int zdrop = 0;                                                          /* :62 */
+    const int zdrop = zdrop_ext;                                                  /* :62 (0 there) */
+    if (w >= 0) {
+        const int maxsc = std::max(c.match, 0);
+        int mx = (int)((double)((int)qlen * maxsc - c.gapo) / c.gape + 1.0);
+        mx = mx > 1 ? mx : 1;
+        w = w < mx ? w : mx;
+    }
     const int o_del = c.gapo, o_ins = c.gapo, e_del = c.gape, e_ins = c.gape;
     struct eh_t { int32_t h, e; };
     std::vector<eh_t> eh((size_t)qlen + 2, eh_t{0, 0});                          /* :69-75 */
@@ -479,13 +493,22 @@ void ksw_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint64_t
     max_ie = -1, gscore = -1;
     max_off = 0;
     beg = 0, end = (int)qlen;
-    for (uint32_t target_tile_id = 0; target_tile_id < target_batch_regs; target_tile_id++) {
+    /* z-drop (an extension: the reference compiles it out with zdrop 0) ends
+     * the whole extension as in bwa's ksw_extend2; the m == 0 break below
+     * keeps the reference's loop structure and leaves the 8-row tile only. */
+    bool zstop = false;
+    for (uint32_t target_tile_id = 0; target_tile_id < target_batch_regs && !zstop; target_tile_id++) {
         uint32_t gpac = T.at((int64_t)tb + target_tile_id);
         for (uint32_t target_base_id = 0; target_base_id < 8; target_base_id++) {
             i = (int)(target_tile_id * 8 + target_base_id);
             if (i >= (int)tlen) break;                                            /* :98-99 */
             uint32_t gbase = (gpac >> (32 - (target_base_id + 1) * 4)) & 0x0Fu;
             int t, f = 0, h1, m = 0, mj = -1;
+            if (w >= 0) {                                                         /* band (extension) */
+                if (beg < i - w) beg = i - w;
+                if (end > i + w + 1) end = i + w + 1;
+                if (end > (int)qlen) end = (int)qlen;
+            }
             if (beg == 0) {                                                       /* :105-110 */
                 h1 = (int32_t)(h0 - (uint32_t)(o_del + e_del * (i + 1)));
                 if (h1 < 0) h1 = 0;
@@ -526,10 +549,11 @@ void ksw_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint64_t
                 max_off = max_off > abs(mj - i) ? max_off : abs(mj - i);
             } else if (zdrop > 0) {
                 if (i - max_i > mj - max_j) {
-                    if (max - m - ((i - max_i) - (mj - max_j)) * e_del > zdrop) break;
+                    if (max - m - ((i - max_i) - (mj - max_j)) * e_del > zdrop) zstop = true;
                 } else {
-                    if (max - m - ((mj - max_j) - (i - max_i)) * e_ins > zdrop) break;
+                    if (max - m - ((mj - max_j) - (i - max_i)) * e_ins > zdrop) zstop = true;
                 }
+                if (zstop) break;
             }
             for (j = beg; (j < end) && eh[j].h == 0 && eh[j].e == 0; ++j) ;      /* :178-183 */
             beg = j;
@@ -729,7 +753,7 @@ extern "C" int oracle_align_batch(const oracle_scores_t *scp, const oracle_param
                 banded_pair(c, Q, T, qb, tb, ql, tl, pr->k_band >> 3, o);         /* src/gasal_align.h:80 */
                 break;
             case A_KSW:
-                ksw_pair(c, Q, T, qb, tb, ql, tl, seed_scores ? seed_scores[k] : 0u, o);
+                ksw_pair(c, Q, T, qb, tb, ql, tl, seed_scores ? seed_scores[k] : 0u, pr->ksw_w, pr->ksw_zdrop, o);
                 break;
             default:
                 break;   /* UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21) */

@@ -41,6 +41,8 @@ typedef struct {
     int32_t k_band;                /* Parameters::k_band      src/args_parser.h:38 */
     int32_t is_packed;             /* Parameters::isPacked    src/args_parser.h:42 */
     int32_t is_reverse_complement; /* Parameters::isReverseComplement :43 */
+    int32_t ksw_w;                 /* KSW band half-width, < 0: none (the reference: none) */
+    int32_t ksw_zdrop;             /* KSW z-drop, 0: off (the reference: 0, ksw_kernel_template.h:62) */
 } oracle_params_t;
 
 typedef struct {

@@ -51,6 +51,29 @@ def test_ksw(second):
     assert_parity(got, ref, g.KSW, g.WITHOUT_START, second, b)
 
 
+@pytest.mark.parametrize("w,zdrop", [(50, 100), (8, 0), (-1, 20), (-1, 5), (0, 0), (-1, 0)])
+def test_ksw_extension(w, zdrop):
+    """KSW with bwa-style band and z-drop (the reference runs w < 0, z-drop 0)."""
+    import pyoracle as O
+    b0 = g.synth_batch(4, 510, 500)
+    qs = [b0.query(i) for i in range(b0.n)] + [b"ACGTTGCAAC" * 3 + b"AAAAA" + b"GATTACAGGC" * 10]
+    ts = [b0.target(i) for i in range(b0.n)] + [b"ACGTTGCAAC" * 3 + b"CCCCC" + b"GATTACAGGC" * 10]
+    sd = list(b0.seeds) + [20]
+    b = g.make_batch(qs, ts, seeds=sd)
+    g.set_ksw_extension(w, zdrop)
+    try:
+        p = g.Parameters(algo=g.KSW, start_pos=g.WITHOUT_START)
+        a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n)
+        try:
+            got = a.run(b)
+        finally:
+            a.close()
+    finally:
+        g.set_ksw_extension()
+    ref = O.align(b, g.KSW, ksw_w=w, ksw_zdrop=zdrop)
+    assert_parity(got, ref, g.KSW, g.WITHOUT_START, False, b)
+
+
 @pytest.mark.parametrize("start,second", [(g.WITH_TB, True), (g.WITH_START, False), (g.WITHOUT_START, True)])
 def test_local_reference_order_kernel(start, second):
     """The reference-order kernel (used when the fast path does not apply)

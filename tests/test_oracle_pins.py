@@ -267,6 +267,70 @@ def test_ksw_random_never_below_seed():
         assert r["score"][0] >= h0 - 5
 
 
+# KSW extension (band w, z-drop).  bwa's ksw_extend2, which these settings
+# follow, is not in the image: parity of the extension is unpinned beyond the
+# properties below (no-op settings, the band constraint on the end cell,
+# z-drop only removing rows, and two constructed cases each setting changes).
+def _ksw(q, t, h0, w=-1, zdrop=0):
+    r = O.align(g.make_batch([q], [t], seeds=[h0]), g.KSW, ksw_w=w, ksw_zdrop=zdrop)
+    return int(r["score"][0]), int(r["qend"][0]), int(r["tend"][0])
+
+
+def test_ksw_extension_noop_settings_equal_reference():
+    rng = random.Random(31)
+    for _ in range(60):
+        q = rnd_seq(rng, rng.randint(5, 120))
+        t = mutate(rng, q, 0.15) if rng.random() < 0.7 else rnd_seq(rng, rng.randint(5, 120))
+        h0 = rng.randint(10, 60)
+        ref = _ksw(q, t, h0)
+        assert _ksw(q, t, h0, w=10 ** 6) == ref
+        assert _ksw(q, t, h0, zdrop=10 ** 6) == ref
+
+
+def test_ksw_extension_band_and_zdrop_properties():
+    rng = random.Random(32)
+    for _ in range(80):
+        q = rnd_seq(rng, rng.randint(20, 150))
+        t = mutate(rng, q, 0.2)
+        h0 = rng.randint(10, 60)
+        full = _ksw(q, t, h0)
+        for w in (0, 3, 12):
+            s, qe, te = _ksw(q, t, h0, w=w)
+            # a best-cell end lies in the band (a query-end result, qe == Lq,
+            # takes the row's last computed H: when Lq is a multiple of 8 the
+            # reference's loop nesting reads that as column Lq on every row)
+            if qe != len(q):
+                assert abs(qe - te) <= w, (w, qe, te)
+        # z-drop only removes rows, so its best cell is at most the full run's;
+        # the full run may report its query-end score instead, which lies
+        # within 4 of its best (PEN_CLIP5)
+        for zd in (1, 10, 40):
+            assert _ksw(q, t, h0, zdrop=zd)[0] <= full[0] + 4
+
+
+def test_ksw_extension_band_cuts_off_a_long_indel():
+    rng = random.Random(33)
+    a, b, z = rnd_seq(rng, 100), rnd_seq(rng, 100), rnd_seq(rng, 20)
+    q, t = a + b, a + z + b
+    full = _ksw(q, t, 20)
+    assert full == (20 + 200 - 6 - 20, 200, 220)   # 200 matches, one 20-base deletion
+    assert _ksw(q, t, 20, w=25) == full
+    assert _ksw(q, t, 20, w=5)[0] < full[0]
+
+
+def test_ksw_extension_zdrop_stops_after_a_divergent_stretch():
+    # 30 matches, 5 mismatches, 100 matches.  z-drop 5 (below the gap-open
+    # penalty 6) fires two rows into the mismatches: the best cell stays at
+    # the end of the first block.  Without z-drop the extension runs through.
+    rng = random.Random(34)
+    a, b = rnd_seq(rng, 30), rnd_seq(rng, 100)
+    q, t = a + b"AAAAA" + b, a + b"CCCCC" + b
+    full = _ksw(q, t, 20)
+    assert full[0] >= 20 + 130 - 20
+    assert _ksw(q, t, 20, zdrop=5) == (50, 30, 30)
+    assert _ksw(q, t, 20, zdrop=100) == full
+
+
 # --------------------------------------------------------------- get_tb -----
 def apply_cigar_score(q, t, qs, ts, cg, sc=(1, 4, 6, 1)):
     m, mm, go, ge = sc
