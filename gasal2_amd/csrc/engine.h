@@ -54,6 +54,11 @@ struct G2Dev {
     // group g starts at dword tb_gbase[g], strips of tb_gqw[g] query words
     const uint64_t *tb_gbase;
     const uint32_t *tb_gqw;
+    // checkpointed traceback (LOCAL WITH_TB, fast lane path): when set, the DP
+    // stores no direction words; tb (same layout) holds each 8x8 tile's left
+    // column state (H, F) and tbh each strip's bottom row (H, E), from which
+    // the traceback recomputes the tiles on its path
+    uint32_t *tbh;
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
     int32_t ksw16;                   // KSW rows as packed uint16 (H, E): values fit 16 bits

@@ -63,16 +63,22 @@ struct TbView {
         return base + ((size_t)strip * qw + (size_t)word) * (WAVE * 8);
     }
 };
-__device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t lane) {
+__device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t lane, uint32_t *region = nullptr) {
     TbView v;
+    uint32_t *const r = region ? region : d.tb;
     if (d.tb_gbase) {
-        v.base = d.tb + d.tb_gbase[grp] + (size_t)lane * 8;
+        v.base = r + d.tb_gbase[grp] + (size_t)lane * 8;
         v.qw = d.tb_gqw[grp];
     } else {
         v.qw = d.tb_qw8 >> 3;
-        v.base = d.tb + ((size_t)grp * d.tb_tw * v.qw * WAVE + lane) * 8;
+        v.base = r + ((size_t)grp * d.tb_tw * v.qw * WAVE + lane) * 8;
     }
     return v;
+}
+__device__ __forceinline__ void tb_load(const uint32_t *p, uint32_t *w) {
+    const uint4 a = reinterpret_cast<const uint4 *>(p)[0], b = reinterpret_cast<const uint4 *>(p)[1];
+    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+    w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
 }
 __device__ __forceinline__ void tb_store(uint32_t *p, const uint32_t *w) {
     reinterpret_cast<uint4 *>(p)[0] = make_uint4(w[0], w[1], w[2], w[3]);
@@ -742,7 +748,13 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
     d.tend[x.pair] = bv > 0 ? bs * 8 + br : 0;
 }
 
-template <int START, bool SECOND>
+// CK (WITH_TB only): checkpointed traceback.  No direction words: per 8x8
+// tile the left column state (H, F of its eight rows, 16 bits each) goes to
+// the tile region and each strip's bottom row (H, E) to a second region of
+// the same layout, which also replaces the boundary row; k_get_tb recomputes
+// the tiles on the traceback path from them.  Exact when H fits 16 bits (the
+// host enables it only then; F is at least -(gap open + extend + penalty)).
+template <int START, bool SECOND, bool CK = false>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
     if (blockIdx.x < d.wf_blocks) {
@@ -765,15 +777,17 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     if (!d.enc_ok) return;   // position encoding does not fit: k_local takes the short pairs
     LaneCtx x;
     if (!lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) return;
-    constexpr bool TB = START == S_WITH_TB;
+    constexpr bool TB = START == S_WITH_TB && !CK;
     const int32_t nval = c.has_np ? -c.np : 0;
     const int32_t gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     const int32_t S = d.enc_s;
     uint32_t *__restrict__ bd = x.bd;
-    const TbView tbv = TB ? tb_view(d, x.grp, x.lane) : TbView{nullptr, 0};
+    const TbView tbv = START == S_WITH_TB ? tb_view(d, x.grp, x.lane) : TbView{nullptr, 0};
+    const TbView hv = CK ? tb_view(d, x.grp, x.lane, d.tbh) : TbView{nullptr, 0};
 
-    for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;
+    if (!CK)
+        for (int32_t cc = 0; cc < 8 * QW; cc++) bd[cc * WAVE] = 0u;
 
     int32_t best = 0;       // encoded first maximal cell
     int32_t sbest = 0;      // encoded first maximal second-best candidate
@@ -789,18 +803,35 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         // column tag: ~strip and ~column fields, pre-shifted by 3
         int32_t tagc = (int32_t)((((~(uint32_t)i) & d.enc_sm) << d.enc_cb | d.enc_cm) << 3);
         uint32_t nxt[8];
+        if (CK) {   // the previous strip's bottom row; zero above strip 0
+            if (i > 0) tb_load(hv.chunk(i - 1, 0), nxt);
+            else {
 #pragma unroll
-        for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
+                for (int kk = 0; kk < 8; kk++) nxt[kk] = 0u;
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
+        }
         for (int32_t j = 0; j < QW; j++) {
             const uint32_t rpac = x.Q[j];
             uint32_t cur[8];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
             if (j + 1 < QW) {
+                if (CK) {
+                    if (i > 0) tb_load(hv.chunk(i - 1, j + 1), nxt);
+                } else {
 #pragma unroll
-                for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+                    for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+                }
             }
             uint32_t dirw[8];
+            if (CK) {   // the tile's left column state
+#pragma unroll
+                for (int m = 1; m < 9; m++) dirw[m - 1] = pk16(h[m], f[m]);
+                tb_store(tbv.chunk(i, j), dirw);
+            }
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t ridx = j * 8 + kk;
@@ -837,13 +868,15 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
                     p = h[m];
                     h[m] = hm;
                 }
-                bd[ridx * WAVE] = pk16(h[8], e);
+                if (CK) dirw[kk] = pk16(h[8], e);
+                else bd[ridx * WAVE] = pk16(h[8], e);
                 best = max(best, colbest | tagc);
                 if (SECOND) sbest = max(sbest, colsec | tagc);
                 tagc -= 8;
                 if (TB) dirw[kk] = dir;
             }
             if (TB) tb_store(tbv.chunk(i, j), dirw);
+            if (CK) tb_store(hv.chunk(i, j), dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -1498,6 +1531,51 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
 }
 
 // --------------------------------------------------------------- get_tb -----
+// Recomputes one 8x8 tile's direction words (k_local_fast's sign-bit format)
+// from the checkpoints k_local_fast<.., CK> stored: the previous strip's
+// bottom row above the tile, its last entry left of it (the corner), and the
+// tile's left column state.  Same recurrences, so the same words.
+__device__ __noinline__ void ck_tile(const TbView &lv, const TbView &hv, int32_t strip, int32_t w, uint32_t qword,
+                                     uint32_t tword, const G2Scores &c, int32_t nval, uint32_t *ch) {
+    uint32_t top[8], left[8];
+    if (strip > 0) tb_load(hv.chunk(strip - 1, w), top);
+    else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) top[k] = 0u;
+    }
+    tb_load(lv.chunk(strip, w), left);
+    int32_t h[9], f[9];
+    h[0] = (strip > 0 && w > 0) ? sx16(hv.chunk(strip - 1, w - 1)[7]) : 0;
+#pragma unroll
+    for (int m = 1; m < 9; m++) { h[m] = sx16(left[m - 1]); f[m] = (int32_t)left[m - 1] >> 16; }
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++) {
+        const uint32_t qb = (qword >> (28 - 4 * kk)) & 15u;
+        int32_t e = (int32_t)top[kk] >> 16;
+        int32_t p = h[0];
+        h[0] = sx16(top[kk]);
+        uint32_t dir = 0;
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            const int32_t sc = sub_local(c, nval, qb, (tword >> (32 - 4 * m)) & 15u);
+            const int32_t tmp = p + sc;
+            const int32_t hm = max(max(max(tmp, f[m]), e), 0);
+            const int32_t t = tmp - gapoe;
+            const int32_t fx = f[m] - gape, ex = e - gape;
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+            f[m] = max(t, fx);
+            e = max(t, ex);
+            p = h[m];
+            h[m] = hm;
+        }
+        ch[kk] = dir;
+    }
+}
+
 // gasal_get_tb<T> (src/kernels/get_tb.h:4-149).  Reads the direction words the
 // DP kernel wrote for this pair's wave slot; cells outside the pair's computed
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
@@ -1523,7 +1601,10 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     uint32_t bflag = 0;
     if (band && (d.band_flags[pr] & 8u)) { d.n_ops[pr] = 0; return; }
     // GLOBAL words are always in the sign-bit format (k_global)
-    const bool fastfmt = !LOCALT || band || wfp || (d.fast && d.enc_ok && census_fast(d.aux[0]));
+    const bool lanefast = d.fast && d.enc_ok && census_fast(d.aux[0]);
+    const bool fastfmt = !LOCALT || band || wfp || lanefast;
+    // lane-path tiles of k_local_fast<.., CK> are recomputed from checkpoints
+    const bool ck = LOCALT && !band && !wfp && lanefast && d.tbh != nullptr;
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1538,71 +1619,82 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int read_len_8 = 8 * QW;
     int n_ops = 0, op_select = 3, op_shift = 0, count = 0, offset = 0;
     uint32_t op_to_fill;
-    while (i >= 0 && j >= 0) {
-        // cell (i, j) in flat (strip, padded column, row) order; a column index
-        // of 8QW (GLOBAL start at Lq == 8QW) is column 0 of the next strip
-        // (src/kernels/get_tb.h:50)
-        const int32_t strip = (i >> 3) + (j >= read_len_8 ? 1 : 0);
-        const int32_t col = j >= read_len_8 ? j - read_len_8 : j;
-        uint32_t cell_op = 0;
-        if (strip < TW) {
-            const int32_t key = strip * QW + (col >> 3);
-            if (key != ch_key) {
-                const uint32_t *chunk;
-                if (band) {
-                    int32_t wlo, whi;
-                    band_words(strip, blo, bhi, QW, wlo, whi);
-                    const int32_t wv = col >> 3;
-                    if (wv < wlo || wv >= whi || wv - wlo >= bbw) { bflag |= 2u; break; }   // walk left the band
-                    if ((wv == wlo && wlo > 0) || (wv == whi - 1 && whi < QW)) bflag |= 1u;
-                    chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
-                } else if (wfp) {
-                    chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)(col >> 3)) * 8;
-                } else {
-                    chunk = tb_view(d, grp, lane).chunk(strip, col >> 3);
+    // Tile loop (outer) and cell loop (inner): the lanes of a wave fetch or
+    // recompute their next tile together, then each walks its cells until the
+    // path leaves that tile, so tile work is not repeated once per lane.
+    // Cell (i, j) in flat (strip, padded column, row) order; a column index of
+    // 8QW (GLOBAL start at Lq == 8QW) is column 0 of the next strip
+    // (src/kernels/get_tb.h:50).  Cells outside the pair's strips read as 0.
+    auto tile_key = [&](int32_t ii, int32_t jj) -> int32_t {
+        const int32_t strip = (ii >> 3) + (jj >= read_len_8 ? 1 : 0);
+        const int32_t col = jj >= read_len_8 ? jj - read_len_8 : jj;
+        return strip < TW ? strip * QW + (col >> 3) : -2;
+    };
+    bool done = !(i >= 0 && j >= 0);
+    while (!done) {
+        const int32_t key = tile_key(i, j);
+        if (key >= 0 && key != ch_key) {
+            const int32_t strip = key / QW, wv = key - strip * QW;
+            const uint32_t *chunk = nullptr;
+            if (band) {
+                int32_t wlo, whi;
+                band_words(strip, blo, bhi, QW, wlo, whi);
+                if (wv < wlo || wv >= whi || wv - wlo >= bbw) { bflag |= 2u; break; }   // walk left the band
+                if ((wv == wlo && wlo > 0) || (wv == whi - 1 && whi < QW)) bflag |= 1u;
+                chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
+            } else if (wfp) {
+                chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)wv) * 8;
+            } else if (!ck) {
+                chunk = tb_view(d, grp, lane).chunk(strip, wv);
+            }
+            if (ck) ck_tile(tb_view(d, grp, lane), tb_view(d, grp, lane, d.tbh), strip, wv, Qw[wv], Tw[strip], c, nval, ch);
+            else tb_load(chunk, ch);
+            ch_key = key;
+        }
+        while (true) {
+            const int32_t strip = (i >> 3) + (j >= read_len_8 ? 1 : 0);
+            const int32_t col = j >= read_len_8 ? j - read_len_8 : j;
+            uint32_t cell_op = 0;
+            if (strip < TW) {
+                const int c7 = col & 7;
+                const uint32_t word = c7 < 4 ? (c7 < 2 ? (c7 == 0 ? ch[0] : ch[1]) : (c7 == 2 ? ch[2] : ch[3]))
+                                             : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
+                cell_op = (word >> (28 - 4 * (i & 7))) & 15u;
+                if (fastfmt) {
+                    // the cell the tile word belongs to: column col, row 8 strip + (i & 7)
+                    const uint32_t nA = (cell_op >> 3) & 1u, nB = (cell_op >> 2) & 1u;
+                    const uint32_t nC3 = (cell_op >> 1) & 1u, nC2 = cell_op & 1u;
+                    if ((col >> 3) != q_key) { q_key = col >> 3; qword = Qw[q_key]; }
+                    if (strip != t_key) { t_key = strip; tword = Tw[t_key]; }
+                    const uint32_t qb = (qword >> (28 - 4 * (col & 7))) & 15u;
+                    const uint32_t tb = (tword >> (28 - 4 * (i & 7))) & 15u;
+                    const int32_t sub = LOCALT ? sub_local(c, nval, qb, tb) : sub_global(c, qb, tb);
+                    const uint32_t mx = sub < 0 ? 1u : 0u;
+                    const uint32_t src = nA == 0 ? mx : (nB == 0 ? 3u : 2u);
+                    cell_op = src | ((nC3 ^ 1u) << 3) | ((nC2 ^ 1u) << 2);
                 }
-                const uint4 *src = reinterpret_cast<const uint4 *>(chunk);
-                const uint4 a = src[0], b = src[1];
-                ch[0] = a.x; ch[1] = a.y; ch[2] = a.z; ch[3] = a.w;
-                ch[4] = b.x; ch[5] = b.y; ch[6] = b.z; ch[7] = b.w;
-                ch_key = key;
             }
-            const int c7 = col & 7;
-            const uint32_t word = c7 < 4 ? (c7 < 2 ? (c7 == 0 ? ch[0] : ch[1]) : (c7 == 2 ? ch[2] : ch[3]))
-                                         : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
-            cell_op = (word >> (28 - 4 * (i & 7))) & 15u;
-            if (fastfmt) {
-                // the cell the tile word belongs to: column col, row 8 strip + (i & 7)
-                const uint32_t nA = (cell_op >> 3) & 1u, nB = (cell_op >> 2) & 1u;
-                const uint32_t nC3 = (cell_op >> 1) & 1u, nC2 = cell_op & 1u;
-                if ((col >> 3) != q_key) { q_key = col >> 3; qword = Qw[q_key]; }
-                if (strip != t_key) { t_key = strip; tword = Tw[t_key]; }
-                const uint32_t qb = (qword >> (28 - 4 * (col & 7))) & 15u;
-                const uint32_t tb = (tword >> (28 - 4 * (i & 7))) & 15u;
-                const int32_t sub = LOCALT ? sub_local(c, nval, qb, tb) : sub_global(c, qb, tb);
-                const uint32_t mx = sub < 0 ? 1u : 0u;
-                const uint32_t src = nA == 0 ? mx : (nB == 0 ? 3u : 2u);
-                cell_op = src | ((nC3 ^ 1u) << 3) | ((nC2 ^ 1u) << 2);
+            const uint32_t op = (cell_op >> op_shift) & (uint32_t)op_select;
+            op_to_fill = op == 0 || op_select == 3 ? op : (uint32_t)op_shift;
+            op_select = op == 0 || (op == 1 && op_select == 3) ? 3 : 1;
+            op_shift = op == 0 || (op == 1 && op_select == 3) ? 0 : ((op == 2 || op == 3) ? (int)op : op_shift);
+            if (count < 63 && op_to_fill == prev_op_to_fill) {
+                count++;
+            } else {
+                if (count > 0) { cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2)); n_ops++; }
+                count = 1;
             }
+            if (LOCALT) {
+                curr_score += ((op_to_fill == 2 || op_to_fill == 3) && prev_op_to_fill != op_to_fill) ? -c.gapoe
+                            : ((op_to_fill == 2 || op_to_fill == 3) ? -c.gape : (op_to_fill == 1 ? -c.mismatch : c.match));
+                if (curr_score == total_score) { done = true; break; }
+            }
+            prev_op_to_fill = op_to_fill;
+            i = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 2 ? i - 1 : i;
+            j = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 3 ? j - 1 : j;
+            if (!(i >= 0 && j >= 0)) { done = true; break; }
+            if (tile_key(i, j) != key) break;
         }
-        const uint32_t op = (cell_op >> op_shift) & (uint32_t)op_select;
-        op_to_fill = op == 0 || op_select == 3 ? op : (uint32_t)op_shift;
-        op_select = op == 0 || (op == 1 && op_select == 3) ? 3 : 1;
-        op_shift = op == 0 || (op == 1 && op_select == 3) ? 0 : ((op == 2 || op == 3) ? (int)op : op_shift);
-        if (count < 63 && op_to_fill == prev_op_to_fill) {
-            count++;
-        } else {
-            if (count > 0) { cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2)); n_ops++; }
-            count = 1;
-        }
-        if (LOCALT) {
-            curr_score += ((op_to_fill == 2 || op_to_fill == 3) && prev_op_to_fill != op_to_fill) ? -c.gapoe
-                        : ((op_to_fill == 2 || op_to_fill == 3) ? -c.gape : (op_to_fill == 1 ? -c.mismatch : c.match));
-            if (curr_score == total_score) break;
-        }
-        prev_op_to_fill = op_to_fill;
-        i = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 2 ? i - 1 : i;
-        j = op_to_fill == 0 || op_to_fill == 1 || op_to_fill == 3 ? j - 1 : j;
     }
     cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2));
     n_ops++;
@@ -1765,7 +1857,10 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
         case A_LOCAL:
             if (d.fast) {
                 if (op.start == S_WITH_TB) {
-                    if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
+                    if (d.tbh) {
+                        if (second) k_local_fast<S_WITH_TB, true, true><<<grid, blk, 0, s>>>(d, sc);
+                        else k_local_fast<S_WITH_TB, false, true><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                    } else if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
                     else k_local_fast<S_WITH_TB, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
                 } else if (op.start == S_WITH_START) {
                     if (second) k_local_fast<S_WITH_START, true><<<grid, blk, 0, s>>>(d, sc);

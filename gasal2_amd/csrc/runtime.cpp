@@ -38,6 +38,7 @@ bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for ever
 constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;
+int g_tb_ckpt = 1;   // checkpointed traceback for the fast LOCAL WITH_TB lane path
 int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)   // lane-path tiles: one size for all groups up to this many bytes
 
@@ -191,6 +192,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 }
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
+extern "C" void gasal2_set_tb_checkpoint(int32_t on) { g_tb_ckpt = on ? 1 : 0; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
@@ -1138,8 +1140,16 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     if (tb_dp && !px.banded) {
         const uint64_t lgroups = ((uint64_t)(n - nlong) + 63) / 64;
         const uint64_t one_size = lgroups * et * (8 * (uint64_t)eq) * 64 * 4;
+        // Checkpointed traceback: two regions of the tile layout (left column
+        // states, strip bottom rows) instead of direction words.  Stored
+        // states are 16-bit, so H must fit: H <= match * min(Lq, Lt).
+        const bool ck = g_tb_ckpt && op.algo == LOCAL && op.start == WITH_TB && d.fast &&
+                        (uint64_t)std::max(sc.match, 0) * 8u * std::min(eq, et) <= 32767u &&
+                        (uint64_t)sc.gapoe + (uint64_t)std::max(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0) <= 32767u;
+        const uint64_t nreg = ck ? 2 : 1;
         if (one_size <= g_tb_one_size_max) {
-            ensure(m->tb, m->tb_bytes, (size_t)one_size + 256);
+            ensure(m->tb, m->tb_bytes, (size_t)(nreg * one_size) + 256);
+            if (ck) d.tbh = m->tb + one_size / 4;
         } else {
             // Per-group regions.  The device orders the lane-path pairs by
             // descending shape bucket (k_keys), so the buckets falling in each
@@ -1184,7 +1194,8 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 m->h_gbase[g] = tot;
                 tot += (uint64_t)gtw[g] * m->h_gqw[g] * 64 * 8;
             }
-            ensure(m->tb, m->tb_bytes, (size_t)tot * 4 + 256);
+            ensure(m->tb, m->tb_bytes, (size_t)(nreg * tot) * 4 + 256);
+            if (ck) d.tbh = m->tb + tot;
             ensure(m->tb_gbase, m->tb_gbase_bytes, (size_t)lgroups * 8 + 256);
             ensure(m->tb_gqw, m->tb_gqw_bytes, (size_t)lgroups * 4 + 256);
             G2_CHECK(hipMemcpyAsync(m->tb_gbase, m->h_gbase.data(), (size_t)lgroups * 8, hipMemcpyHostToDevice, s->str));

@@ -158,6 +158,16 @@ void gasal2_set_pk16(int32_t on);
  * Defaults -1, 0 reproduce the reference. */
 void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
 
+/* Checkpointed traceback for LOCAL WITH_TB (default on).  The DP stores, per
+ * 8x8 tile, the left column state (H, F) and, per strip, the bottom row
+ * (H, E) -- two regions of the tile layout, twice the bytes of the direction
+ * words the reference stores (src/kernels/local_kernel_template.h:45-60) --
+ * and gasal_get_tb's kernel recomputes the tiles on each path from them.
+ * Used on the one-lane-per-pair path when H fits 16 bits (match * min(Lq, Lt)
+ * <= 32767); results are identical either way.  on = 0 stores direction
+ * words. */
+void gasal2_set_tb_checkpoint(int32_t on);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape

@@ -51,6 +51,19 @@ def test_ksw(second):
     assert_parity(got, ref, g.KSW, g.WITHOUT_START, second, b)
 
 
+@pytest.mark.parametrize("second", [False, True])
+@pytest.mark.parametrize("ckpt", [0, 1])
+def test_local_tb_checkpoint_switch(ckpt, second):
+    """LOCAL WITH_TB with the checkpointed traceback off (direction words) and on."""
+    g.lib().gasal2_set_tb_checkpoint(ckpt)
+    try:
+        b = g.synth_batch(2, 930 + ckpt, 600)
+        got, ref = run_both(b, g.LOCAL, g.WITH_TB, second, is_rc=True, n_penalty=1)
+        assert_parity(got, ref, g.LOCAL, g.WITH_TB, second, b)
+    finally:
+        g.lib().gasal2_set_tb_checkpoint(1)
+
+
 @pytest.mark.parametrize("w,zdrop", [(50, 100), (8, 0), (-1, 20), (-1, 5), (0, 0), (-1, 0)])
 def test_ksw_extension(w, zdrop):
     """KSW with bwa-style band and z-drop (the reference runs w < 0, z-drop 0)."""
