@@ -10,7 +10,8 @@ batch with the inputs already resident in HBM.
 
 Prints one JSON line.  `value` is GCUPS = sum over pairs of Lq*Lt (unpadded
 cells) / step time.  `roofline` prices the DP kernel against the integer-VALU
-peak (N_CU * 64 lanes * clock) at 16 algorithmic int ops per cell; `traffic`
+peak (N_CU * 64 lanes * clock) at the DP kernel's algorithmic int ops per
+cell (15 for the default mode: recurrence 11 + second best 4); `traffic`
 comes from rocprofv3 counter runs when given.  `cpu_baseline` times the CPU
 oracle (an OpenMP restatement of the reference kernels) on a bounded prefix of
 the same batch on this host and reports the GPU's agreement with it there.
@@ -26,8 +27,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-OPS_TB = 16      # algorithmic int ops per cell, local WITH_TB
-OPS_SCORE = 11   # score-only
+# Algorithmic int ops per cell of the DP kernel: the recurrence with end
+# tracking (substitution, diagonal add, 4-way max, gap open, two gap extends,
+# two gap maxima, rank encode, running max) is 11; second best adds 4 (compare,
+# select, max, running max); direction bits stored by the DP add 8 (four
+# comparisons, four inserts).  With the checkpointed traceback the DP stores
+# no direction bits (the traceback kernel recomputes the tiles it needs).
+OPS_SCORE = 11
+OPS_SECOND = 4
+OPS_DIR = 8
 
 
 def log(*a):
@@ -88,6 +96,8 @@ def main():
                     help="one-wave-per-pair switch thresholds (gasal2_set_wavefront)")
     ap.add_argument("--compare-lane-only", action="store_true",
                     help="also time dispatch policy 2 (one lane per pair only) on the same batch")
+    ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1],
+                    help="LOCAL WITH_TB: checkpointed traceback (1, default) or direction words stored by the DP (0)")
     ap.add_argument("--ksw-ext", type=int, nargs=2, default=None, metavar=("W", "ZDROP"),
                     help="KSW band half-width and z-drop (seed extension as bwa; default: the reference's none, 0)")
     ap.add_argument("--cpu-pairs", type=int, default=-1,
@@ -133,10 +143,10 @@ def main():
 
     if mode == "local_tb_second":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True)
-        ops = OPS_TB
+        ops = OPS_SCORE + OPS_SECOND + (0 if args.tb_checkpoint else OPS_DIR)
     elif mode == "local_tb":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, isReverseComplement=True)
-        ops = OPS_TB
+        ops = OPS_SCORE + (0 if args.tb_checkpoint else OPS_DIR)
     elif mode == "local_score":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START, isReverseComplement=True)
         ops = OPS_SCORE
@@ -163,8 +173,11 @@ def main():
 
     L = g.lib()
     g.set_ksw_extension(kw, kz)
+    L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
     if args.wf:
         L.gasal2_set_wavefront(*args.wf)
+
+    dev_ms = []   # device time of each timed step (first to last event on the stream)
 
     def timed_steps(policy):
         L.gasal2_set_dispatch(policy)
@@ -172,6 +185,7 @@ def main():
             a.run_resident(timing=True)
             a.sync()
         dp, tb = [], []
+        dev_ms.clear()
         a.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -179,6 +193,7 @@ def main():
             a.sync()
             dp.append(a.stage_ms(3))
             tb.append(a.stage_ms(4))
+            dev_ms.append(a.stage_ms(0))
         a.sync()
         return time.perf_counter() - t0, dp, tb
 
@@ -193,7 +208,8 @@ def main():
     gcups = cells / (wall / args.steps) / 1e9
     dp_avg = float(np.mean(dp_ms))
     achieved_tops = cells * ops / (dp_avg * 1e-3) / 1e12
-    log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); DP kernel {dp_avg:.2f} ms; traceback {np.mean(tb_ms):.2f} ms")
+    log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); device {np.mean(dev_ms):.2f} ms; DP kernel {dp_avg:.2f} ms; "
+        f"traceback {np.mean(tb_ms):.2f} ms")
 
     # PCIe-inclusive rate of one batch through gasal_aln_async: host batch
     # already filled, timed from the call to results on the host
@@ -272,7 +288,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "mode": mode, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
+                   "mode": mode, "tb_checkpoint": args.tb_checkpoint, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
         "roofline": {"bound": "valu",
                      "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",

@@ -56,22 +56,51 @@ struct LaneCtx {
 // eight columns.
 // A lane's tiles: its first chunk and the strip stride in query words, taken
 // once per kernel (one region size for all groups, or per-group regions).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// A tile's eight words are stored as two halves of four, each half of all 64
+// lanes contiguous ([half][lane][4]), so every 16-byte store instruction of a
+// wave writes whole cache lines (1 KiB).  Stores and the DP's loads are
+// non-temporal: the tiles stream through L2 (written once, read once much
+// later) and would otherwise evict the sequence words the lanes re-read.
 struct TbView {
-    uint32_t *base;
+    uint32_t *base;   // the group's chunk (0, 0) plus 4 * lane
     uint32_t qw;
     __device__ __forceinline__ uint32_t *chunk(int32_t strip, int32_t word) const {
         return base + ((size_t)strip * qw + (size_t)word) * (WAVE * 8);
+    }
+    __device__ __forceinline__ void store(int32_t strip, int32_t word, const uint32_t *w) const {
+        uint32_t *p = chunk(strip, word);
+        const u32x4 a = {w[0], w[1], w[2], w[3]}, b = {w[4], w[5], w[6], w[7]};
+        __builtin_nontemporal_store(a, reinterpret_cast<u32x4 *>(p));
+        __builtin_nontemporal_store(b, reinterpret_cast<u32x4 *>(p + WAVE * 4));
+    }
+    __device__ __forceinline__ void load_nt(int32_t strip, int32_t word, uint32_t *w) const {
+        const uint32_t *p = chunk(strip, word);
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+        const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p + WAVE * 4));
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+        w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+    }
+    __device__ __forceinline__ void load(int32_t strip, int32_t word, uint32_t *w) const {
+        const uint32_t *p = chunk(strip, word);
+        const uint4 a = *reinterpret_cast<const uint4 *>(p), b = *reinterpret_cast<const uint4 *>(p + WAVE * 4);
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+        w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+    }
+    __device__ __forceinline__ uint32_t at(int32_t strip, int32_t word, int k) const {
+        return chunk(strip, word)[k < 4 ? k : WAVE * 4 + k - 4];
     }
 };
 __device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t lane, uint32_t *region = nullptr) {
     TbView v;
     uint32_t *const r = region ? region : d.tb;
     if (d.tb_gbase) {
-        v.base = r + d.tb_gbase[grp] + (size_t)lane * 8;
+        v.base = r + d.tb_gbase[grp] + (size_t)lane * 4;
         v.qw = d.tb_gqw[grp];
     } else {
         v.qw = d.tb_qw8 >> 3;
-        v.base = r + ((size_t)grp * d.tb_tw * v.qw * WAVE + lane) * 8;
+        v.base = r + (size_t)grp * d.tb_tw * v.qw * WAVE * 8 + (size_t)lane * 4;
     }
     return v;
 }
@@ -366,7 +395,7 @@ __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
                 prev_maxHH = max(maxHH, prev_maxHH);
                 if (TB) dirw[kk] = dir;
             }
-            if (TB) tb_store(tbv.chunk(i, j), dirw);
+            if (TB) tbv.store(i, j, dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -754,10 +783,16 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
 // the same layout, which also replaces the boundary row; k_get_tb recomputes
 // the tiles on the traceback path from them.  Exact when H fits 16 bits (the
 // host enables it only then; F is at least -(gap open + extend + penalty)).
-template <int START, bool SECOND, bool CK = false>
+// WF: the launch carries one-wave-per-pair blocks (d.wf_blocks > 0); without
+// them the kernel holds only the lane path and keeps its smaller register
+// footprint.
+#ifndef G2_CK_LDS
+#define G2_CK_LDS 0
+#endif
+template <int START, bool SECOND, bool CK = false, bool WF = true>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
-    if (blockIdx.x < d.wf_blocks) {
+    if (WF && blockIdx.x < d.wf_blocks) {
         if (START == S_WITHOUT_START && !SECOND) {
             if (census_fast(census)) wf_local<false, true>(d, c, census);
             else wf_local<false, false>(d, c, census);
@@ -804,7 +839,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         int32_t tagc = (int32_t)((((~(uint32_t)i) & d.enc_sm) << d.enc_cb | d.enc_cm) << 3);
         uint32_t nxt[8];
         if (CK) {   // the previous strip's bottom row; zero above strip 0
-            if (i > 0) tb_load(hv.chunk(i - 1, 0), nxt);
+            if (i > 0) hv.load_nt(i - 1, 0, nxt);
             else {
 #pragma unroll
                 for (int kk = 0; kk < 8; kk++) nxt[kk] = 0u;
@@ -820,7 +855,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
             for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
             if (j + 1 < QW) {
                 if (CK) {
-                    if (i > 0) tb_load(hv.chunk(i - 1, j + 1), nxt);
+                    if (i > 0) hv.load_nt(i - 1, j + 1, nxt);
                 } else {
 #pragma unroll
                     for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
@@ -830,7 +865,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
             if (CK) {   // the tile's left column state
 #pragma unroll
                 for (int m = 1; m < 9; m++) dirw[m - 1] = pk16(h[m], f[m]);
-                tb_store(tbv.chunk(i, j), dirw);
+                tbv.store(i, j, dirw);
             }
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
@@ -875,8 +910,8 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
                 tagc -= 8;
                 if (TB) dirw[kk] = dir;
             }
-            if (TB) tb_store(tbv.chunk(i, j), dirw);
-            if (CK) tb_store(hv.chunk(i, j), dirw);
+            if (TB) tbv.store(i, j, dirw);
+            if (CK) hv.store(i, j, dirw);
         }
     }
     const uint32_t pr = x.pair;
@@ -1148,7 +1183,7 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                     for (int m = 1; m < 9; m++) score = (m == rowsel) ? h[m] : score;
                 }
             }
-            if (TB) tb_store(tbv.chunk(i, j), dirw);
+            if (TB) tbv.store(i, j, dirw);
         }
     }
     d.score[x.pair] = score;
@@ -1534,31 +1569,32 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
 // Recomputes one 8x8 tile's direction words (k_local_fast's sign-bit format)
 // from the checkpoints k_local_fast<.., CK> stored: the previous strip's
 // bottom row above the tile, its last entry left of it (the corner), and the
-// tile's left column state.  Same recurrences, so the same words.
-__device__ __noinline__ void ck_tile(const TbView &lv, const TbView &hv, int32_t strip, int32_t w, uint32_t qword,
-                                     uint32_t tword, const G2Scores &c, int32_t nval, uint32_t *ch) {
+// tile's left column state.  Same recurrences and score tables (lut: the
+// strip's eight rows, as k_local_fast builds them), so the same words.
+__device__ __forceinline__ void ck_tile(const TbView &lv, const TbView &hv, int32_t strip, int32_t w, uint32_t qword,
+                                        const uint32_t *lut, const G2Scores &c, uint32_t *ch) {
     uint32_t top[8], left[8];
-    if (strip > 0) tb_load(hv.chunk(strip - 1, w), top);
+    if (strip > 0) hv.load(strip - 1, w, top);
     else {
 #pragma unroll
         for (int k = 0; k < 8; k++) top[k] = 0u;
     }
-    tb_load(lv.chunk(strip, w), left);
+    lv.load(strip, w, left);
     int32_t h[9], f[9];
-    h[0] = (strip > 0 && w > 0) ? sx16(hv.chunk(strip - 1, w - 1)[7]) : 0;
+    h[0] = (strip > 0 && w > 0) ? sx16(hv.at(strip - 1, w - 1, 7)) : 0;
 #pragma unroll
     for (int m = 1; m < 9; m++) { h[m] = sx16(left[m - 1]); f[m] = (int32_t)left[m - 1] >> 16; }
     const int32_t gapoe = c.gapoe, gape = c.gape;
 #pragma unroll
     for (int kk = 0; kk < 8; kk++) {
-        const uint32_t qb = (qword >> (28 - 4 * kk)) & 15u;
+        const uint32_t qsh = kk < 7 ? (qword >> (26 - 4 * kk)) & 0x1Cu : (qword << 2) & 0x1Cu;
         int32_t e = (int32_t)top[kk] >> 16;
         int32_t p = h[0];
         h[0] = sx16(top[kk]);
         uint32_t dir = 0;
 #pragma unroll
         for (int m = 1; m < 9; m++) {
-            const int32_t sc = sub_local(c, nval, qb, (tword >> (32 - 4 * m)) & 15u);
+            const int32_t sc = __builtin_amdgcn_sbfe(lut[m - 1], qsh, 4);
             const int32_t tmp = p + sc;
             const int32_t hm = max(max(max(tmp, f[m]), e), 0);
             const int32_t t = tmp - gapoe;
@@ -1601,7 +1637,8 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     uint32_t bflag = 0;
     if (band && (d.band_flags[pr] & 8u)) { d.n_ops[pr] = 0; return; }
     // GLOBAL words are always in the sign-bit format (k_global)
-    const bool lanefast = d.fast && d.enc_ok && census_fast(d.aux[0]);
+    const uint32_t census = d.aux[0];
+    const bool lanefast = d.fast && d.enc_ok && census_fast(census);
     const bool fastfmt = !LOCALT || band || wfp || lanefast;
     // lane-path tiles of k_local_fast<.., CK> are recomputed from checkpoints
     const bool ck = LOCALT && !band && !wfp && lanefast && d.tbh != nullptr;
@@ -1609,6 +1646,8 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
     int32_t ch_key = -1;
+    uint32_t lut[8];                // score tables of the strip being recomputed (ck)
+    int32_t lut_key = -1;
     uint32_t qword = 0, tword = 0;  // cached sequence words (fast format)
     int32_t q_key = -1, t_key = -1;
     uint8_t *__restrict__ cg = d.cigar + (size_t)pr * d.cigar_cap;
@@ -1644,11 +1683,20 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
                 chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
             } else if (wfp) {
                 chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)wv) * 8;
-            } else if (!ck) {
-                chunk = tb_view(d, grp, lane).chunk(strip, wv);
             }
-            if (ck) ck_tile(tb_view(d, grp, lane), tb_view(d, grp, lane, d.tbh), strip, wv, Qw[wv], Tw[strip], c, nval, ch);
-            else tb_load(chunk, ch);
+            if (ck) {
+                if (strip != lut_key) {   // the walk stays in one strip for several tiles
+                    lut_key = strip;
+                    const uint32_t gw = Tw[strip];
+#pragma unroll
+                    for (int m = 0; m < 8; m++) lut[m] = score_lut(c, nval, census, (gw >> (28 - 4 * m)) & 15u);
+                }
+                ck_tile(tb_view(d, grp, lane), tb_view(d, grp, lane, d.tbh), strip, wv, Qw[wv], lut, c, ch);
+            } else if (chunk) {
+                tb_load(chunk, ch);
+            } else {
+                tb_view(d, grp, lane).load(strip, wv, ch);
+            }
             ch_key = key;
         }
         while (true) {
@@ -1858,16 +1906,19 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             if (d.fast) {
                 if (op.start == S_WITH_TB) {
                     if (d.tbh) {
-                        if (second) k_local_fast<S_WITH_TB, true, true><<<grid, blk, 0, s>>>(d, sc);
-                        else k_local_fast<S_WITH_TB, false, true><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
-                    } else if (second) k_local_fast<S_WITH_TB, true><<<grid, blk, 0, s>>>(d, sc);
-                    else k_local_fast<S_WITH_TB, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                        if (second) k_local_fast<S_WITH_TB, true, true, false><<<grid, blk, G2_CK_LDS, s>>>(d, sc);
+                        else if (d.wf_blocks) k_local_fast<S_WITH_TB, false, true><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                        else k_local_fast<S_WITH_TB, false, true, false><<<grid, blk, G2_CK_LDS, s>>>(d, sc);
+                    } else if (second) k_local_fast<S_WITH_TB, true, false, false><<<grid, blk, 0, s>>>(d, sc);
+                    else if (d.wf_blocks) k_local_fast<S_WITH_TB, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITH_TB, false, false, false><<<grid, blk, 0, s>>>(d, sc);
                 } else if (op.start == S_WITH_START) {
-                    if (second) k_local_fast<S_WITH_START, true><<<grid, blk, 0, s>>>(d, sc);
-                    else k_local_fast<S_WITH_START, false><<<grid, blk, 0, s>>>(d, sc);
+                    if (second) k_local_fast<S_WITH_START, true, false, false><<<grid, blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITH_START, false, false, false><<<grid, blk, 0, s>>>(d, sc);
                 } else {
-                    if (second) k_local_fast<S_WITHOUT_START, true><<<grid, blk, 0, s>>>(d, sc);
-                    else k_local_fast<S_WITHOUT_START, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                    if (second) k_local_fast<S_WITHOUT_START, true, false, false><<<grid, blk, 0, s>>>(d, sc);
+                    else if (d.wf_blocks) k_local_fast<S_WITHOUT_START, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
+                    else k_local_fast<S_WITHOUT_START, false, false, false><<<grid, blk, 0, s>>>(d, sc);
                 }
             }
             if (op.start == S_WITH_TB) {

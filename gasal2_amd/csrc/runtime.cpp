@@ -1088,10 +1088,11 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
         if (!g_wf_user && op.start == WITH_TB) { min_tw = WF_TB_MIN_TW; min_cells = WF_TB_MIN_CELLS; }
         {
-            uint64_t sum_q = 0;
-            for (uint32_t k = 0; k < n; k++) sum_q += hq[k];
-            const uint64_t mean_q = n ? sum_q / n : 0;
             std::lock_guard<std::mutex> lk(g_tune_mu);
+            uint64_t sum_q = 0;
+            if (!g_tune.empty())
+                for (uint32_t k = 0; k < n; k++) sum_q += hq[k];
+            const uint64_t mean_q = n ? sum_q / n : 0;
             for (const TuneEntry &e : g_tune)
                 if (e.algo == op.algo && e.start == op.start && e.second == op.second && mean_q >= e.lq_lo && mean_q < e.lq_hi) {
                     min_tw = e.min_tw; min_cells = e.min_cells; max_blocks = e.max_blocks;
@@ -1104,7 +1105,9 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         const bool wtb = op.start == WITH_TB;
         if (wtb) m->h_wf_tbo.resize(n);
         uint64_t wtot = 0;   // 32-byte tiles of the long pairs
-        for (uint32_t k = 0; k < n; k++) {
+        // no pair can pass the shape test when the largest shapes do not
+        const bool any_long = tw >= d.wf_min_tw && (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
+        for (uint32_t k = 0; any_long && k < n; k++) {
             const uint32_t kq = g2_regs(hq[k]), kt = g2_regs(ht[k]);
             if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
                 nlong++;
