@@ -786,9 +786,6 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
 // WF: the launch carries one-wave-per-pair blocks (d.wf_blocks > 0); without
 // them the kernel holds only the lane path and keeps its smaller register
 // footprint.
-#ifndef G2_CK_LDS
-#define G2_CK_LDS 0
-#endif
 template <int START, bool SECOND, bool CK = false, bool WF = true>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
@@ -1906,9 +1903,9 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             if (d.fast) {
                 if (op.start == S_WITH_TB) {
                     if (d.tbh) {
-                        if (second) k_local_fast<S_WITH_TB, true, true, false><<<grid, blk, G2_CK_LDS, s>>>(d, sc);
+                        if (second) k_local_fast<S_WITH_TB, true, true, false><<<grid, blk, 0, s>>>(d, sc);
                         else if (d.wf_blocks) k_local_fast<S_WITH_TB, false, true><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
-                        else k_local_fast<S_WITH_TB, false, true, false><<<grid, blk, G2_CK_LDS, s>>>(d, sc);
+                        else k_local_fast<S_WITH_TB, false, true, false><<<grid, blk, 0, s>>>(d, sc);
                     } else if (second) k_local_fast<S_WITH_TB, true, false, false><<<grid, blk, 0, s>>>(d, sc);
                     else if (d.wf_blocks) k_local_fast<S_WITH_TB, false><<<dim3(groups + d.wf_blocks), blk, 0, s>>>(d, sc);
                     else k_local_fast<S_WITH_TB, false, false, false><<<grid, blk, 0, s>>>(d, sc);
