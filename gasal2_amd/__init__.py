@@ -173,6 +173,7 @@ _SIGS = {
     "gasal2_set_pk16": (None, [C.c_int32]),
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
     "gasal2_set_tb_checkpoint": (None, [C.c_int32]),
+    "gasal2_set_panels": (None, [C.c_int32]),
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_tune_clear": (None, []),
     "gasal2_tune_count": (C.c_int32, []),

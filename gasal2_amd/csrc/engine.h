@@ -59,6 +59,9 @@ struct G2Dev {
     // column state (H, F) and tbh each strip's bottom row (H, E), from which
     // the traceback recomputes the tiles on its path
     uint32_t *tbh;
+    // transposed sequence panels of the lane path ([group][word][lane]; null: off)
+    uint32_t *qpan, *tpan;
+    uint32_t qpan_w, tpan_w;
     int32_t *eh;                     // KSW rows: per group eh_cols*64*2 ints
     uint32_t eh_cols;
     int32_t ksw16;                   // KSW rows as packed uint16 (H, E): values fit 16 bits

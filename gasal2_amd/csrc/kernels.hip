@@ -47,15 +47,18 @@ struct LaneCtx {
     uint32_t qlen, tlen;
     int32_t QW, TW;
     const uint32_t *Q, *T;
+    // sequence words as the fast LOCAL kernels read them: word j at Qp[j * ps]
+    // (the group's transposed panels, ps = 64, or the pair's words, ps = 1)
+    const uint32_t *Qp, *Tp;
+    uint32_t ps;
     uint32_t *bd;     // boundary row: element c at bd[c * WAVE]
 };
 
-// Direction words of a wave slot are laid out [strip][query word][lane][8
-// columns]: a lane writes one 32-byte chunk per 8x8 tile (two 16-byte stores,
-// contiguous across the wave) and a traceback walk stays inside one chunk for
-// eight columns.
-// A lane's tiles: its first chunk and the strip stride in query words, taken
-// once per kernel (one region size for all groups, or per-group regions).
+// Direction words of a wave slot are laid out per 8x8 tile, [strip][query
+// word], 32 bytes per lane (8 columns), and a traceback walk stays inside one
+// tile for eight columns.  A lane's tiles: its first chunk and the strip
+// stride in query words, taken once per kernel (one region size for all
+// groups, or per-group regions).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // A tile's eight words are stored as two halves of four, each half of all 64
@@ -128,6 +131,15 @@ __device__ __forceinline__ bool lane_setup(const G2Dev &d, LaneCtx &x, uint32_t 
     x.TW = dregs(x.tlen);
     x.Q = d.pq + (d.qoff[x.pair] >> 3);
     x.T = d.pt + (d.toff[x.pair] >> 3);
+    if (d.qpan) {
+        x.Qp = d.qpan + (size_t)x.grp * d.qpan_w * WAVE + x.lane;
+        x.Tp = d.tpan + (size_t)x.grp * d.tpan_w * WAVE + x.lane;
+        x.ps = WAVE;
+    } else {
+        x.Qp = x.Q;
+        x.Tp = x.T;
+        x.ps = 1;
+    }
     x.bd = d.bnd + (size_t)x.grp * d.bnd_cols * WAVE + x.lane;
     return true;
 }
@@ -309,6 +321,25 @@ __global__ __launch_bounds__(256) void k_scatter(G2Dev d) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d.n) return;
     d.perm[atomicAdd(&d.hist[d.keys[k]], 1u)] = k;
+}
+
+// Transposed sequence panels of the lane path: word j of the pair in lane l of
+// group g at qpan[(g * qpan_w + j) * 64 + l] (0 past the pair's words), the
+// same for targets.  A wave's read of one word is then 256 contiguous bytes
+// instead of 64 scattered lines.  One thread per sorted slot after the long
+// pairs.
+__global__ __launch_bounds__(256) void k_panel(G2Dev d) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t slot0 = d.wf_blocks ? d.aux[2] : 0u;
+    if (k >= d.n || k < slot0) return;
+    const uint32_t r = k - slot0, g = r / WAVE, l = r & (WAVE - 1);
+    const uint32_t pr = d.perm[k];
+    const uint32_t QW = (uint32_t)dregs(d.qlen[pr]), TW = (uint32_t)dregs(d.tlen[pr]);
+    const uint32_t *Q = d.pq + (d.qoff[pr] >> 3), *T = d.pt + (d.toff[pr] >> 3);
+    uint32_t *qo = d.qpan + (size_t)g * d.qpan_w * WAVE + l;
+    uint32_t *to = d.tpan + (size_t)g * d.tpan_w * WAVE + l;
+    for (uint32_t j = 0; j < d.qpan_w; j++) qo[(size_t)j * WAVE] = j < QW ? Q[j] : 0u;
+    for (uint32_t j = 0; j < d.tpan_w; j++) to[(size_t)j * WAVE] = j < TW ? T[j] : 0u;
 }
 
 // ---------------------------------------------------------------- LOCAL -----
@@ -708,7 +739,7 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
     for (int32_t s0 = 0; s0 < TW; s0 += 2) {
         const bool hi_ok = s0 + 1 < TW;
         uint32_t lutL[9], lutH[9];
-        const uint32_t gL = x.T[s0], gH = hi_ok ? x.T[s0 + 1] : 0u;
+        const uint32_t gL = x.Tp[(size_t)s0 * x.ps], gH = hi_ok ? x.Tp[(size_t)(s0 + 1) * x.ps] : 0u;
 #pragma unroll
         for (int m = 1; m < 9; m++) {
             lutL[m] = score_lut(c, nval, census, (gL >> (32 - 4 * m)) & 15u);
@@ -726,7 +757,7 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
             uint32_t topL = 0;
             if (t < C) {
                 const int kk = t & 7;
-                if (kk == 0) rpac = x.Q[t >> 3];
+                if (kk == 0) rpac = x.Qp[(size_t)(t >> 3) * x.ps];
                 qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
                 topL = bd[t * WAVE];
             }
@@ -828,7 +859,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         int32_t h[9], f[9];
 #pragma unroll
         for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; }
-        const uint32_t gpac = x.T[i];
+        const uint32_t gpac = x.Tp[(size_t)i * x.ps];
         uint32_t lut[9];
 #pragma unroll
         for (int m = 1; m < 9; m++) lut[m] = score_lut(c, nval, census, (gpac >> (32 - 4 * m)) & 15u);
@@ -846,7 +877,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
             for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
         }
         for (int32_t j = 0; j < QW; j++) {
-            const uint32_t rpac = x.Q[j];
+            const uint32_t rpac = x.Qp[(size_t)j * x.ps];
             uint32_t cur[8];
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
@@ -1887,6 +1918,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     k_keys<<<nb256, 256, 0, s>>>(d);
     k_scan_desc<<<1, 1024, 0, s>>>(d.hist);
     k_scatter<<<nb256, 256, 0, s>>>(d);
+    if (d.qpan) k_panel<<<nb256, 256, 0, s>>>(d);
     G2_LAUNCH_CHECK();
     // 4. DP (dispatch table src/gasal_align.h:7-107)
     const uint32_t groups = (d.n + WAVE - 1) / WAVE;

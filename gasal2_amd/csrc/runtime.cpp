@@ -39,6 +39,7 @@ constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;
 int g_tb_ckpt = 1;   // checkpointed traceback for the fast LOCAL WITH_TB lane path
+int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
 int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)   // lane-path tiles: one size for all groups up to this many bytes
 
@@ -101,6 +102,8 @@ struct Impl {
     size_t tb_gbase_bytes = 0, tb_gqw_bytes = 0;
     std::vector<uint32_t> h_hist, h_gqw;
     std::vector<uint64_t> h_gbase;
+    uint32_t *qpan = nullptr, *tpan = nullptr;   // transposed sequence panels of the lane path
+    size_t qpan_bytes = 0, tpan_bytes = 0;
     uint32_t *wf_tb = nullptr;       // direction tiles of one-wave-per-pair pairs
     size_t wf_tb_bytes = 0;
     uint64_t *wf_tbo = nullptr;
@@ -193,6 +196,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
 extern "C" void gasal2_set_tb_checkpoint(int32_t on) { g_tb_ckpt = on ? 1 : 0; }
+extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
@@ -757,7 +761,7 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
 
 static void impl_free(Impl *m) {
     if (!m) return;
-    void *ptrs[] = {m->tb_gbase, m->tb_gqw, m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
+    void *ptrs[] = {m->qpan, m->tpan, m->tb_gbase, m->tb_gqw, m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
                     m->sub_nops, m->sub_cigar,
                     m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
                     m->nops, m->cigar, m->cigar_q, m->scratch_i32, m->aux, m->wf};
@@ -1138,6 +1142,17 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 d.wf_tbo = m->wf_tbo;
             }
         }
+    }
+    // Transposed sequence panels for the fast LOCAL lane path (k_panel): groups
+    // after the long pairs, eq query words and et target words each.
+    if (g_panels && d.fast && op.algo == LOCAL && !px.banded && n > nlong) {
+        const uint64_t lgroups = ((uint64_t)(n - nlong) + 63) / 64;
+        ensure(m->qpan, m->qpan_bytes, (size_t)(lgroups * eq * 256) + 256);
+        ensure(m->tpan, m->tpan_bytes, (size_t)(lgroups * et * 256) + 256);
+        d.qpan = m->qpan;
+        d.tpan = m->tpan;
+        d.qpan_w = eq;
+        d.tpan_w = et;
     }
     // direction tiles of the one-lane-per-pair kernels: groups after the long pairs
     if (tb_dp && !px.banded) {

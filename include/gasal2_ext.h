@@ -168,6 +168,13 @@ void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
  * words. */
 void gasal2_set_tb_checkpoint(int32_t on);
 
+/* Transposed sequence panels (default on; no counterpart in the reference).
+ * After the shape sort, the fast LOCAL one-lane-per-pair kernels read their
+ * packed query and target words from per-group panels laid out
+ * [group][word][lane], written by one gather kernel, instead of each lane
+ * reading its own pair's words.  Results are identical either way. */
+void gasal2_set_panels(int32_t on);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape

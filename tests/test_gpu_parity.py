@@ -64,6 +64,19 @@ def test_local_tb_checkpoint_switch(ckpt, second):
         g.lib().gasal2_set_tb_checkpoint(1)
 
 
+@pytest.mark.parametrize("start,second", [(g.WITH_TB, True), (g.WITH_TB, False), (g.WITHOUT_START, False),
+                                          (g.WITH_START, True)])
+def test_local_panels_off(start, second):
+    """The fast LOCAL kernels reading each pair's own words (panels off)."""
+    g.lib().gasal2_set_panels(0)
+    try:
+        b = g.synth_batch(2, 940 + start, 500)
+        got, ref = run_both(b, g.LOCAL, start, second, is_rc=True, n_penalty=1)
+        assert_parity(got, ref, g.LOCAL, start, second, b)
+    finally:
+        g.lib().gasal2_set_panels(1)
+
+
 @pytest.mark.parametrize("w,zdrop", [(50, 100), (8, 0), (-1, 20), (-1, 5), (0, 0), (-1, 0)])
 def test_ksw_extension(w, zdrop):
     """KSW with bwa-style band and z-drop (the reference runs w < 0, z-drop 0)."""
