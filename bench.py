@@ -96,8 +96,9 @@ def main():
                     help="one-wave-per-pair switch thresholds (gasal2_set_wavefront)")
     ap.add_argument("--compare-lane-only", action="store_true",
                     help="also time dispatch policy 2 (one lane per pair only) on the same batch")
-    ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1],
-                    help="LOCAL WITH_TB: checkpointed traceback (1, default) or direction words stored by the DP (0)")
+    ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1, 2],
+                    help="LOCAL WITH_TB: checkpointed traceback on the lane path (1, default), also for one-wave "
+                         "pairs (2), or direction words stored by the DP (0)")
     ap.add_argument("--ksw-ext", type=int, nargs=2, default=None, metavar=("W", "ZDROP"),
                     help="KSW band half-width and z-drop (seed extension as bwa; default: the reference's none, 0)")
     ap.add_argument("--cpu-pairs", type=int, default=-1,

@@ -80,8 +80,10 @@ struct G2Dev {
     uint32_t wf_blocks, wf_min_tw, wf_min_cells;
     uint32_t *wf_scratch;            // per wavefront block: per pass 17 x 64 dwords of strip state
     uint32_t wf_stride;              // dwords per wavefront block
-    uint32_t *wf_tb;                 // WITH_TB: direction tiles of the long pairs, per pair [strip][word] x 8 dwords
+    uint32_t *wf_tb;                 // WITH_TB: tiles of the long pairs, per pair [strip][word] x 8 dwords
     const uint64_t *wf_tbo;          // first 32-byte tile of each long pair in wf_tb
+    uint32_t *wf_tbh;                // checkpointed traceback of the long pairs: strip bottom rows
+                                     // (wf_tb then holds the tiles' left column states); null: direction words
     // banded LOCAL (k_local_band): per-pair diagonal band [band_lo, band_hi]
     // of d = query column - target row; direction tiles of pair k start at
     // 32-byte chunk band_tbo[k], band_bw[k] chunks per strip.

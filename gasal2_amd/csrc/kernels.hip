@@ -573,7 +573,10 @@ __device__ __forceinline__ bool wf_before(int32_t av, int32_t as, int32_t ac, in
 // tile, to the pair's own region wf_tb + wf_tbo[pair] laid out [strip][word].
 // LUT: score tables (the batch's nibble classes allow them), else the
 // substitution rule per cell with the query base travelling instead of its class.
-template <bool TB, bool LUT>
+// CK (TB only): checkpointed traceback, as k_local_fast<.., CK>: per tile the
+// left column state (at wf_tb) and per strip the bottom row (at wf_tbh), per
+// pair [strip][word] x 8 dwords, instead of direction words.
+template <bool TB, bool LUT, bool CK = false>
 __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     __shared__ uint32_t ring[WF_W];
     const int32_t l = (int32_t)threadIdx.x;
@@ -600,7 +603,9 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
         const int32_t C = 8 * QW;
         const int32_t npass = (TW + 63) / 64;
         const int32_t nblk = (C + WF_W - 1) / WF_W;
+        // tiles per pair [strip][word] x 8 dwords
         uint32_t *tbp = TB ? d.wf_tb + d.wf_tbo[pr] * 8 : nullptr;
+        uint32_t *thp = CK ? d.wf_tbh + d.wf_tbo[pr] * 8 : nullptr;
         WfBest lb = {0, 0x7FFFFFFF, 0x7FFFFFFF};
         for (int32_t b = 0; b < nblk; b++) {
             const int32_t c0 = b * WF_W;
@@ -645,6 +650,13 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                     qcls = (uint32_t)__builtin_amdgcn_update_dpp((int)q0, (int)qcls, 0x138, 0xF, 0xF, false);
                     const int32_t rel = tau - l;
                     if (act && rel >= 0 && rel < wb) {
+                        const int32_t col = c0 + rel;
+                        if (CK && (col & 7) == 0) {   // the tile's left column state
+                            uint32_t lw[8];
+#pragma unroll
+                            for (int m = 1; m < 9; m++) lw[m - 1] = pk16(h[m], f[m]);
+                            tb_store(tbp + ((size_t)s * QW + (size_t)(col >> 3)) * 8, lw);
+                        }
                         int32_t e = (int32_t)tin >> 16;
                         int32_t p = ptop;
                         ptop = sx16(tin);
@@ -657,7 +669,7 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                             const int32_t hm = max(max(max(tmp, f[m]), e), 0);
                             const int32_t t = tmp - gapoe;
                             const int32_t fx = f[m] - gape, ex = e - gape;
-                            if (TB) {
+                            if (TB && !CK) {
                                 dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
                                 dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
                                 dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
@@ -672,14 +684,13 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                         }
                         // strictly higher h only: an equal h in a later column never wins
                         if (colbest > (rv | 7)) { rv = colbest; rt = tau; }
-                        if (TB) {
+                        out = pk16(h[8], e);
+                        if (TB) {   // direction words, or with CK the strip's bottom row
 #pragma unroll
                             for (int m = 0; m < 7; m++) win[m] = win[m + 1];
-                            win[7] = dir;
-                            const int32_t col = c0 + rel;
-                            if ((col & 7) == 7) tb_store(tbp + ((size_t)s * QW + (size_t)(col >> 3)) * 8, win);
+                            win[7] = CK ? out : dir;
+                            if ((col & 7) == 7) tb_store((CK ? thp : tbp) + ((size_t)s * QW + (size_t)(col >> 3)) * 8, win);
                         }
-                        out = pk16(h[8], e);
                         if (l == 63 && k + 1 < npass) ring[rel] = out;
                     }
                 }
@@ -826,8 +837,14 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
             else wf_local<false, false>(d, c, census);
         }
         if (START == S_WITH_TB && !SECOND) {
-            if (census_fast(census)) wf_local<true, true>(d, c, census);
-            else wf_local<true, false>(d, c, census);
+            // checkpoints need the score tables for the recompute; without them
+            // the pairs store direction words in the same region
+            if (census_fast(census)) {
+                if (d.wf_tbh) wf_local<true, true, true>(d, c, census);
+                else wf_local<true, true>(d, c, census);
+            } else {
+                wf_local<true, false>(d, c, census);
+            }
         }
         return;
     }
@@ -1599,17 +1616,13 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
 // bottom row above the tile, its last entry left of it (the corner), and the
 // tile's left column state.  Same recurrences and score tables (lut: the
 // strip's eight rows, as k_local_fast builds them), so the same words.
-__device__ __forceinline__ void ck_tile(const TbView &lv, const TbView &hv, int32_t strip, int32_t w, uint32_t qword,
+// top: the row above (zero above strip 0), left: the left column state,
+// corner: H above-left of the tile; lut: the strip's score tables (the kernels
+// store checkpoints only when the batch's nibble census allows tables).
+__device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *left, int32_t corner, uint32_t qword,
                                         const uint32_t *lut, const G2Scores &c, uint32_t *ch) {
-    uint32_t top[8], left[8];
-    if (strip > 0) hv.load(strip - 1, w, top);
-    else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) top[k] = 0u;
-    }
-    lv.load(strip, w, left);
     int32_t h[9], f[9];
-    h[0] = (strip > 0 && w > 0) ? sx16(hv.at(strip - 1, w - 1, 7)) : 0;
+    h[0] = corner;
 #pragma unroll
     for (int m = 1; m < 9; m++) { h[m] = sx16(left[m - 1]); f[m] = (int32_t)left[m - 1] >> 16; }
     const int32_t gapoe = c.gapoe, gape = c.gape;
@@ -1644,7 +1657,7 @@ __device__ __forceinline__ void ck_tile(const TbView &lv, const TbView &hv, int3
 // DP kernel wrote for this pair's wave slot; cells outside the pair's computed
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
 template <bool LOCALT>
-__global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_get_tb(G2Dev d, G2Scores c) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= d.n) return;
     const uint32_t pr = d.perm[slot];
@@ -1670,6 +1683,8 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
     const bool fastfmt = !LOCALT || band || wfp || lanefast;
     // lane-path tiles of k_local_fast<.., CK> are recomputed from checkpoints
     const bool ck = LOCALT && !band && !wfp && lanefast && d.tbh != nullptr;
+    // long pairs of k_local_fast's one-wave blocks with checkpoints (wf_local<.., CK>)
+    const bool wck = LOCALT && wfp && d.wf_tbh != nullptr && census_fast(census);
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -1709,17 +1724,34 @@ __global__ __launch_bounds__(64) void k_get_tb(G2Dev d, G2Scores c) {
                 if (wv < wlo || wv >= whi || wv - wlo >= bbw) { bflag |= 2u; break; }   // walk left the band
                 if ((wv == wlo && wlo > 0) || (wv == whi - 1 && whi < QW)) bflag |= 1u;
                 chunk = d.tb + (d.band_tbo[pr] + (size_t)strip * bbw + (size_t)(wv - wlo)) * 8;
-            } else if (wfp) {
+            } else if (wfp && !wck) {
                 chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)wv) * 8;
             }
-            if (ck) {
+            if (ck || wck) {
+                const uint32_t gw = Tw[strip];
                 if (strip != lut_key) {   // the walk stays in one strip for several tiles
                     lut_key = strip;
-                    const uint32_t gw = Tw[strip];
 #pragma unroll
                     for (int m = 0; m < 8; m++) lut[m] = score_lut(c, nval, census, (gw >> (28 - 4 * m)) & 15u);
                 }
-                ck_tile(tb_view(d, grp, lane), tb_view(d, grp, lane, d.tbh), strip, wv, Qw[wv], lut, c, ch);
+                uint32_t top[8], left[8];
+                int32_t corner = 0;
+                if (ck) {
+                    const TbView lv = tb_view(d, grp, lane), hv = tb_view(d, grp, lane, d.tbh);
+                    if (strip > 0) hv.load(strip - 1, wv, top);
+                    lv.load(strip, wv, left);
+                    if (strip > 0 && wv > 0) corner = sx16(hv.at(strip - 1, wv - 1, 7));
+                } else {   // per pair [strip][word] x 8
+                    const uint32_t *lp = d.wf_tb + d.wf_tbo[pr] * 8, *hp = d.wf_tbh + d.wf_tbo[pr] * 8;
+                    if (strip > 0) tb_load(hp + ((size_t)(strip - 1) * QW + (size_t)wv) * 8, top);
+                    tb_load(lp + ((size_t)strip * QW + (size_t)wv) * 8, left);
+                    if (strip > 0 && wv > 0) corner = sx16(hp[((size_t)(strip - 1) * QW + (size_t)(wv - 1)) * 8 + 7]);
+                }
+                if (strip == 0) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) top[k] = 0u;
+                }
+                ck_tile(top, left, corner, Qw[wv], lut, c, ch);
             } else if (chunk) {
                 tb_load(chunk, ch);
             } else {

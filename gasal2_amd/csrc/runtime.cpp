@@ -38,7 +38,7 @@ bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for ever
 constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;
-int g_tb_ckpt = 1;   // checkpointed traceback for the fast LOCAL WITH_TB lane path
+int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
 int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)   // lane-path tiles: one size for all groups up to this many bytes
@@ -195,7 +195,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 }
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
-extern "C" void gasal2_set_tb_checkpoint(int32_t on) { g_tb_ckpt = on ? 1 : 0; }
+extern "C" void gasal2_set_tb_checkpoint(int32_t mode) { g_tb_ckpt = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
@@ -1105,7 +1105,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         }
         d.wf_min_tw = all ? 1u : min_tw;
         d.wf_min_cells = all ? 0u : min_cells;
-        uint32_t ltw = 0, sq = 0, st = 0;
+        uint32_t ltw = 0, lqw = 0, sq = 0, st = 0;
         const bool wtb = op.start == WITH_TB;
         if (wtb) m->h_wf_tbo.resize(n);
         uint64_t wtot = 0;   // 32-byte tiles of the long pairs
@@ -1116,6 +1116,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
             if (kt >= d.wf_min_tw && (uint64_t)kq * kt * 64u >= (uint64_t)d.wf_min_cells) {
                 nlong++;
                 ltw = std::max(ltw, kt);
+                lqw = std::max(lqw, kq);
                 if (wtb) { m->h_wf_tbo[k] = wtot; wtot += (uint64_t)kq * kt; }
             } else {
                 sq = std::max(sq, kq);
@@ -1134,12 +1135,19 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 d.wf_stride = 0;
                 d.wf_scratch = nullptr;
             }
-            if (wtb) {   // per-pair direction tiles of the long pairs and their offsets
-                ensure(m->wf_tb, m->wf_tb_bytes, (size_t)wtot * 32 + 256);
+            if (wtb) {   // per-pair direction tiles (or checkpoints) of the long pairs and their offsets
+                // checkpoints take twice the bytes of direction words: only within the tile budget
+                const bool wck = g_tb_ckpt == 2 && wtot * 64 <= g_tb_one_size_max &&
+                                 (uint64_t)std::max(g_scores.match, 0) * 8u * std::min(lqw, ltw) <= 32767u &&
+                                 (uint64_t)g_scores.gapoe +
+                                         (uint64_t)std::max(std::max(g_scores.mismatch, g_scores.has_np ? g_scores.np : 0), 0) <=
+                                     32767u;
+                ensure(m->wf_tb, m->wf_tb_bytes, (size_t)wtot * 32 * (wck ? 2 : 1) + 256);
                 ensure(m->wf_tbo, m->wf_tbo_bytes, (size_t)n * 8);
                 G2_CHECK(hipMemcpyAsync(m->wf_tbo, m->h_wf_tbo.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->str));
                 d.wf_tb = m->wf_tb;
                 d.wf_tbo = m->wf_tbo;
+                if (wck) d.wf_tbh = m->wf_tb + wtot * 8;
             }
         }
     }

@@ -158,15 +158,17 @@ void gasal2_set_pk16(int32_t on);
  * Defaults -1, 0 reproduce the reference. */
 void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
 
-/* Checkpointed traceback for LOCAL WITH_TB (default on).  The DP stores, per
- * 8x8 tile, the left column state (H, F) and, per strip, the bottom row
- * (H, E) -- two regions of the tile layout, twice the bytes of the direction
- * words the reference stores (src/kernels/local_kernel_template.h:45-60) --
- * and gasal_get_tb's kernel recomputes the tiles on each path from them.
- * Used on the one-lane-per-pair path when H fits 16 bits (match * min(Lq, Lt)
- * <= 32767); results are identical either way.  on = 0 stores direction
- * words. */
-void gasal2_set_tb_checkpoint(int32_t on);
+/* Checkpointed traceback for LOCAL WITH_TB.  The DP stores, per 8x8 tile,
+ * the left column state (H, F) and, per strip, the bottom row (H, E) -- two
+ * regions of the tile layout, twice the bytes of the direction words the
+ * reference stores (src/kernels/local_kernel_template.h:45-60) -- and
+ * gasal_get_tb's kernel recomputes the tiles on each path from them.  Used
+ * when H fits 16 bits (match * min(Lq, Lt) <= 32767); results are identical
+ * either way.  mode 0: direction words everywhere; 1 (default): checkpoints
+ * on the one-lane-per-pair path; 2: also for the pairs that run one wave per
+ * pair, when their checkpoints fit the tile budget (their long paths make the
+ * recompute cost about what the DP saves). */
+void gasal2_set_tb_checkpoint(int32_t mode);
 
 /* Transposed sequence panels (default on; no counterpart in the reference).
  * After the shape sort, the fast LOCAL one-lane-per-pair kernels read their

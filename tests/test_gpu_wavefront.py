@@ -152,25 +152,33 @@ def test_autotune_choices_identical_results():
         autotune.clear()
 
 
-def check_tb(b, **kw):
-    got, ref = run_both(b, g.LOCAL, g.WITH_TB, False, **kw)
+def check_tb(b, ckpt=2, **kw):
+    """ckpt 2: checkpoints and tile recompute on both paths; 1: lane path only
+    (default); 0: direction words."""
+    g.lib().gasal2_set_tb_checkpoint(ckpt)
+    try:
+        got, ref = run_both(b, g.LOCAL, g.WITH_TB, False, **kw)
+    finally:
+        g.lib().gasal2_set_tb_checkpoint(1)
     assert_parity(got, ref, g.LOCAL, g.WITH_TB, False, b)
 
 
+@pytest.mark.parametrize("ckpt", [2, 1, 0])
 @pytest.mark.parametrize("cfg,n_penalty,is_rc", [(1, None, False), (2, 1, True)])
-def test_tb_every_pair_one_wave(cfg, n_penalty, is_rc):
-    """WITH_TB: direction tiles per pair, traceback through them."""
+def test_tb_every_pair_one_wave(cfg, n_penalty, is_rc, ckpt):
+    """WITH_TB: per-pair tiles (checkpoints or direction words), traceback through them."""
     b = g.synth_batch(cfg, 1700 + cfg, 500)
     with policy(dispatch=3):
-        check_tb(b, is_rc=is_rc, n_penalty=n_penalty)
+        check_tb(b, ckpt=ckpt, is_rc=is_rc, n_penalty=n_penalty)
 
 
+@pytest.mark.parametrize("ckpt", [2, 1, 0])
 @pytest.mark.parametrize("scores", [(1, 4, 6, 1), (2, 3, 5, 2)])
-def test_tb_pass_and_block_boundaries(scores):
+def test_tb_pass_and_block_boundaries(scores, ckpt):
     qs, ts = related_pairs(SHAPES, 1800 + scores[0])
     b = g.make_batch(qs, ts)
     with policy(dispatch=3):
-        check_tb(b, scores=scores, n_penalty=1)
+        check_tb(b, ckpt=ckpt, scores=scores, n_penalty=1)
 
 
 def test_tb_mixed_switch_and_lane_groups():
@@ -190,3 +198,5 @@ def test_census_collision_runs_one_wave_without_tables():
     with policy(dispatch=0, wf=(16, 64 * 16 * 16, 64)):
         check(b)
         check_tb(b)
+        check_tb(b, ckpt=1)
+        check_tb(b, ckpt=0)
