@@ -72,6 +72,7 @@ struct G2Dev {
     // fits 31 bits (otherwise k_local takes the pairs the wavefront leaves).
     int32_t fast, enc_ok;
     int32_t pk16;                    // LOCAL score and ends in packed 16-bit halves (lane_local_pk)
+    int32_t pk16g;                   // GLOBAL score in packed 16-bit halves (lane_global_pk)
     int32_t enc_s, enc_cb;           // position encoding: h << enc_s | ~strip << (3+enc_cb) | ~col << 3 | 7-row
     uint32_t enc_cm, enc_sm;         // column / strip field masks
     // intra-pair wavefront for long pairs (LOCAL score + ends): pairs whose

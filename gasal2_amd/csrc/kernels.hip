@@ -1158,12 +1158,109 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
 // gasal_global_kernel<S> (src/kernels/global.h:30-303).
 // LUT: substitution from a per-strip 4-bit score table (see k_local_fast),
 // launched beside the generic variant; the nibble census picks one.
+// ------------------------------------------ GLOBAL, packed 16-bit, one lane ----
+// k_global's score (WITHOUT_START / WITH_START) with two 8-row strips per lane
+// in the 16-bit halves of each register, skewed by one column as in
+// lane_local_pk: the low half runs strip 2k at column t, the high half strip
+// 2k+1 at column t-1 with the low half's bottom row of the previous step as
+// its top row.  Same boundaries as k_global (top row -(open + extend * c),
+// column 0 left values per row, src/kernels/global.h:57-71), no clamp at 0.
+// Used when every value fits int16 with room for the packed minus infinity
+// (NEG16 stands for MINUS_INF: it never wins a maximum against a real value
+// in that range, so the scores are those of the int32 kernel).
+constexpr int32_t NEG16 = -30000;
+__device__ void lane_global_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census) {
+    const int32_t gapo = c.gapo, gape = c.gape;
+    const pk16x2 gapoe2 = {(short)c.gapoe, (short)c.gapoe}, gape2 = {(short)c.gape, (short)c.gape};
+    const int32_t QW = x.QW, TW = x.TW, C = 8 * QW;
+    uint32_t *__restrict__ bd = x.bd;
+    bd[0] = pk16(0, NEG16);                                                                // :57-60
+    for (int32_t cc = 1; cc < C; cc++) bd[cc * WAVE] = pk16(-(gapo + gape * cc), NEG16);
+    // Diagonal entering row R + 1 (0-based R) at column 0: the reference's
+    // p[m] for 1-based row R + 1, 0 for the first row, else -(open + extend * R)
+    // (src/kernels/global.h:67-71)
+    auto hl = [&](int32_t R) -> int32_t { return R == 0 ? 0 : -(gapo + gape * R); };
+    const int32_t rowsel = 8 - ((TW << 3) - (int32_t)x.tlen);                              // :299
+    const int32_t lastcol = (int32_t)x.qlen - 1;
+    int32_t score = 0;
+    for (int32_t s0 = 0; s0 < TW; s0 += 2) {
+        const bool hi_ok = s0 + 1 < TW;
+        uint32_t lutL[9], lutH[9];
+        const uint32_t gL = x.Tp[(size_t)s0 * x.ps], gH = hi_ok ? x.Tp[(size_t)(s0 + 1) * x.ps] : 0u;
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            lutL[m] = score_lut<true>(c, 0, census, (gL >> (32 - 4 * m)) & 15u);
+            lutH[m] = score_lut<true>(c, 0, census, (gH >> (32 - 4 * m)) & 15u);
+        }
+        // running diagonal: h[m] holds H(row m, previous column); column -1 is the left boundary
+        pk16x2 h[9], f[9];
+        const pk16x2 neg2 = {(short)NEG16, (short)NEG16};
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            h[m] = as_pk(pack_lo((uint32_t)hl(8 * s0 + m), (uint32_t)hl(8 * (s0 + 1) + m)));
+            f[m] = neg2;
+        }
+        uint32_t carry = 0;            // low strip's bottom (H | E << 16) of the previous step
+        uint32_t qshL = 0, qshH = 0;
+        uint32_t rpac = 0;
+        const bool last_lo = s0 == TW - 1, last_hi = hi_ok && s0 + 1 == TW - 1;
+        for (int32_t t = 0; t <= C; t++) {
+            qshH = qshL;
+            uint32_t topL = 0;
+            if (t < C) {
+                const int kk = t & 7;
+                if (kk == 0) rpac = x.Qp[(size_t)(t >> 3) * x.ps];
+                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                topL = bd[t * WAVE];
+            }
+            pk16x2 e = as_pk(pack_hi(topL, carry));
+            pk16x2 p = h[0];
+            h[0] = as_pk(pack_lo(topL, carry));
+#pragma unroll
+            for (int m = 1; m < 9; m++) {
+                const uint32_t sl = (uint32_t)__builtin_amdgcn_sbfe(lutL[m], qshL, 4);
+                const uint32_t sh = (uint32_t)__builtin_amdgcn_sbfe(lutH[m], qshH, 4);
+                const pk16x2 tmp = p + as_pk(pack_lo(sl, sh));
+                const pk16x2 hm = pmax(pmax(tmp, f[m]), e);
+                const pk16x2 tt = tmp - gapoe2;
+                f[m] = pmax(tt, f[m] - gape2);
+                e = pmax(tt, e - gape2);
+                p = h[m];
+                h[m] = hm;
+            }
+            const uint32_t h8 = as_u32(h[8]), e8 = as_u32(e);
+            if (t >= 1) bd[(t - 1) * WAVE] = pack_hi(h8, e8);
+            carry = pack_lo(h8, e8);
+            if (last_lo && t == lastcol) {                                                 // :98-103
+#pragma unroll
+                for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) & 0xFFFFu) : score;
+            }
+            if (last_hi && t - 1 == lastcol) {
+#pragma unroll
+                for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) >> 16) : score;
+            }
+            if (t == 0) {   // the high strip starts at the next step: its column -1 again
+#pragma unroll
+                for (int m = 0; m < 9; m++) {
+                    h[m] = as_pk((as_u32(h[m]) & 0xFFFFu) | ((uint32_t)hl(8 * (s0 + 1) + m) << 16));
+                    f[m] = as_pk((as_u32(f[m]) & 0xFFFFu) | ((uint32_t)NEG16 << 16));
+                }
+            }
+        }
+    }
+    d.score[x.pair] = score;
+}
+
 template <bool TB, bool LUT>
 __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
     if (LUT ? !census_fast(census) : (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
+    if (LUT && !TB && d.pk16g) {
+        lane_global_pk(d, c, x, census);
+        return;
+    }
     const int32_t gapo = c.gapo, gapoe = c.gapoe, gape = c.gape;
     const int32_t QW = x.QW, TW = x.TW;
     uint32_t *__restrict__ bd = x.bd;

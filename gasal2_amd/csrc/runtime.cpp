@@ -1244,6 +1244,17 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
         if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
     }
+    // Packed 16-bit GLOBAL score: every value of the lane path fits int16 with
+    // room below for the packed "minus infinity" (-30000): |H| <= match and
+    // penalty times the shorter side plus open and extend over both sides,
+    // E and F at most one open and one penalty lower.
+    d.pk16g = 0;
+    if (d.fast && op.algo == GLOBAL && op.start != WITH_TB && g_pk16_mode && sc.gape <= 255) {
+        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
+        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
+        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
+        if (bound <= 16000) d.pk16g = 1;
+    }
     // LOCAL lane-per-pair score-table kernel: position encoding
     // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
     // (all pairs, or those the one-wave path leaves).

@@ -1,4 +1,4 @@
-"""Packed 16-bit LOCAL (two strips per lane in register halves): bit-exact
+"""Packed 16-bit LOCAL and GLOBAL (two strips per lane in register halves): bit-exact
 against the oracle and against the int32 kernels, including odd strip
 counts, ties, other scorings and the fall back to int32 when a score could
 exceed int16."""
@@ -57,3 +57,57 @@ def test_pk16_falls_back_when_scores_could_exceed_int16():
     b = g.make_batch([q, q[:300]], [q, q[:300]])
     got, ref = run_both(b, g.LOCAL, g.WITHOUT_START, False, scores=(7, 2, 3, 1))
     assert_parity(got, ref, g.LOCAL, g.WITHOUT_START, False, b)
+
+
+# ---------------------------------------------------------------- GLOBAL ----
+def both_ways_global(b, start=g.WITH_START, **kw):
+    L = g.lib()
+    try:
+        L.gasal2_set_pk16(0)
+        g32, _ = run_both(b, g.GLOBAL, start, False, **kw)
+        L.gasal2_set_pk16(1)
+        g16, ref = run_both(b, g.GLOBAL, start, False, **kw)
+    finally:
+        L.gasal2_set_pk16(1)
+    assert_parity(g16, ref, g.GLOBAL, start, False, b)
+    assert np.array_equal(g16["score"], g32["score"])
+
+
+@pytest.mark.parametrize("start", [g.WITHOUT_START, g.WITH_START])
+@pytest.mark.parametrize("cfg,n_penalty,is_rc", [(1, None, False), (2, 1, True), (3, None, False)])
+def test_pk16_global_synthetic(cfg, n_penalty, is_rc, start):
+    b = g.synth_batch(cfg, 2600 + cfg, 700)
+    both_ways_global(b, start, n_penalty=n_penalty, is_rc=is_rc)
+
+
+@pytest.mark.parametrize("scores", [(2, 3, 5, 2), (7, 1, 1, 1), (1, 8, 7, 1)])
+def test_pk16_global_other_scores(scores):
+    b = g.synth_batch(2, 2610, 500)
+    both_ways_global(b, scores=scores, n_penalty=1)
+
+
+def test_pk16_global_odd_strips_and_tiny():
+    """Odd and even strip counts, one-base pairs, lengths off multiples of 8
+    (the score row of the last strip in either register half), unrelated
+    pairs (deeply negative scores), N bases under both N rules."""
+    rng = np.random.default_rng(2620)
+    qs = [b"A", b"ACGT", b"ACGTACGTA", b"N" * 9, b"ACGTN" * 30]
+    ts = [b"A", b"TACGT", b"ACGTACGT" * 3, b"NACGTNACGTN", b"ACGT" * 2]
+    for _ in range(60):
+        lq, lt = int(rng.integers(1, 220)), int(rng.integers(1, 220))
+        qs.append(bytes(rng.choice(list(b"ACGT"), lq).tolist()))
+        ts.append(bytes(rng.choice(list(b"ACGT"), lt).tolist()))
+    b = g.make_batch(qs, ts)
+    both_ways_global(b)
+    both_ways_global(b, n_penalty=1)
+
+
+def test_pk16_global_falls_back_when_values_could_exceed_int16():
+    """Unrelated pairs of 5000 bases: scores far below -16000 are possible, the
+    int32 kernel runs."""
+    rng = np.random.default_rng(2630)
+    q = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, 5000)].tobytes()
+    t = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, 4800)].tobytes()
+    b = g.make_batch([q, q[:300]], [t, t[:280]])
+    got, ref = run_both(b, g.GLOBAL, g.WITH_START, False, scores=(1, 4, 6, 1))
+    assert_parity(got, ref, g.GLOBAL, g.WITH_START, False, b)
