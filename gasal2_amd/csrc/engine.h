@@ -73,6 +73,7 @@ struct G2Dev {
     int32_t fast, enc_ok;
     int32_t pk16;                    // LOCAL score and ends in packed 16-bit halves (lane_local_pk)
     int32_t pk16g;                   // GLOBAL score in packed 16-bit halves (lane_global_pk)
+    int32_t pk16s;                   // SEMI_GLOBAL (tails none / target) in packed 16-bit halves (semi_pass_pk)
     int32_t enc_s, enc_cb;           // position encoding: h << enc_s | ~strip << (3+enc_cb) | ~col << 3 | 7-row
     uint32_t enc_cm, enc_sm;         // column / strip field masks
     // intra-pair wavefront for long pairs (LOCAL score + ends): pairs whose

@@ -1399,6 +1399,126 @@ __device__ __forceinline__ uint32_t rev_word(const uint32_t *W, int32_t len, int
     return nib_reverse(win);
 }
 
+// One pass of k_semi (forward, or with REV the WITH_START pass over reversed
+// copies) with two strips per lane in packed 16-bit halves, skewed by one
+// column as in lane_local_pk; tails that end on the target only (TT) or none.
+// Same recurrences (semi_column), strip initialisation (semi_strip_init) and
+// end bookkeeping per row in strip order (src/kernels/semiglobal_kernel_template.h
+// :114-178, :269-360); NEG16 stands for MINUS_INF.  REV: strips from i0 while
+// the best is below fwd_score; the strip after the one that reaches it is not
+// run (the reference's loop exit).
+template <bool SECOND, int HEAD, bool TT, bool REV>
+__device__ void semi_pass_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census, int32_t nval, int32_t i0,
+                             int32_t fwd_score, int32_t &maxHH, int32_t &maxY, int32_t &sec, int32_t &secy) {
+    const int32_t QW = x.QW, TW = x.TW, C = 8 * QW;
+    const int32_t read_len = (int32_t)x.qlen, ref_len = (int32_t)x.tlen;
+    const int32_t gapo = c.gapo, gape = c.gape;
+    const pk16x2 gapoe2 = {(short)c.gapoe, (short)c.gapoe}, gape2 = {(short)c.gape, (short)c.gape};
+    const pk16x2 neg2 = {(short)NEG16, (short)NEG16};
+    constexpr bool HT = HEAD == D_TARGET || HEAD == D_BOTH;
+    uint32_t *__restrict__ bd = x.bd;
+    for (int32_t cc = 0; cc < C; cc++)                                                     // :87-99
+        bd[cc * WAVE] = (HEAD == D_QUERY || HEAD == D_BOTH) ? 0u : (cc == 0 ? pk16(0, NEG16) : pk16(-(gapo + gape * cc), NEG16));
+    // semi_strip_init for the strip whose row counter is u (HEAD NONE / QUERY)
+    auto h_init = [&](int32_t u, int m) -> int32_t { return HT ? 0 : -(gapo + gape * (u + m - 2)); };
+    auto p_init = [&](int32_t u, int m) -> int32_t { return (HT || u + m - 1 == 1) ? 0 : -(gapo + gape * (u + m - 2)); };
+    const int32_t tend_col = read_len - 1;
+    int32_t u = 1;
+    bool stop = false;
+    for (int32_t s0 = i0; s0 < TW && !stop; s0 += 2) {
+        if (REV && !(maxHH < fwd_score)) break;
+        const bool hi_ok = s0 + 1 < TW;
+        auto strip_word = [&](int32_t i) -> uint32_t { return REV ? (i >= 0 ? rev_word(x.T, ref_len, i) : 0u) : x.T[i]; };
+        const uint32_t gL = strip_word(s0), gH = hi_ok ? strip_word(s0 + 1) : 0u;
+        uint32_t lutL[9], lutH[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            lutL[m] = score_lut(c, nval, census, (gL >> (32 - 4 * m)) & 15u);
+            lutH[m] = score_lut(c, nval, census, (gH >> (32 - 4 * m)) & 15u);
+        }
+        const int32_t uL = u, uH = u + 8;
+        u += 16;
+        pk16x2 h[9], f[9], p[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            h[m] = as_pk(pack_lo((uint32_t)h_init(uL, m), (uint32_t)h_init(uH, m)));
+            p[m] = as_pk(pack_lo((uint32_t)p_init(uL, m), (uint32_t)p_init(uH, m)));
+            f[m] = neg2;
+        }
+        const int32_t gL8 = s0 << 3, gH8 = (s0 + 1) << 3;
+        bool hi_live = hi_ok;
+        uint32_t carry = 0, qshL = 0, qshH = 0, rpac = 0;
+        for (int32_t t = 0; t <= C; t++) {
+            qshH = qshL;
+            uint32_t topL = 0;
+            if (t < C) {
+                const int kk = t & 7;
+                if (kk == 0) rpac = REV ? rev_word(x.Q, read_len, t >> 3) : x.Q[t >> 3];
+                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                topL = bd[t * WAVE];
+            }
+            pk16x2 e = as_pk(pack_hi(topL, carry));
+            pk16x2 ph = as_pk(pack_lo(topL, carry));   // H of the row above, this column
+            pk16x2 prev = ph - gapoe2;
+#pragma unroll
+            for (int m = 1; m < 9; m++) {                                                  // :17-28
+                const uint32_t sl = (uint32_t)__builtin_amdgcn_sbfe(lutL[m], qshL, 4);
+                const uint32_t sh = (uint32_t)__builtin_amdgcn_sbfe(lutH[m], qshH, 4);
+                f[m] = pmax(h[m] - gapoe2, f[m] - gape2);
+                pk16x2 cur = pmax(p[m] + as_pk(pack_lo(sl, sh)), f[m]);
+                e = pmax(prev, e - gape2);
+                cur = pmax(cur, e);
+                h[m] = cur;
+                p[m] = ph;
+                ph = cur;
+                prev = cur - gapoe2;
+            }
+            const uint32_t h8 = as_u32(h[8]), e8 = as_u32(e);
+            if (t >= 1 && hi_ok) bd[(t - 1) * WAVE] = pack_hi(h8, e8);
+            carry = pack_lo(h8, e8);
+            if (TT && t == tend_col) {                                                     // :160-178 (low strip)
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t hv = (int32_t)(short)(as_u32(h[m]) & 0xFFFFu);
+                    const bool in = (gL8 + m - 1) < ref_len;
+                    const bool upd = hv > maxHH && in;
+                    maxY = upd ? gL8 + (m - 1) : maxY;
+                    maxHH = upd ? hv : maxHH;
+                    if (SECOND) {
+                        const bool ov = hv > sec && hv < maxHH && in;
+                        secy = ov ? gL8 + (m - 1) : secy;
+                        sec = ov ? hv : sec;
+                    }
+                }
+                if (REV && !(maxHH < fwd_score)) { hi_live = false; stop = true; }
+            }
+            if (TT && hi_live && t - 1 == tend_col) {                                      // (high strip)
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const int32_t hv = (int32_t)(short)(as_u32(h[m]) >> 16);
+                    const bool in = (gH8 + m - 1) < ref_len;
+                    const bool upd = hv > maxHH && in;
+                    maxY = upd ? gH8 + (m - 1) : maxY;
+                    maxHH = upd ? hv : maxHH;
+                    if (SECOND) {
+                        const bool ov = hv > sec && hv < maxHH && in;
+                        secy = ov ? gH8 + (m - 1) : secy;
+                        sec = ov ? hv : sec;
+                    }
+                }
+            }
+            if (t == 0) {   // the high strip starts at the next step: its column -1 again
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    h[m] = as_pk((as_u32(h[m]) & 0xFFFFu) | ((uint32_t)h_init(uH, m) << 16));
+                    p[m] = as_pk((as_u32(p[m]) & 0xFFFFu) | ((uint32_t)p_init(uH, m) << 16));
+                    f[m] = as_pk((as_u32(f[m]) & 0xFFFFu) | ((uint32_t)NEG16 << 16));
+                }
+            }
+        }
+    }
+}
+
 template <int START, bool SECOND, int HEAD, int TAIL, bool LUT>
 __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
     // the WITH_START pass reads reversed copies padded with nibble 0
@@ -1412,6 +1532,25 @@ __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
     uint32_t *__restrict__ bd = x.bd;
     constexpr bool TT = (TAIL == D_TARGET || TAIL == D_BOTH);
     constexpr bool TQ = (TAIL == D_QUERY || TAIL == D_BOTH);
+    if (LUT && !TQ && d.pk16s) {   // packed 16-bit passes (values fit int16, tails on the target only)
+        int32_t maxHH = MINUS_INF, maxY = read_len, sec = MINUS_INF, secy = read_len;
+        semi_pass_pk<SECOND, HEAD, TT, false>(d, c, x, census, nval, 0, 0, maxHH, maxY, sec, secy);
+        const uint32_t pr = x.pair;
+        d.score[pr] = maxHH;
+        d.tend[pr] = maxY;
+        d.qend[pr] = ref_len;   // maxXY_x keeps its initial value without a query tail (:63-64)
+        if (SECOND) { d.s_score[pr] = sec; d.s_tend[pr] = secy; d.s_qend[pr] = ref_len; }
+        if (START == S_WITH_START) {                                                       // :227-384
+            const int32_t fwd_score = maxHH;
+            const int32_t dd = TW - ((maxY >> 3) + 1);
+            const int32_t gend_reg = dd > 0 ? dd - 1 : dd;                                  // :273
+            int32_t rmax = MINUS_INF, ry = 0, rs = 0, rsy = 0;
+            semi_pass_pk<false, HEAD, TT, true>(d, c, x, census, nval, gend_reg, fwd_score, rmax, ry, rs, rsy);
+            d.tstart[pr] = (ref_len - 1) - ry;                                             // :380-381
+            d.qstart[pr] = (read_len - 1) - ref_len;
+        }
+        return;
+    }
 
     int32_t maxHH = MINUS_INF, maxXY_x = ref_len, maxXY_y = read_len;                      // :49,63-64
     int32_t sec = MINUS_INF, secx = ref_len, secy = read_len;                              // :71-74

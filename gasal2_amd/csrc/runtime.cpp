@@ -1255,6 +1255,16 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
         if (bound <= 16000) d.pk16g = 1;
     }
+    // Packed 16-bit SEMI_GLOBAL: the same bound (head gaps as GLOBAL's
+    // boundaries, classic Gotoh) for tails that end on the target or nowhere.
+    d.pk16s = 0;
+    if (d.fast && op.algo == SEMI_GLOBAL && op.start != WITH_TB && (op.tail == NONE || op.tail == TARGET) &&
+        g_pk16_mode && sc.gape <= 255) {
+        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
+        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
+        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
+        if (bound <= 16000) d.pk16s = 1;
+    }
     // LOCAL lane-per-pair score-table kernel: position encoding
     // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
     // (all pairs, or those the one-wave path leaves).

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import gasal2_amd as g
-from parity import assert_parity, run_both
+from parity import assert_parity, fields, run_both
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +111,42 @@ def test_pk16_global_falls_back_when_values_could_exceed_int16():
     b = g.make_batch([q, q[:300]], [t, t[:280]])
     got, ref = run_both(b, g.GLOBAL, g.WITH_START, False, scores=(1, 4, 6, 1))
     assert_parity(got, ref, g.GLOBAL, g.WITH_START, False, b)
+
+
+# ----------------------------------------------------------- SEMI_GLOBAL ----
+SEMI_PK = [(h, t) for h in (g.NONE, g.QUERY, g.TARGET, g.BOTH) for t in (g.NONE, g.TARGET)]
+
+
+def both_ways_semi(b, head, tail, start, second, **kw):
+    L = g.lib()
+    try:
+        L.gasal2_set_pk16(0)
+        g32, _ = run_both(b, g.SEMI_GLOBAL, start, second, head=head, tail=tail, **kw)
+        L.gasal2_set_pk16(1)
+        g16, ref = run_both(b, g.SEMI_GLOBAL, start, second, head=head, tail=tail, **kw)
+    finally:
+        L.gasal2_set_pk16(1)
+    assert_parity(g16, ref, g.SEMI_GLOBAL, start, second, b)
+    for k in fields(g.SEMI_GLOBAL, start, second):
+        assert np.array_equal(g16[k], g32[k]), k
+
+
+@pytest.mark.parametrize("head,tail", SEMI_PK)
+@pytest.mark.parametrize("start,second", [(g.WITHOUT_START, True), (g.WITH_START, False)])
+def test_pk16_semi_global(head, tail, start, second):
+    b = g.synth_batch(2, 2700 + 4 * head + tail, 400)
+    both_ways_semi(b, head, tail, start, second, n_penalty=1, is_rc=True)
+
+
+def test_pk16_semi_global_odd_strips_tiny_and_scores():
+    rng = np.random.default_rng(2720)
+    qs = [b"A", b"ACGT", b"ACGTACGTA", b"N" * 9, b"ACGTN" * 30]
+    ts = [b"A", b"TACGT", b"ACGTACGT" * 3, b"NACGTNACGTN", b"ACGT" * 2]
+    for _ in range(60):
+        lq, lt = int(rng.integers(1, 220)), int(rng.integers(1, 220))
+        qs.append(bytes(rng.choice(list(b"ACGT"), lq).tolist()))
+        ts.append(bytes(rng.choice(list(b"ACGT"), lt).tolist()))
+    b = g.make_batch(qs, ts)
+    for head, tail in ((g.TARGET, g.TARGET), (g.NONE, g.TARGET), (g.QUERY, g.NONE)):
+        both_ways_semi(b, head, tail, g.WITH_START, False, n_penalty=1)
+        both_ways_semi(b, head, tail, g.WITHOUT_START, True, scores=(2, 3, 5, 2))
