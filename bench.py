@@ -10,8 +10,9 @@ batch with the inputs already resident in HBM.
 
 Prints one JSON line.  `value` is GCUPS = sum over pairs of Lq*Lt (unpadded
 cells) / step time.  `roofline` prices the DP kernel against the integer-VALU
-peak (N_CU * 64 lanes * clock) at the DP kernel's algorithmic int ops per
-cell (15 for the default mode: recurrence 11 + second best 4); `traffic`
+peak (N_CU * 64 lanes * clock; twice that when the batch ran a packed 16-bit
+kernel, two cells per lane instruction) at the DP kernel's algorithmic int
+ops per cell (15 for the default mode: recurrence 11 + second best 4); `traffic`
 comes from rocprofv3 counter runs when given.  `cpu_baseline` times the CPU
 oracle (an OpenMP restatement of the reference kernels) on a bounded prefix of
 the same batch on this host and reports the GPU's agreement with it there.
@@ -209,6 +210,11 @@ def main():
     gcups = cells / (wall / args.steps) / 1e9
     dp_avg = float(np.mean(dp_ms))
     achieved_tops = cells * ops / (dp_avg * 1e-3) / 1e12
+    # packed 16-bit kernels do two cells' ops per lane instruction: their peak is
+    # twice the int32 one (only when no one-wave blocks shared the launch)
+    paths = a.last_paths()
+    packed = (paths & 1) and not (paths & 2) and mode != "ksw"
+    peak_used = peak_tops * (2 if packed else 1)
     log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); device {np.mean(dev_ms):.2f} ms; DP kernel {dp_avg:.2f} ms; "
         f"traceback {np.mean(tb_ms):.2f} ms")
 
@@ -293,9 +299,9 @@ def main():
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
         "roofline": {"bound": "valu",
                      "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",
-                     "ops_per_cell": ops,
-                     "achieved": round(achieved_tops, 3), "peak": round(peak_tops, 3), "unit": "TOP/s",
-                     "frac": round(achieved_tops / peak_tops, 4), "traffic": traffic,
+                     "ops_per_cell": ops, "packed_int16": bool(packed),
+                     "achieved": round(achieved_tops, 3), "peak": round(peak_used, 3), "unit": "TOP/s",
+                     "frac": round(achieved_tops / peak_used, 4), "traffic": traffic,
                      "dp_kernel_ms": round(dp_avg, 3)},
         "cpu_baseline": cpu,
         "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 3),

@@ -174,6 +174,7 @@ _SIGS = {
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
     "gasal2_set_tb_checkpoint": (None, [C.c_int32]),
     "gasal2_set_panels": (None, [C.c_int32]),
+    "gasal2_last_paths": (C.c_int32, [C.POINTER(gasal_gpu_storage_t)]),
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_tune_clear": (None, []),
     "gasal2_tune_count": (C.c_int32, []),
@@ -630,6 +631,11 @@ class Aligner:
 
     def stage_ms(self, stage):
         return self._L.gasal2_stage_ms(C.byref(self.s), stage)
+
+    def last_paths(self):
+        """gasal2_last_paths: bit 0 packed 16-bit lane kernel, bit 1 one-wave
+        blocks, bit 2 checkpointed traceback."""
+        return int(self._L.gasal2_last_paths(C.byref(self.s)))
 
 
 def cigar_to_string(cg):

@@ -121,6 +121,7 @@ struct Impl {
     G2Dev main_dev;                  // device view of the last full or banded run
     bool main_valid = false;
     bool prepped = false;            // resident batch packed/reverse-complemented by gasal2_prep_resident
+    int32_t last_paths = 0;          // gasal2_last_paths of the last launched batch
 };
 
 Impl *impl_of(gasal_gpu_storage_t *s) {
@@ -197,6 +198,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
 extern "C" void gasal2_set_tb_checkpoint(int32_t mode) { g_tb_ckpt = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
+extern "C" int32_t gasal2_last_paths(gasal_gpu_storage_t *s) { return s && s->impl ? impl_of(s)->last_paths : 0; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
@@ -1313,6 +1315,7 @@ void upload_async(gasal_gpu_storage_t *s, uint32_t n, Parameters *params) {
 // trigger a new capture.  The program holds kernels only (no memset or copy
 // nodes).  Timed runs (per-stage events) launch eagerly.
 hipError_t launch_program(Impl *m, const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t st) {
+    m->last_paths = ((d.pk16 || d.pk16g || d.pk16s || d.ksw16) ? 1 : 0) | (d.wf_blocks ? 2 : 0) | (d.tbh ? 4 : 0);
     if (!g_graph_mode || m->tm.enabled) return g2_run(sc, op, d, st, &m->tm);
     const bool same = m->g_valid && memcmp(&m->g_dev, &d, sizeof(d)) == 0 && memcmp(&m->g_sc, &sc, sizeof(sc)) == 0 &&
                       memcmp(&m->g_op, &op, sizeof(op)) == 0;

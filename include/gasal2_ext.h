@@ -177,6 +177,12 @@ void gasal2_set_tb_checkpoint(int32_t mode);
  * reading its own pair's words.  Results are identical either way. */
 void gasal2_set_panels(int32_t on);
 
+/* Which kernel paths the storage's last batch took (no counterpart in the
+ * reference; for measurement): bit 0 a packed 16-bit kernel on the
+ * one-lane-per-pair path (or KSW's packed row), bit 1 one-wave-per-pair
+ * blocks, bit 2 checkpointed traceback on the one-lane-per-pair path. */
+int32_t gasal2_last_paths(gasal_gpu_storage_t *storage);
+
 /* Graph mode (default on): each storage captures its device program (pack,
  * reverse/complement, shape ordering, DP, traceback) into a HIP graph the first
  * time a batch shape is seen and replays it with one launch while the shape
