@@ -1726,8 +1726,11 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // eh[] row lives in device scratch laid out [column][lane] (int2 per entry).
 // EH16: H and E of the eh row packed as two uint16 (every KSW value is >= 0 and
 // the host checked h0 + match * min(Lq, Lt) <= 65535), halving the row's traffic.
-template <bool EH16>
+// LUT: scores from the row's table (batch census allows it), else per cell.
+template <bool EH16, bool LUT>
 __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop) {
+    const uint32_t census = d.aux[0];
+    if (LUT != (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const int32_t nval = c.has_np ? -c.np : 0;
@@ -1773,6 +1776,7 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             const int i = (int)(tt * 8 + tb);
             if (i >= (int)tlen) break;                                                     // :98-99
             const uint32_t gbase = (gpac >> (32 - (tb + 1) * 4)) & 0x0Fu;
+            const uint32_t lut = LUT ? score_lut(c, nval, census, gbase) : 0u;
             int t, f = 0, h1, m = 0, mj = -1;
             if (w >= 0) {
                 if (beg < i - w) beg = i - w;
@@ -1790,14 +1794,18 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // the lane's own beg skipped, so the wave's eh accesses stay on one
             // row (coalesced).
             uint32_t rpac = 0;
+            // the row entry of the next column is loaded one step ahead (it was
+            // written by the previous row; this row writes it only at its step)
+            int2 nx = jlo < end ? ld(jlo) : make_int2(0, 0);
             for (int jj = jlo; jj < end; jj++) {
                 if ((jj & 7) == 0) rpac = x.Q[jj >> 3];
+                int2 pv = nx;
+                if (jj + 1 < end) nx = ld(jj + 1);
                 if (jj < beg) continue;
                 const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
-                int2 pv = ld(jj);
                 int h, M = pv.x, e = pv.y;
                 pv.x = h1;
-                const int32_t s = sub_local(c, nval, rbase, gbase);
+                const int32_t s = LUT ? __builtin_amdgcn_sbfe(lut, (rbase & 7u) << 2, 4) : sub_local(c, nval, rbase, gbase);
                 M = M ? M + s : 0;
                 h = M > e ? M : e;
                 h = h > f ? h : f;
@@ -2258,8 +2266,13 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
             break;
         case A_KSW:
-            if (d.ksw16) k_ksw<true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
-            else k_ksw<false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+            if (d.ksw16) {
+                k_ksw<true, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+                k_ksw<true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+            } else {
+                k_ksw<false, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+                k_ksw<false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+            }
             break;
         default:
             break;  // UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21)

@@ -37,11 +37,11 @@ bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for ever
 // tiles favour sending more pairs to one wave each)
 constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
-uint64_t g_tb_one_size_max = 64ull << 30;
+uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all groups up to this many bytes
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
-int g_pk16_mode = 1;    // packed 16-bit LOCAL score kernel when values fit int16
-int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)   // lane-path tiles: one size for all groups up to this many bytes
+int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16
+int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
 // range); filled by the autotuner (gasal2_tune_set), consulted per batch.
@@ -1076,7 +1076,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     d.enc_ok = 0;
     auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
     const G2Scores &sc = g_scores;
-    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL;
+    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL || op.algo == KSW;
     if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
         auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
         // the N score of the substitution macros (src/gasal_kernels.h:37-56)
