@@ -1863,6 +1863,9 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
 // top: the row above (zero above strip 0), left: the left column state,
 // corner: H above-left of the tile; lut: the strip's score tables (the kernels
 // store checkpoints only when the batch's nibble census allows tables).
+// The words hold the reference's nibbles (src/kernels/local_kernel_template.h
+// :45-60): [3] F continues (t <= f - extend), [2] E continues, [1:0] source:
+// diagonal 0 / 1 by the score's sign, F 3, E 2, ties in that order.
 __device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *left, int32_t corner, uint32_t qword,
                                         const uint32_t *lut, const G2Scores &c, uint32_t *ch) {
     int32_t h[9], f[9];
@@ -1884,10 +1887,12 @@ __device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *lef
             const int32_t hm = max(max(max(tmp, f[m]), e), 0);
             const int32_t t = tmp - gapoe;
             const int32_t fx = f[m] - gape, ex = e - gape;
-            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
-            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
-            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
-            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+            const uint32_t na = (uint32_t)(tmp - hm);                       // sign: not from the diagonal
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)t + ~(uint32_t)fx, 31);   // sign: fx >= t
+            dir = __builtin_amdgcn_alignbit(dir, (uint32_t)t + ~(uint32_t)ex, 31);   // sign: ex >= t
+            dir = __builtin_amdgcn_alignbit(dir, na, 31);
+            // low bit: from F (hm <= f) when not diagonal, else the mismatch flag (score < 0)
+            dir = __builtin_amdgcn_alignbit(dir, (na & ((uint32_t)hm + ~(uint32_t)f[m])) | (~na & (uint32_t)sc), 31);
             f[m] = max(t, fx);
             e = max(t, ex);
             p = h[m];
@@ -1929,6 +1934,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const bool ck = LOCALT && !band && !wfp && lanefast && d.tbh != nullptr;
     // long pairs of k_local_fast's one-wave blocks with checkpoints (wf_local<.., CK>)
     const bool wck = LOCALT && wfp && d.wf_tbh != nullptr && census_fast(census);
+    // recomputed tiles hold the reference's nibbles; stored sign-bit words are decoded
+    const bool decode = fastfmt && !ck && !wck;
     const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
@@ -2012,7 +2019,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 const uint32_t word = c7 < 4 ? (c7 < 2 ? (c7 == 0 ? ch[0] : ch[1]) : (c7 == 2 ? ch[2] : ch[3]))
                                              : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
                 cell_op = (word >> (28 - 4 * (i & 7))) & 15u;
-                if (fastfmt) {
+                if (decode) {
                     // the cell the tile word belongs to: column col, row 8 strip + (i & 7)
                     const uint32_t nA = (cell_op >> 3) & 1u, nB = (cell_op >> 2) & 1u;
                     const uint32_t nC3 = (cell_op >> 1) & 1u, nC2 = cell_op & 1u;
