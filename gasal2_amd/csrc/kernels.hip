@@ -2010,6 +2010,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
             }
             ch_key = key;
         }
+        if (LOCALT && (ck || wck)) {
+            // recomputed tiles: reference nibbles, and a LOCAL walk stays inside
+            // the matrix, so a step reads the column's word (re-selected only on
+            // a column move) and the tile changes when i >> 3 or j >> 3 does
+            auto colword = [&](int32_t jj) -> uint32_t {
+                const int c7 = jj & 7;
+                return c7 < 4 ? (c7 < 2 ? (c7 == 0 ? ch[0] : ch[1]) : (c7 == 2 ? ch[2] : ch[3]))
+                              : (c7 < 6 ? (c7 == 4 ? ch[4] : ch[5]) : (c7 == 6 ? ch[6] : ch[7]));
+            };
+            uint32_t wcol = colword(j);
+            while (true) {
+                const uint32_t cell_op = (wcol >> (28 - 4 * (i & 7))) & 15u;
+                const uint32_t op = (cell_op >> op_shift) & (uint32_t)op_select;
+                op_to_fill = op == 0 || op_select == 3 ? op : (uint32_t)op_shift;
+                op_select = op == 0 || (op == 1 && op_select == 3) ? 3 : 1;
+                op_shift = op == 0 || (op == 1 && op_select == 3) ? 0 : ((op == 2 || op == 3) ? (int)op : op_shift);
+                if (count < 63 && op_to_fill == prev_op_to_fill) {
+                    count++;
+                } else {
+                    if (count > 0) { cg[offset++] = (uint8_t)(prev_op_to_fill | (uint32_t)(count << 2)); n_ops++; }
+                    count = 1;
+                }
+                curr_score += ((op_to_fill == 2 || op_to_fill == 3) && prev_op_to_fill != op_to_fill) ? -c.gapoe
+                            : ((op_to_fill == 2 || op_to_fill == 3) ? -c.gape : (op_to_fill == 1 ? -c.mismatch : c.match));
+                if (curr_score == total_score) { done = true; break; }
+                prev_op_to_fill = op_to_fill;
+                const int32_t i0 = i, j0 = j;
+                i = op_to_fill != 3 ? i - 1 : i;   // ops 0, 1, 2 move up
+                j = op_to_fill != 2 ? j - 1 : j;   // ops 0, 1, 3 move left
+                if (!(i >= 0 && j >= 0)) { done = true; break; }
+                if (((i0 ^ i) | (j0 ^ j)) & ~7) break;
+                if (j != j0) wcol = colword(j);
+            }
+            continue;
+        }
         while (true) {
             const int32_t strip = (i >> 3) + (j >= read_len_8 ? 1 : 0);
             const int32_t col = j >= read_len_8 ? j - read_len_8 : j;
