@@ -1907,6 +1907,13 @@ __device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *lef
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
 template <bool LOCALT>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_get_tb(G2Dev d, G2Scores c) {
+    // score table of each target nibble for the tile recompute (one per block,
+    // built before any thread leaves)
+    __shared__ uint32_t lut16[16];
+    if (LOCALT) {
+        if (threadIdx.x < 16) lut16[threadIdx.x] = score_lut(c, c.has_np ? -c.np : 0, d.aux[0], threadIdx.x);
+        __syncthreads();
+    }
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= d.n) return;
     const uint32_t pr = d.perm[slot];
@@ -1983,7 +1990,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 if (strip != lut_key) {   // the walk stays in one strip for several tiles
                     lut_key = strip;
 #pragma unroll
-                    for (int m = 0; m < 8; m++) lut[m] = score_lut(c, nval, census, (gw >> (28 - 4 * m)) & 15u);
+                    for (int m = 0; m < 8; m++) lut[m] = lut16[(gw >> (28 - 4 * m)) & 15u];
                 }
                 uint32_t top[8], left[8];
                 int32_t corner = 0;
