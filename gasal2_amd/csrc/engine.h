@@ -48,7 +48,7 @@ struct G2Dev {
     uint32_t *hist;                  // G2_SHAPE_BUCKETS counters / bucket offsets
     uint32_t *bnd;                   // boundary rows: per group bnd_cols*64 dwords
     uint32_t bnd_cols;
-    uint32_t *tb;                    // direction words: per group tb_tw*tb_qw8*64
+    uint32_t *tb;                    // direction words (or tile left states): per group tb_tw*tb_qw8*64
     uint32_t tb_tw, tb_qw8;
     // per-group tile regions (set when one size for all groups would not fit):
     // group g starts at dword tb_gbase[g], strips of tb_gqw[g] query words
