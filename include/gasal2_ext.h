@@ -139,8 +139,11 @@ int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint
 /* Direction tiles of the one-lane-per-pair kernels (WITH_TB) use one region
  * size for every 64-pair group while that totals at most `bytes` (default
  * 64 GiB); above it each group gets a region sized for its own pairs' shape
- * bucket (uneven batches).  Results are identical either way.  No reference
- * counterpart (src/ctors.cpp:48-53 sizes for the maximum lengths). */
+ * bucket (uneven batches).  With the checkpointed traceback the tiles take
+ * two regions of that layout (the budget applies to one); checkpoints for
+ * one-wave pairs (mode 2) engage only when both of theirs fit in it.
+ * Results are identical either way.  No reference counterpart
+ * (src/ctors.cpp:48-53 sizes for the maximum lengths). */
 void gasal2_set_tile_budget(uint64_t bytes);
 
 /* Packed 16-bit DP (default on): LOCAL score-and-ends batches whose values fit

@@ -167,3 +167,31 @@ def test_per_group_tile_regions(algo, second):
         assert_parity(got, ref, algo, g.WITH_TB, second, b)
     finally:
         L.gasal2_set_tile_budget(64 << 30)
+
+
+def test_last_paths_report_the_kernels_taken():
+    """gasal2_last_paths: checkpoints for LOCAL WITH_TB, packed 16-bit for the
+    score modes of the full-feature shapes, neither for WITH_TB off."""
+    b = g.synth_batch(2, 990, 300)
+    cases = [(g.LOCAL, g.WITH_TB, True, 4), (g.LOCAL, g.WITHOUT_START, False, 1), (g.GLOBAL, g.WITH_START, False, 1),
+             (g.SEMI_GLOBAL, g.WITH_START, False, 1)]
+    for algo, start, second, bit in cases:
+        p = g.Parameters(algo=algo, start_pos=start, secondBest=g.TRUE if second else g.FALSE,
+                         semiglobal_skipping_head=g.TARGET, semiglobal_skipping_tail=g.TARGET)
+        a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n, n_penalty=1)
+        try:
+            a.run(b)
+            assert a.last_paths() & bit, (algo, start, a.last_paths())
+        finally:
+            a.close()
+    g.lib().gasal2_set_tb_checkpoint(0)
+    try:
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB)
+        a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n, n_penalty=1)
+        try:
+            a.run(b)
+            assert not a.last_paths() & 4
+        finally:
+            a.close()
+    finally:
+        g.lib().gasal2_set_tb_checkpoint(1)
