@@ -97,6 +97,9 @@ def main():
                     help="one-wave-per-pair switch thresholds (gasal2_set_wavefront)")
     ap.add_argument("--compare-lane-only", action="store_true",
                     help="also time dispatch policy 2 (one lane per pair only) on the same batch")
+    ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
+                    help="tune the lane / one-wave switch on a sample of this many pairs of the batch "
+                         "(gasal2_amd.autotune) and time the batch with the winner")
     ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1, 2],
                     help="LOCAL WITH_TB: checkpointed traceback on the lane path (1, default), also for one-wave "
                          "pairs (2), or direction words stored by the DP (0)")
@@ -178,6 +181,18 @@ def main():
     L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
     if args.wf:
         L.gasal2_set_wavefront(*args.wf)
+    tuned = None
+    if args.autotune:
+        from gasal2_amd import autotune
+        rng = np.random.default_rng(7)
+        sample = batch.subset(np.sort(rng.choice(batch.n, min(args.autotune, batch.n), replace=False)))
+        t0 = time.time()
+        r = autotune.tune(sample, p, n_penalty=1)
+        L.gasal2_set_wavefront(*r["best"])
+        tuned = {"sample_pairs": sample.n, "best": r["best"], "seconds": round(time.time() - t0, 1),
+                 "candidates": len(r["timings"])}
+        log(f"autotune on {sample.n} pairs: best {r['best']} of {len(r['timings'])} candidates "
+            f"({tuned['seconds']} s)")
 
     dev_ms = []   # device time of each timed step (first to last event on the stream)
 
@@ -320,6 +335,8 @@ def main():
     }
     if lane_only:
         line["lane_only"] = lane_only
+    if tuned:
+        line["autotune"] = tuned
     if pipe:
         line["pipeline"] = pipe
     print(json.dumps(line), flush=True)

@@ -8,9 +8,11 @@ between 8-row strips (src/kernels/local_kernel_template.h:152-154), so any
 other height could change results on overflow.  Cells per lane are fixed by
 the kernels (8 rows x 8 columns per register tile).
 
-How: on a sample batch resident in device memory, every candidate runs once to
+How: on a batch resident in device memory, every candidate runs once to
 capture its graph, then `repeats` timed replays; the candidate with the lowest
-median wins.  Every candidate's outputs must be identical to the first one's
+median wins.  Tune on batches of the size the caller runs: the best switch
+depends on how many long pairs share the persistent blocks (a 50k-pair sample
+of the 500k uneven batch picks thresholds that cost that batch 43%).  Every candidate's outputs must be identical to the first one's
 (a tuned choice never changes results) -- the tuner raises otherwise.
 
 The winners go into a JSON cache keyed by device, mode and mean-query-length
@@ -77,7 +79,7 @@ def tune(batch, params, candidates=None, repeats=3, scores=(1, 4, 6, 1), n_penal
             if log:
                 log(f"autotune {cand}: {ms:.3f} ms")
     finally:
-        L.gasal2_set_wavefront(*DEFAULT_WF)
+        L.gasal2_set_wavefront(0, 0, 0)   # back to the per-mode defaults
         a.close()
     best = min(timings, key=lambda t: t["ms"])
     mean_q = float(batch.qlen.mean()) if batch.n else 0.0

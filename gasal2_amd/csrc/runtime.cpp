@@ -208,6 +208,13 @@ extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
 extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
 extern "C" int32_t gasal2_get_graph_mode(void) { return g_graph_mode; }
 extern "C" void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks) {
+    if (!min_target_words && !min_cells && !max_blocks) {   // back to the per-mode defaults
+        g_wf_min_tw = 48;
+        g_wf_min_cells = 1u << 20;
+        g_wf_max_blocks = 2048;
+        g_wf_user = false;
+        return;
+    }
     g_wf_min_tw = min_target_words ? min_target_words : 1u;
     g_wf_min_cells = min_cells;
     g_wf_max_blocks = max_blocks ? max_blocks : 1u;

@@ -203,8 +203,9 @@ void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *la
  * pair when its target has at least min_target_words 8-base words and
  * 64 * query words * target words >= min_cells; at most max_blocks persistent
  * one-wave blocks take such pairs from a device queue.  Defaults 48, 2^20,
- * 2048 (WITH_TB: 32, 2^18, 2048 until this is called).  No reference counterpart (the reference has one thread per pair,
- * src/kernels/local_kernel_template.h:74-75). */
+ * 2048 (WITH_TB: 32, 2^18, 2048 until this is called); (0, 0, 0) restores
+ * those per-mode defaults.  No reference counterpart (the reference has one
+ * thread per pair, src/kernels/local_kernel_template.h:74-75). */
 void gasal2_set_wavefront(uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks);
 
 /* Properties of the current device used to state the integer-VALU peak:

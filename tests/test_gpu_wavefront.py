@@ -10,22 +10,22 @@ from parity import assert_parity, run_both
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT_WF = (48, 1 << 20, 2048)
 
 
 class policy:
     """Dispatch policy / switch thresholds for one block, restored after."""
 
-    def __init__(self, dispatch=0, wf=DEFAULT_WF):
+    def __init__(self, dispatch=0, wf=None):
         self.dispatch, self.wf = dispatch, wf
 
     def __enter__(self):
         g.lib().gasal2_set_dispatch(self.dispatch)
-        g.lib().gasal2_set_wavefront(*self.wf)
+        if self.wf:
+            g.lib().gasal2_set_wavefront(*self.wf)
 
     def __exit__(self, *exc):
         g.lib().gasal2_set_dispatch(0)
-        g.lib().gasal2_set_wavefront(*DEFAULT_WF)
+        g.lib().gasal2_set_wavefront(0, 0, 0)   # the per-mode defaults
 
 
 def related_pairs(shapes, seed, sub=0.03, indel=0.01, alphabet=b"ACGT"):
