@@ -79,7 +79,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--mode", default=None,
-                    choices=["local_tb_second", "local_tb", "local_score", "global", "semi_global", "ksw", "banded"],
+                    choices=["local_tb_second", "local_tb", "local_score", "local_start", "global", "semi_global", "ksw",
+                             "banded"],
                     help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
     ap.add_argument("--workload", default="full", choices=["full", "uneven", "seed", "band"],
                     help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
@@ -154,6 +155,9 @@ def main():
         ops = OPS_SCORE + (0 if args.tb_checkpoint else OPS_DIR)
     elif mode == "local_score":
         p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START, isReverseComplement=True)
+        ops = OPS_SCORE
+    elif mode == "local_start":   # forward pass plus the reverse pass up to the start (counted once)
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_START, isReverseComplement=True)
         ops = OPS_SCORE
     elif mode == "global":
         p = g.Parameters(algo=g.GLOBAL, start_pos=g.WITH_START, isReverseComplement=True)

@@ -996,22 +996,22 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
 #pragma unroll
             for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
             const uint32_t gpac = *(Te - i);
-            uint32_t gb[9];
+            uint32_t gb[9];   // score tables of the reversed rows (the batch census allows them here)
 #pragma unroll
-            for (int m = 1; m < 9; m++) gb[m] = (gpac >> (4 * (m - 1))) & 15u;
+            for (int m = 1; m < 9; m++) gb[m] = score_lut(c, nval, census, (gpac >> (4 * (m - 1))) & 15u);
             gidx -= 8;
             int32_t ridx = (rend_reg << 3) - 1;
             int32_t gi = 0;
             for (int32_t j = 0; j < rend_reg && go; j++) {
                 const uint32_t rpac = *(Qe - j);
                 for (int kk = 0; kk < 8 && go; kk++) {
-                    const uint32_t rbase = (rpac >> (4 * kk)) & 15u;
+                    const uint32_t qsh = ((rpac >> (4 * kk)) & 7u) << 2;
                     uint32_t hd = bd[gi * WAVE];
                     h[0] = sx16(hd);
                     int32_t e = sx16(hd >> 16);
 #pragma unroll
                     for (int m = 1; m < 9; m++) {
-                        const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                        const int32_t s = __builtin_amdgcn_sbfe(gb[m], qsh, 4);
                         const int32_t tmp = p[m] + s;
                         int32_t hm = max(max(tmp, f[m]), max(e, 0));
                         f[m] = max(tmp - gapoe, f[m] - gape);
