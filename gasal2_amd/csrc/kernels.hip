@@ -348,7 +348,8 @@ __device__ __forceinline__ bool census_fast(uint32_t m);
 
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
-    if (d.fast && (d.enc_ok || (START == S_WITHOUT_START && !SECOND && d.pk16)) && census_fast(d.aux[0]))
+    if (d.fast && (d.enc_ok || ((START == S_WITHOUT_START || START == S_WITH_START) && !SECOND && d.pk16)) &&
+        census_fast(d.aux[0]))
         return;   // k_local_fast handles this batch
     LaneCtx x;
     // long pairs of a score-only batch sort first and run one wave per pair
@@ -720,6 +721,70 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
     }
 }
 
+// The WITH_START reverse pass of the LOCAL kernels (src/kernels/local_kernel_template.h
+// :441-511): from the words holding the forward end, the DP over the reversed
+// prefix rectangle until a cell reaches the forward score; the start is the
+// reverse pass's first maximal cell.  Scores from the reversed rows' tables
+// (the batch census allows them for the kernels that call it).
+__device__ void local_reverse_start(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census, int32_t fwd_score,
+                                    int32_t maxXY_x, int32_t maxXY_y) {
+    const int32_t nval = c.has_np ? -c.np : 0;
+    const int32_t gapoe = c.gapoe, gape = c.gape;
+    const int32_t QW = x.QW, TW = x.TW;
+    uint32_t *__restrict__ bd = x.bd;
+    const uint32_t pr = x.pair;
+    {
+        int32_t maxHH = 0, prev_maxHH = 0;
+        const int32_t rend_reg = ((maxXY_x >> 3) + 1) < QW ? ((maxXY_x >> 3) + 1) : QW;
+        const int32_t gend_reg = ((maxXY_y >> 3) + 1) < TW ? ((maxXY_y >> 3) + 1) : TW;
+        const uint32_t *Qe = x.Q + (rend_reg - 1), *Te = x.T + (gend_reg - 1);
+        maxXY_x = 0; maxXY_y = 0;
+        for (int32_t cc = 0; cc < 8 * rend_reg; cc++) bd[cc * WAVE] = 0u;
+        int32_t gidx = ((gend_reg << 3) + 8) - 1;
+        bool go = maxHH < fwd_score;
+        for (int32_t i = 0; i < gend_reg && go; i++) {
+            int32_t h[9], f[9], p[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
+            const uint32_t gpac = *(Te - i);
+            uint32_t gb[9];   // score tables of the reversed rows (the batch census allows them here)
+#pragma unroll
+            for (int m = 1; m < 9; m++) gb[m] = score_lut(c, nval, census, (gpac >> (4 * (m - 1))) & 15u);
+            gidx -= 8;
+            int32_t ridx = (rend_reg << 3) - 1;
+            int32_t gi = 0;
+            for (int32_t j = 0; j < rend_reg && go; j++) {
+                const uint32_t rpac = *(Qe - j);
+                for (int kk = 0; kk < 8 && go; kk++) {
+                    const uint32_t qsh = ((rpac >> (4 * kk)) & 7u) << 2;
+                    uint32_t hd = bd[gi * WAVE];
+                    h[0] = sx16(hd);
+                    int32_t e = sx16(hd >> 16);
+#pragma unroll
+                    for (int m = 1; m < 9; m++) {
+                        const int32_t s = __builtin_amdgcn_sbfe(gb[m], qsh, 4);
+                        const int32_t tmp = p[m] + s;
+                        int32_t hm = max(max(tmp, f[m]), max(e, 0));
+                        f[m] = max(tmp - gapoe, f[m] - gape);
+                        e = max(tmp - gapoe, e - gape);
+                        maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
+                        maxHH = max(maxHH, hm);
+                        p[m] = h[m - 1];
+                        h[m] = hm;
+                    }
+                    bd[gi * WAVE] = pk16(h[8], e);
+                    maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
+                    prev_maxHH = max(maxHH, prev_maxHH);
+                    ridx--; gi++;
+                    go = maxHH < fwd_score;
+                }
+            }
+        }
+        d.qstart[pr] = maxXY_x;
+        d.tstart[pr] = maxXY_y;
+    }
+}
+
 // ------------------------------------------ LOCAL, packed 16-bit, one lane ----
 // Score and ends with two 8-row strips per lane, one in each 16-bit half of
 // every register (v_pk_add_u16 / v_pk_sub_i16 / v_pk_max_i16): the low half
@@ -739,7 +804,7 @@ __device__ __forceinline__ pk16x2 pmax(pk16x2 a, pk16x2 b) { return __builtin_el
 __device__ __forceinline__ uint32_t pack_lo(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x05040100u); }
 __device__ __forceinline__ uint32_t pack_hi(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x07060302u); }
 
-__device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census) {
+__device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x, uint32_t census, bool with_start) {
     const int32_t nval = c.has_np ? -c.np : 0;
     const pk16x2 gapoe2 = {(short)c.gapoe, (short)c.gapoe}, gape2 = {(short)c.gape, (short)c.gape};
     const pk16x2 zero2 = {0, 0};
@@ -817,6 +882,7 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
     d.score[x.pair] = bv;
     d.qend[x.pair] = bv > 0 ? bc : 0;
     d.tend[x.pair] = bv > 0 ? bs * 8 + br : 0;
+    if (with_start) local_reverse_start(d, c, x, census, bv, bv > 0 ? bc : 0, bv > 0 ? bs * 8 + br : 0);
 }
 
 // CK (WITH_TB only): checkpointed traceback.  No direction words: per 8x8
@@ -849,9 +915,9 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         return;
     }
     if (!census_fast(census)) return;
-    if (START == S_WITHOUT_START && !SECOND && d.pk16) {
+    if ((START == S_WITHOUT_START || START == S_WITH_START) && !SECOND && d.pk16) {
         LaneCtx x;
-        if (lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) lane_local_pk(d, c, x, census);
+        if (lane_setup(d, x, d.wf_blocks, d.wf_blocks ? d.aux[2] : 0u)) lane_local_pk(d, c, x, census, START == S_WITH_START);
         return;
     }
     if (!d.enc_ok) return;   // position encoding does not fit: k_local takes the short pairs
@@ -981,57 +1047,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         d.s_tend[pr] = sy;
         d.s_qend[pr] = vbest > 0 ? 8 * QW - 1 : 0;   // compares with the best (src/kernels/local_kernel_template.h:162)
     }
-    if (START == S_WITH_START) {                                                           // :441-511
-        int32_t maxHH = 0, prev_maxHH = 0;
-        const int32_t fwd_score = vbest;
-        const int32_t rend_reg = ((maxXY_x >> 3) + 1) < QW ? ((maxXY_x >> 3) + 1) : QW;
-        const int32_t gend_reg = ((maxXY_y >> 3) + 1) < TW ? ((maxXY_y >> 3) + 1) : TW;
-        const uint32_t *Qe = x.Q + (rend_reg - 1), *Te = x.T + (gend_reg - 1);
-        maxXY_x = 0; maxXY_y = 0;
-        for (int32_t cc = 0; cc < 8 * rend_reg; cc++) bd[cc * WAVE] = 0u;
-        int32_t gidx = ((gend_reg << 3) + 8) - 1;
-        bool go = maxHH < fwd_score;
-        for (int32_t i = 0; i < gend_reg && go; i++) {
-            int32_t h[9], f[9], p[9];
-#pragma unroll
-            for (int m = 0; m < 9; m++) { h[m] = 0; f[m] = 0; p[m] = 0; }
-            const uint32_t gpac = *(Te - i);
-            uint32_t gb[9];   // score tables of the reversed rows (the batch census allows them here)
-#pragma unroll
-            for (int m = 1; m < 9; m++) gb[m] = score_lut(c, nval, census, (gpac >> (4 * (m - 1))) & 15u);
-            gidx -= 8;
-            int32_t ridx = (rend_reg << 3) - 1;
-            int32_t gi = 0;
-            for (int32_t j = 0; j < rend_reg && go; j++) {
-                const uint32_t rpac = *(Qe - j);
-                for (int kk = 0; kk < 8 && go; kk++) {
-                    const uint32_t qsh = ((rpac >> (4 * kk)) & 7u) << 2;
-                    uint32_t hd = bd[gi * WAVE];
-                    h[0] = sx16(hd);
-                    int32_t e = sx16(hd >> 16);
-#pragma unroll
-                    for (int m = 1; m < 9; m++) {
-                        const int32_t s = __builtin_amdgcn_sbfe(gb[m], qsh, 4);
-                        const int32_t tmp = p[m] + s;
-                        int32_t hm = max(max(tmp, f[m]), max(e, 0));
-                        f[m] = max(tmp - gapoe, f[m] - gape);
-                        e = max(tmp - gapoe, e - gape);
-                        maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
-                        maxHH = max(maxHH, hm);
-                        p[m] = h[m - 1];
-                        h[m] = hm;
-                    }
-                    bd[gi * WAVE] = pk16(h[8], e);
-                    maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
-                    prev_maxHH = max(maxHH, prev_maxHH);
-                    ridx--; gi++;
-                    go = maxHH < fwd_score;
-                }
-            }
-        }
-        d.qstart[pr] = maxXY_x;
-        d.tstart[pr] = maxXY_y;
-    }
+    if (START == S_WITH_START) local_reverse_start(d, c, x, census, vbest, maxXY_x, maxXY_y);
 }
 
 // ----------------------------------------------------- LOCAL, diagonal band ----

@@ -1244,11 +1244,12 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         s->packed_tb_matrices = (uint4 *)m->tb;
         s->packed_tb_matrix_size = m->tb_bytes / 16;
     }
-    // Packed 16-bit LOCAL (score and ends): every value of the pairs the lane
+    // Packed 16-bit LOCAL (score and ends; WITH_START then runs the int32
+    // reverse pass on the packed forward's end): every value of the pairs the lane
     // path runs fits int16 -- H <= match * min(8 eq, 8 et); E, F >= -(mismatch,
     // N penalty) - open - 9 extend -- and the policy allows it.
     d.pk16 = 0;
-    if (d.fast && op.algo == LOCAL && op.start == WITHOUT_START && !op.second && g_pk16_mode) {
+    if (d.fast && op.algo == LOCAL && (op.start == WITHOUT_START || op.start == WITH_START) && !op.second && g_pk16_mode) {
         const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
         const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
         if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
