@@ -326,20 +326,43 @@ __global__ __launch_bounds__(256) void k_scatter(G2Dev d) {
 // Transposed sequence panels of the lane path: word j of the pair in lane l of
 // group g at qpan[(g * qpan_w + j) * 64 + l] (0 past the pair's words), the
 // same for targets.  A wave's read of one word is then 256 contiguous bytes
-// instead of 64 scattered lines.  One thread per sorted slot after the long
-// pairs.
+// instead of 64 scattered lines.  One block per lane group: 32 words of the
+// group's 64 pairs at a time are read pair-contiguously into LDS and written
+// out word-contiguously.
 __global__ __launch_bounds__(256) void k_panel(G2Dev d) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t tile[32][WAVE + 1];
+    __shared__ uint32_t s_qo[WAVE], s_to[WAVE], s_qw[WAVE], s_tw[WAVE];
     const uint32_t slot0 = d.wf_blocks ? d.aux[2] : 0u;
-    if (k >= d.n || k < slot0) return;
-    const uint32_t r = k - slot0, g = r / WAVE, l = r & (WAVE - 1);
-    const uint32_t pr = d.perm[k];
-    const uint32_t QW = (uint32_t)dregs(d.qlen[pr]), TW = (uint32_t)dregs(d.tlen[pr]);
-    const uint32_t *Q = d.pq + (d.qoff[pr] >> 3), *T = d.pt + (d.toff[pr] >> 3);
-    uint32_t *qo = d.qpan + (size_t)g * d.qpan_w * WAVE + l;
-    uint32_t *to = d.tpan + (size_t)g * d.tpan_w * WAVE + l;
-    for (uint32_t j = 0; j < d.qpan_w; j++) qo[(size_t)j * WAVE] = j < QW ? Q[j] : 0u;
-    for (uint32_t j = 0; j < d.tpan_w; j++) to[(size_t)j * WAVE] = j < TW ? T[j] : 0u;
+    const uint32_t g = blockIdx.x, t = threadIdx.x;
+    if ((uint64_t)g * WAVE + slot0 >= d.n) return;   // the whole block: past the lane groups
+    if (t < WAVE) {
+        const uint32_t slot = slot0 + g * WAVE + t;
+        const bool ok = slot < d.n;
+        const uint32_t pr = ok ? d.perm[slot] : 0u;
+        s_qo[t] = ok ? d.qoff[pr] >> 3 : 0u;
+        s_to[t] = ok ? d.toff[pr] >> 3 : 0u;
+        s_qw[t] = ok ? (uint32_t)dregs(d.qlen[pr]) : 0u;
+        s_tw[t] = ok ? (uint32_t)dregs(d.tlen[pr]) : 0u;
+    }
+    __syncthreads();
+    for (int side = 0; side < 2; side++) {
+        const uint32_t W = side ? d.tpan_w : d.qpan_w;
+        const uint32_t *src = side ? d.pt : d.pq;
+        uint32_t *out = (side ? d.tpan : d.qpan) + (size_t)g * W * WAVE;
+        for (uint32_t w0 = 0; w0 < W; w0 += 32) {
+            for (uint32_t idx = t; idx < 32 * WAVE; idx += 256) {   // pair-contiguous reads
+                const uint32_t l = idx >> 5, w = idx & 31u;
+                const uint32_t cnt = side ? s_tw[l] : s_qw[l];
+                tile[w][l] = w0 + w < cnt ? src[(side ? s_to[l] : s_qo[l]) + w0 + w] : 0u;
+            }
+            __syncthreads();
+            for (uint32_t idx = t; idx < 32 * WAVE; idx += 256) {   // word-contiguous writes
+                const uint32_t w = idx >> 6, l = idx & (WAVE - 1);
+                if (w0 + w < W) out[(size_t)(w0 + w) * WAVE + l] = tile[w][l];
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // ---------------------------------------------------------------- LOCAL -----
@@ -2259,7 +2282,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     k_keys<<<nb256, 256, 0, s>>>(d);
     k_scan_desc<<<1, 1024, 0, s>>>(d.hist);
     k_scatter<<<nb256, 256, 0, s>>>(d);
-    if (d.qpan) k_panel<<<nb256, 256, 0, s>>>(d);
+    if (d.qpan) k_panel<<<(d.n + WAVE - 1) / WAVE, 256, 0, s>>>(d);
     G2_LAUNCH_CHECK();
     // 4. DP (dispatch table src/gasal_align.h:7-107)
     const uint32_t groups = (d.n + WAVE - 1) / WAVE;
