@@ -217,40 +217,58 @@ __device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
         for (uint32_t i = 0; i < regs; i++) w[i] = nib_complement(w[i]);
 }
 
-// 16 threads per pair.  When the last word holds no nibble equal to N_CODE
+// RC_T threads per pair.  When the last word holds no nibble equal to N_CODE
 // (always so for a byte-sized N_CODE such as 'N'), the reference's reverse is a
 // plain reversal of the padded word sequence: thread t swaps words (i,
-// regs-1-i) for i = t, t+16, ...; otherwise the first thread runs the
-// reference's sequential loop.
+// regs-1-i) for i = t, t+RC_T, ...; otherwise the first thread runs the
+// reference's sequential loop.  The last word and the first RC_U word pairs
+// of each thread are loaded together (every index stays inside the pair's
+// words), so a pair costs one memory round trip after its metadata.
+constexpr uint32_t RC_T = 8, RC_U = 4;
 __device__ __forceinline__ void rc_side(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code, uint32_t t) {
     const uint32_t regs = (len >> 3) + ((len & 7) ? 1 : 0);
     if (regs == 0 || op == 0) return;
-    bool plain = true;
-    if (op & 1u) {
-        const uint32_t last = w[regs - 1];
-        for (int jj = 0; jj < 32; jj += 4) plain &= (((last >> jj) & 15u) != (uint32_t)n_code);
-    }
-    if (!plain) {
-        if (t == 0) rc_one(w, len, op, n_code);
-        return;
-    }
     const uint32_t to_swap = (regs >> 1) + (regs & 1);
-    for (uint32_t i = t; i < to_swap; i += 16) {
-        const uint32_t a = w[i], b = w[regs - 1 - i];
-        uint32_t na = (op & 1u) ? nib_reverse(b) : a;
-        uint32_t nb = (op & 1u) ? nib_reverse(a) : b;
-        if (op & 2u) { na = nib_complement(na); nb = nib_complement(nb); }
-        w[i] = na;
-        if (regs - 1 - i != i) w[regs - 1 - i] = nb;
+    for (uint32_t base = t; base < to_swap; base += RC_T * RC_U) {
+        uint32_t a[RC_U], b[RC_U];
+#pragma unroll
+        for (uint32_t u = 0; u < RC_U; u++) {
+            const uint32_t i = base + u * RC_T;
+            a[u] = i < to_swap ? w[i] : 0u;
+            b[u] = i < to_swap ? w[regs - 1 - i] : 0u;
+        }
+        if (base == t) {   // first round: decide between the plain reversal and the reference's loop
+            bool plain = true;
+            if (op & 1u) {
+                const uint32_t last = w[regs - 1];
+                for (int jj = 0; jj < 32; jj += 4) plain &= (((last >> jj) & 15u) != (uint32_t)n_code);
+            }
+            if (!plain) {
+                if (t == 0) rc_one(w, len, op, n_code);
+                return;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < RC_U; u++) {
+            const uint32_t i = base + u * RC_T;
+            if (i >= to_swap) break;
+            uint32_t na = (op & 1u) ? nib_reverse(b[u]) : a[u];
+            uint32_t nb = (op & 1u) ? nib_reverse(a[u]) : b[u];
+            if (op & 2u) { na = nib_complement(na); nb = nib_complement(nb); }
+            w[i] = na;
+            if (regs - 1 - i != i) w[regs - 1 - i] = nb;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
-    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, t = threadIdx.x & 15u;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / RC_T, t = threadIdx.x % RC_T;
     if (k >= d.n) return;
+    // all metadata in one round trip, before the operation bytes are tested
     const uint32_t qo = d.qop[k], to = d.top[k];
-    if (qo) rc_side(d.pq + (d.qoff[k] >> 3), d.qlen[k], qo, n_code, t);
-    if (to) rc_side(d.pt + (d.toff[k] >> 3), d.tlen[k], to, n_code, t);
+    const uint32_t qf = d.qoff[k], ql = d.qlen[k], tf = d.toff[k], tl = d.tlen[k];
+    if (qo) rc_side(d.pq + (qf >> 3), ql, qo, n_code, t);
+    if (to) rc_side(d.pt + (tf >> 3), tl, to, n_code, t);
 }
 
 // ------------------------------------------------------------------ sort ----
@@ -272,12 +290,9 @@ __device__ __forceinline__ uint32_t shape8(uint32_t x) {
     return v > 255u ? 255u : v;
 }
 
-// aux[2] counts the long pairs.
-__global__ __launch_bounds__(256) void k_keys(G2Dev d) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
+__device__ __forceinline__ uint32_t pair_key(const G2Dev &d, uint32_t k, bool &lng) {
     const uint32_t qw = (uint32_t)dregs(d.qlen[k]), tw = (uint32_t)dregs(d.tlen[k]);
-    const bool lng = d.wf_blocks > 0 && tw >= d.wf_min_tw && (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
+    lng = d.wf_blocks > 0 && tw >= d.wf_min_tw && (uint64_t)qw * tw * 64u >= (uint64_t)d.wf_min_cells;
     // banded runs: the band's words per strip stand in for the query words
     uint32_t cw = qw;
     if (d.band_lo) {
@@ -285,10 +300,49 @@ __global__ __launch_bounds__(256) void k_keys(G2Dev d) {
         const int64_t bw = span < 0 ? 1 : ((span + 14) >> 3) + 1;
         cw = (uint32_t)(bw < (int64_t)qw ? bw : (int64_t)qw);
     }
-    const uint32_t key = (lng ? 0x10000u : 0u) | (shape8(cw) << 8) | shape8(tw);
-    d.keys[k] = key;
-    atomicAdd(&d.hist[key], 1u);
-    if (lng) atomicAdd(&d.aux[2], 1u);
+    return (lng ? 0x10000u : 0u) | (shape8(cw) << 8) | shape8(tw);
+}
+
+// A batch holds a few hundred distinct keys, so one global atomic per pair
+// serialises on the same counters.  Each block takes KH_CHUNK pairs and counts
+// them in an LDS hash table (open addressing, at most KH_PROBE probes), then
+// adds each distinct key's count to the global histogram once.  A key that
+// finds no slot within KH_PROBE probes goes to the global counter directly.
+constexpr uint32_t KH_SLOTS = 2048, KH_PROBE = 32, KH_CHUNK = 8192, KH_THREADS = 1024;
+constexpr uint32_t KH_EMPTY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int kh_slot(uint32_t *tkey, uint32_t key) {
+    uint32_t h = (key * 2654435761u) >> 21;   // 11-bit multiplicative hash
+    for (uint32_t p = 0; p < KH_PROBE; p++, h = (h + 1u) & (KH_SLOTS - 1u)) {
+        const uint32_t prev = atomicCAS(&tkey[h], KH_EMPTY, key);
+        if (prev == KH_EMPTY || prev == key) return (int)h;
+    }
+    return -1;
+}
+
+// aux[2] counts the long pairs.
+__global__ __launch_bounds__(KH_THREADS) void k_keys(G2Dev d) {
+    static_assert(KH_SLOTS == 1u << 11, "kh_slot hashes to 11 bits");
+    __shared__ uint32_t tkey[KH_SLOTS], tcnt[KH_SLOTS];
+    __shared__ uint32_t nlong;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < KH_SLOTS; i += KH_THREADS) { tkey[i] = KH_EMPTY; tcnt[i] = 0; }
+    if (t == 0) nlong = 0;
+    __syncthreads();
+    const uint32_t k0 = blockIdx.x * KH_CHUNK, k1 = d.n - k0 < KH_CHUNK ? d.n : k0 + KH_CHUNK;
+    for (uint32_t k = k0 + t; k < k1; k += KH_THREADS) {
+        bool lng;
+        const uint32_t key = pair_key(d, k, lng);
+        d.keys[k] = key;
+        const int h = kh_slot(tkey, key);
+        if (h >= 0) atomicAdd(&tcnt[h], 1u);
+        else atomicAdd(&d.hist[key], 1u);
+        if (lng) atomicAdd(&nlong, 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < KH_SLOTS; i += KH_THREADS)
+        if (tcnt[i]) atomicAdd(&d.hist[tkey[i]], tcnt[i]);
+    if (t == 0 && nlong) atomicAdd(&d.aux[2], nlong);
 }
 
 // Exclusive scan of the histogram in descending key order, in place: hist[b]
@@ -317,10 +371,36 @@ __global__ __launch_bounds__(1024) void k_scan_desc(uint32_t *hist) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(G2Dev d) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.n) return;
-    d.perm[atomicAdd(&d.hist[d.keys[k]], 1u)] = k;
+// Places each pair at its bucket's next free slot.  As in k_keys, a block
+// ranks its KH_CHUNK pairs within their keys in LDS, claims each key's range
+// with one global atomic, then writes base + rank.  The order inside a bucket
+// is whatever the atomics give; no output depends on it.
+__global__ __launch_bounds__(KH_THREADS) void k_scatter(G2Dev d) {
+    __shared__ uint32_t tkey[KH_SLOTS], tcnt[KH_SLOTS];
+    __shared__ uint32_t rank[KH_CHUNK];   // slot << 16 | rank inside the slot, or KH_EMPTY
+    static_assert(KH_CHUNK <= 65536u && KH_SLOTS <= 65536u, "slot and rank share one word");
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < KH_SLOTS; i += KH_THREADS) { tkey[i] = KH_EMPTY; tcnt[i] = 0; }
+    __syncthreads();
+    const uint32_t k0 = blockIdx.x * KH_CHUNK, k1 = d.n - k0 < KH_CHUNK ? d.n : k0 + KH_CHUNK;
+    for (uint32_t k = k0 + t; k < k1; k += KH_THREADS) {
+        const uint32_t key = d.keys[k];
+        const int h = kh_slot(tkey, key);
+        if (h >= 0) {
+            rank[k - k0] = ((uint32_t)h << 16) | atomicAdd(&tcnt[h], 1u);
+        } else {
+            rank[k - k0] = KH_EMPTY;
+            d.perm[atomicAdd(&d.hist[key], 1u)] = k;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < KH_SLOTS; i += KH_THREADS)   // count -> first global slot
+        if (tcnt[i]) tcnt[i] = atomicAdd(&d.hist[tkey[i]], tcnt[i]);
+    __syncthreads();
+    for (uint32_t k = k0 + t; k < k1; k += KH_THREADS) {
+        const uint32_t v = rank[k - k0];
+        if (v != KH_EMPTY) d.perm[tcnt[v >> 16] + (v & 0xFFFFu)] = k;
+    }
 }
 
 // Transposed sequence panels of the lane path: word j of the pair in lane l of
@@ -544,11 +624,22 @@ __device__ __forceinline__ uint32_t score_lut(const G2Scores &c, int32_t nval, u
 __global__ __launch_bounds__(256) void k_census(const uint32_t *__restrict__ pq, uint32_t nq, const uint32_t *__restrict__ pt,
                                                uint32_t nt, uint32_t *__restrict__ out) {
     uint32_t m = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq + nt; i += stride) {
-        uint32_t w = i < nq ? pq[i] : pt[i - nq];
+    const uint32_t stride = gridDim.x * blockDim.x, n = nq + nt;
+    constexpr uint32_t U = 4;   // independent loads in flight per thread
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * stride) {
+        uint32_t w[U];
 #pragma unroll
-        for (int k = 0; k < 32; k += 4) m |= 1u << ((w >> k) & 15u);
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t j = i + u * stride;
+            w[u] = j < nq ? pq[j] : (j < n ? pt[j - nq] : 0u);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            uint32_t mw = 0;
+#pragma unroll
+            for (int k = 0; k < 32; k += 4) mw |= 1u << ((w[u] >> k) & 15u);
+            m |= i + u * stride < n ? mw : 0u;
+        }
     }
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
@@ -2261,7 +2352,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const uint32_t nb256 = (d.n + 255) / 256;
     // 2. reverse / complement (src/gasal_align.cu:232-246)
     if (op.is_rc && !op.skip_prep) {
-        k_rc<<<(d.n + 15) / 16, 256, 0, s>>>(d, sc.n_code);
+        k_rc<<<(d.n + 256 / RC_T - 1) / (256 / RC_T), 256, 0, s>>>(d, sc.n_code);
         G2_LAUNCH_CHECK();
     }
     if (op.prep_only) return hipSuccess;
@@ -2279,9 +2370,10 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds;
     // long pairs for the one-wave-per-pair path sort ahead of all others
     k_zero_u32<<<G2_SHAPE_BUCKETS / 256, 256, 0, s>>>(d.hist, G2_SHAPE_BUCKETS);
-    k_keys<<<nb256, 256, 0, s>>>(d);
+    const uint32_t nkh = (d.n + KH_CHUNK - 1) / KH_CHUNK;
+    k_keys<<<nkh, KH_THREADS, 0, s>>>(d);
     k_scan_desc<<<1, 1024, 0, s>>>(d.hist);
-    k_scatter<<<nb256, 256, 0, s>>>(d);
+    k_scatter<<<nkh, KH_THREADS, 0, s>>>(d);
     if (d.qpan) k_panel<<<(d.n + WAVE - 1) / WAVE, 256, 0, s>>>(d);
     G2_LAUNCH_CHECK();
     // 4. DP (dispatch table src/gasal_align.h:7-107)
