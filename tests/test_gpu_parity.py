@@ -195,3 +195,49 @@ def test_last_paths_report_the_kernels_taken():
             a.close()
     finally:
         g.lib().gasal2_set_tb_checkpoint(1)
+
+
+def _many_shapes_batch(n, seed, n_code=0x4E):
+    """n pairs, query and target lengths 1..400 drawn independently (so a block
+    of 8192 pairs holds more distinct (query, target) word shapes than the
+    sort's 2048-slot hash table), each target a 5%-mutated copy of the query
+    with random flanks or a mutated piece of it; the first 16 pairs are long
+    (2000 x 2600 bp); random reverse / complement ops on both sides."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+
+    def rand(m):
+        return acgt[rng.integers(0, 4, m)]
+
+    def mutate(s):
+        s = s.copy()
+        m = rng.random(len(s)) < 0.05
+        s[m] = rand(int(m.sum()))
+        return s
+
+    qs, ts = [], []
+    for k in range(n):
+        lq, lt = (2000, 2600) if k < 16 else (int(rng.integers(1, 401)), int(rng.integers(1, 401)))
+        q = rand(lq)
+        if lt >= lq:
+            a = int(rng.integers(0, lt - lq + 1))
+            t = np.concatenate([rand(a), mutate(q), rand(lt - lq - a)])
+        else:
+            a = int(rng.integers(0, lq - lt + 1))
+            t = mutate(q[a:a + lt])
+        qs.append(q.tobytes())
+        ts.append(t.tobytes())
+    return g.make_batch(qs, ts, qop=rng.integers(0, 4, n), top=rng.integers(0, 4, n), n_code=n_code)
+
+
+@pytest.mark.parametrize("start,second,n_code", [(g.WITHOUT_START, False, 0x4E), (g.WITH_TB, True, 0x4E),
+                                                 (g.WITH_TB, False, 0x0E)])
+def test_pair_order_many_shapes(start, second, n_code):
+    """The shape sort over two blocks of pairs, one of them overflowing its
+    hash table into global counters, with long one-wave pairs and multi-round
+    reverse / complement (N_CODE 0x0E: the reference's shifting reverse):
+    every pair is aligned once, into its own results."""
+    b = _many_shapes_batch(9000, 77, n_code)
+    got, ref = run_both(b, g.LOCAL, start, second, is_rc=True, n_code=n_code)
+    assert_parity(got, ref, g.LOCAL, start, second, b)
