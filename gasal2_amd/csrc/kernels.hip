@@ -61,13 +61,19 @@ struct LaneCtx {
 // groups, or per-group regions).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// A tile's eight words are stored as two halves of four, each half of all 64
-// lanes contiguous ([half][lane][4]), so every 16-byte store instruction of a
-// wave writes whole cache lines (1 KiB).  Stores and the DP's loads are
+// A tile's eight words are 32 contiguous bytes per lane ([lane][8] within the
+// 2 KiB chunk of a wave), so a traceback lane, whose walk has left its
+// neighbours' tiles, fetches one line per tile record instead of two; the
+// DP's two 16-byte stores per lane fill each line within one pair of
+// instructions.  (Halves of all 64 lanes laid out contiguously, [half][lane][4],
+// made every DP store a whole line but cost the traceback 4.78 against 3.72 ms
+// on 1M pairs and the DP 0.2-0.7 ms less.)  Stores and the DP's loads are
 // non-temporal: the tiles stream through L2 (written once, read once much
 // later) and would otherwise evict the sequence words the lanes re-read.
+constexpr uint32_t TB_HALF = 4u;   // word offset of a lane's second half
+constexpr uint32_t TB_LANE = 8u;   // word stride between lanes
 struct TbView {
-    uint32_t *base;   // the group's chunk (0, 0) plus 4 * lane
+    uint32_t *base;   // the group's chunk (0, 0) plus TB_LANE * lane
     uint32_t qw;
     __device__ __forceinline__ uint32_t *chunk(int32_t strip, int32_t word) const {
         return base + ((size_t)strip * qw + (size_t)word) * (WAVE * 8);
@@ -76,34 +82,34 @@ struct TbView {
         uint32_t *p = chunk(strip, word);
         const u32x4 a = {w[0], w[1], w[2], w[3]}, b = {w[4], w[5], w[6], w[7]};
         __builtin_nontemporal_store(a, reinterpret_cast<u32x4 *>(p));
-        __builtin_nontemporal_store(b, reinterpret_cast<u32x4 *>(p + WAVE * 4));
+        __builtin_nontemporal_store(b, reinterpret_cast<u32x4 *>(p + TB_HALF));
     }
     __device__ __forceinline__ void load_nt(int32_t strip, int32_t word, uint32_t *w) const {
         const uint32_t *p = chunk(strip, word);
         const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
-        const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p + WAVE * 4));
+        const u32x4 b = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p + TB_HALF));
         w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
         w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
     }
     __device__ __forceinline__ void load(int32_t strip, int32_t word, uint32_t *w) const {
         const uint32_t *p = chunk(strip, word);
-        const uint4 a = *reinterpret_cast<const uint4 *>(p), b = *reinterpret_cast<const uint4 *>(p + WAVE * 4);
+        const uint4 a = *reinterpret_cast<const uint4 *>(p), b = *reinterpret_cast<const uint4 *>(p + TB_HALF);
         w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
         w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
     }
     __device__ __forceinline__ uint32_t at(int32_t strip, int32_t word, int k) const {
-        return chunk(strip, word)[k < 4 ? k : WAVE * 4 + k - 4];
+        return chunk(strip, word)[k < 4 ? k : TB_HALF + k - 4];
     }
 };
 __device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t lane, uint32_t *region = nullptr) {
     TbView v;
     uint32_t *const r = region ? region : d.tb;
     if (d.tb_gbase) {
-        v.base = r + d.tb_gbase[grp] + (size_t)lane * 4;
+        v.base = r + d.tb_gbase[grp] + (size_t)lane * TB_LANE;
         v.qw = d.tb_gqw[grp];
     } else {
         v.qw = d.tb_qw8 >> 3;
-        v.base = r + (size_t)grp * d.tb_tw * v.qw * WAVE * 8 + (size_t)lane * 4;
+        v.base = r + (size_t)grp * d.tb_tw * v.qw * WAVE * 8 + (size_t)lane * TB_LANE;
     }
     return v;
 }
