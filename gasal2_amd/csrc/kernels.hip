@@ -2038,6 +2038,19 @@ __device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *lef
     }
 }
 
+// Word a of a packed sequence array through a 4-word register cache: the
+// aligned 16 bytes holding it are fetched when the walk first needs one of
+// them.  The arrays are 256-byte aligned with 4 words of slack past the last
+// sequence word, so the aligned read stays inside the allocation.
+__device__ __forceinline__ uint32_t seq_word4(const uint32_t *arr, uint32_t a, uint4 &cache, uint32_t &key) {
+    if ((a >> 2) != key) {
+        key = a >> 2;
+        cache = *reinterpret_cast<const uint4 *>(arr + (a & ~3u));
+    }
+    const uint32_t k = a & 3u;
+    return k < 2 ? (k == 0 ? cache.x : cache.y) : (k == 2 ? cache.z : cache.w);
+}
+
 // gasal_get_tb<T> (src/kernels/get_tb.h:4-149).  Reads the direction words the
 // DP kernel wrote for this pair's wave slot; cells outside the pair's computed
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
@@ -2079,7 +2092,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const bool wck = LOCALT && wfp && d.wf_tbh != nullptr && census_fast(census);
     // recomputed tiles hold the reference's nibbles; stored sign-bit words are decoded
     const bool decode = fastfmt && !ck && !wck;
-    const uint32_t *Qw = d.pq + (d.qoff[pr] >> 3), *Tw = d.pt + (d.toff[pr] >> 3);
+    const uint32_t qw0 = d.qoff[pr] >> 3;
+    const uint32_t *Qw = d.pq + qw0, *Tw = d.pt + (d.toff[pr] >> 3);
+    uint4 q4 = {0u, 0u, 0u, 0u};   // recompute walks: cached query words
+    uint32_t q4_key = 0xFFFFFFFFu;
     const int32_t nval = c.has_np ? -c.np : 0;
     uint32_t ch[8];                 // cached 32-byte chunk: 8 columns of one 8x8 tile
     int32_t ch_key = -1;
@@ -2122,9 +2138,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 chunk = d.wf_tb + (d.wf_tbo[pr] + (size_t)strip * QW + (size_t)wv) * 8;
             }
             if (ck || wck) {
-                const uint32_t gw = Tw[strip];
                 if (strip != lut_key) {   // the walk stays in one strip for several tiles
                     lut_key = strip;
+                    const uint32_t gw = Tw[strip];
 #pragma unroll
                     for (int m = 0; m < 8; m++) lut[m] = lut16[(gw >> (28 - 4 * m)) & 15u];
                 }
@@ -2145,7 +2161,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 #pragma unroll
                     for (int k = 0; k < 8; k++) top[k] = 0u;
                 }
-                ck_tile(top, left, corner, Qw[wv], lut, c, ch);
+                ck_tile(top, left, corner, seq_word4(d.pq, qw0 + (uint32_t)wv, q4, q4_key), lut, c, ch);
             } else if (chunk) {
                 tb_load(chunk, ch);
             } else {
