@@ -1069,6 +1069,7 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         // column tag: ~strip and ~column fields, pre-shifted by 3
         int32_t tagc = (int32_t)((((~(uint32_t)i) & d.enc_sm) << d.enc_cb | d.enc_cm) << 3);
         uint32_t nxt[8];
+        int32_t hb = 0;   // CK: H of the bottom row one column back (0 left of column 0)
         if (CK) {   // the previous strip's bottom row; zero above strip 0
             if (i > 0) hv.load_nt(i - 1, 0, nxt);
             else {
@@ -1104,9 +1105,16 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
                 // 4 * (query nibble & 7): the nibble of column kk sits at bits 31-4kk .. 28-4kk
                 const uint32_t qsh = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
                 int32_t e = (int32_t)cur[kk] >> 16;
-                // p walks down the previous column: H(row-1, col-1) for each row
-                int32_t p = h[0];
-                h[0] = sx16(cur[kk]);
+                // p walks down the previous column: H(row-1, col-1) for each row.
+                // Checkpoint rows hold H one column behind E, so entry kk
+                // already is H(row-1, col-1).
+                int32_t p;
+                if (CK) {
+                    p = sx16(cur[kk]);
+                } else {
+                    p = h[0];
+                    h[0] = sx16(cur[kk]);
+                }
                 uint32_t dir = 0;
                 int32_t colbest = 0, colsec = 0;
 #pragma unroll
@@ -1134,8 +1142,12 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
                     p = h[m];
                     h[m] = hm;
                 }
-                if (CK) dirw[kk] = pk16(h[8], e);
-                else bd[ridx * WAVE] = pk16(h[8], e);
+                if (CK) {
+                    dirw[kk] = pk16(hb, e);   // H(bottom, col - 1), E(bottom, col)
+                    hb = h[8];
+                } else {
+                    bd[ridx * WAVE] = pk16(h[8], e);
+                }
                 best = max(best, colbest | tagc);
                 if (SECOND) sbest = max(sbest, colsec | tagc);
                 tagc -= 8;
@@ -1993,19 +2005,19 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
 // --------------------------------------------------------------- get_tb -----
 // Recomputes one 8x8 tile's direction words (k_local_fast's sign-bit format)
 // from the checkpoints k_local_fast<.., CK> stored: the previous strip's
-// bottom row above the tile, its last entry left of it (the corner), and the
-// tile's left column state.  Same recurrences and score tables (lut: the
-// strip's eight rows, as k_local_fast builds them), so the same words.
-// top: the row above (zero above strip 0), left: the left column state,
-// corner: H above-left of the tile; lut: the strip's score tables (the kernels
-// store checkpoints only when the batch's nibble census allows tables).
+// bottom row above the tile and the tile's left column state.  Same
+// recurrences and score tables (lut: the strip's eight rows, as k_local_fast
+// builds them), so the same words.
+// top: the row above, entry kk = (H at column kk - 1, E at column kk), so the
+// entry left of the tile's first column (the corner) comes with it; zero above
+// strip 0.  left: the left column state; lut: the strip's score tables (the
+// kernels store checkpoints only when the batch's nibble census allows tables).
 // The words hold the reference's nibbles (src/kernels/local_kernel_template.h
 // :45-60): [3] F continues (t <= f - extend), [2] E continues, [1:0] source:
 // diagonal 0 / 1 by the score's sign, F 3, E 2, ties in that order.
-__device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *left, int32_t corner, uint32_t qword,
+__device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *left, uint32_t qword,
                                         const uint32_t *lut, const G2Scores &c, uint32_t *ch) {
     int32_t h[9], f[9];
-    h[0] = corner;
 #pragma unroll
     for (int m = 1; m < 9; m++) { h[m] = sx16(left[m - 1]); f[m] = (int32_t)left[m - 1] >> 16; }
     const int32_t gapoe = c.gapoe, gape = c.gape;
@@ -2013,8 +2025,7 @@ __device__ __forceinline__ void ck_tile(const uint32_t *top, const uint32_t *lef
     for (int kk = 0; kk < 8; kk++) {
         const uint32_t qsh = kk < 7 ? (qword >> (26 - 4 * kk)) & 0x1Cu : (qword << 2) & 0x1Cu;
         int32_t e = (int32_t)top[kk] >> 16;
-        int32_t p = h[0];
-        h[0] = sx16(top[kk]);
+        int32_t p = sx16(top[kk]);
         uint32_t dir = 0;
 #pragma unroll
         for (int m = 1; m < 9; m++) {
@@ -2145,23 +2156,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                     for (int m = 0; m < 8; m++) lut[m] = lut16[(gw >> (28 - 4 * m)) & 15u];
                 }
                 uint32_t top[8], left[8];
-                int32_t corner = 0;
                 if (ck) {
                     const TbView lv = tb_view(d, grp, lane), hv = tb_view(d, grp, lane, d.tbh);
                     if (strip > 0) hv.load(strip - 1, wv, top);
                     lv.load(strip, wv, left);
-                    if (strip > 0 && wv > 0) corner = sx16(hv.at(strip - 1, wv - 1, 7));
-                } else {   // per pair [strip][word] x 8
+                } else {   // per pair [strip][word] x 8, H and E of the same column
                     const uint32_t *lp = d.wf_tb + d.wf_tbo[pr] * 8, *hp = d.wf_tbh + d.wf_tbo[pr] * 8;
-                    if (strip > 0) tb_load(hp + ((size_t)(strip - 1) * QW + (size_t)wv) * 8, top);
+                    if (strip > 0) {
+                        uint32_t raw[8];
+                        tb_load(hp + ((size_t)(strip - 1) * QW + (size_t)wv) * 8, raw);
+                        const uint32_t corner = wv > 0 ? hp[((size_t)(strip - 1) * QW + (size_t)(wv - 1)) * 8 + 7] : 0u;
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            top[k] = (raw[k] & 0xFFFF0000u) | ((k == 0 ? corner : raw[k - 1]) & 0xFFFFu);
+                    }
                     tb_load(lp + ((size_t)strip * QW + (size_t)wv) * 8, left);
-                    if (strip > 0 && wv > 0) corner = sx16(hp[((size_t)(strip - 1) * QW + (size_t)(wv - 1)) * 8 + 7]);
                 }
                 if (strip == 0) {
 #pragma unroll
                     for (int k = 0; k < 8; k++) top[k] = 0u;
                 }
-                ck_tile(top, left, corner, seq_word4(d.pq, qw0 + (uint32_t)wv, q4, q4_key), lut, c, ch);
+                ck_tile(top, left, seq_word4(d.pq, qw0 + (uint32_t)wv, q4, q4_key), lut, c, ch);
             } else if (chunk) {
                 tb_load(chunk, ch);
             } else {
