@@ -230,7 +230,7 @@ __device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
 // reference's sequential loop.  The last word and the first RC_U word pairs
 // of each thread are loaded together (every index stays inside the pair's
 // words), so a pair costs one memory round trip after its metadata.
-constexpr uint32_t RC_T = 8, RC_U = 4;
+constexpr uint32_t RC_T = 4, RC_U = 4;
 __device__ __forceinline__ void rc_side(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code, uint32_t t) {
     const uint32_t regs = (len >> 3) + ((len & 7) ? 1 : 0);
     if (regs == 0 || op == 0) return;
@@ -627,26 +627,38 @@ __device__ __forceinline__ uint32_t score_lut(const G2Scores &c, int32_t nval, u
     return lut;
 }
 
+__device__ __forceinline__ uint32_t nib_set(uint32_t w) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 32; k += 4) m |= 1u << ((w >> k) & 15u);
+    return m;
+}
+
+// Bit v of aux[0] is set when some packed nibble equals v.  The packed arrays
+// are 256-byte aligned, so their whole 16-byte groups are read as uint4 (more
+// bytes in flight per wave); the last nq % 4 and nt % 4 words one by one.
 __global__ __launch_bounds__(256) void k_census(const uint32_t *__restrict__ pq, uint32_t nq, const uint32_t *__restrict__ pt,
                                                uint32_t nt, uint32_t *__restrict__ out) {
     uint32_t m = 0;
-    const uint32_t stride = gridDim.x * blockDim.x, n = nq + nt;
-    constexpr uint32_t U = 4;   // independent loads in flight per thread
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += U * stride) {
-        uint32_t w[U];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+    const uint32_t nq4 = nq >> 2, n4 = nq4 + (nt >> 2);
+    const uint4 *q4 = reinterpret_cast<const uint4 *>(pq), *t4 = reinterpret_cast<const uint4 *>(pt);
+    constexpr uint32_t U = 2;   // independent 16-byte loads in flight per thread
+    for (uint32_t i = gtid; i < n4; i += U * stride) {
+        uint4 v[U];
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
             const uint32_t j = i + u * stride;
-            w[u] = j < nq ? pq[j] : (j < n ? pt[j - nq] : 0u);
+            v[u] = j < nq4 ? q4[j] : (j < n4 ? t4[j - nq4] : uint4{0u, 0u, 0u, 0u});
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; u++) {
-            uint32_t mw = 0;
-#pragma unroll
-            for (int k = 0; k < 32; k += 4) mw |= 1u << ((w[u] >> k) & 15u);
-            m |= i + u * stride < n ? mw : 0u;
+            const uint32_t mw = nib_set(v[u].x) | nib_set(v[u].y) | nib_set(v[u].z) | nib_set(v[u].w);
+            m |= i + u * stride < n4 ? mw : 0u;
         }
     }
+    if (gtid < (nq & 3u)) m |= nib_set(pq[(nq & ~3u) + gtid]);
+    if (gtid < (nt & 3u)) m |= nib_set(pt[(nt & ~3u) + gtid]);
     // wave reduction then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
     if ((threadIdx.x & 63) == 0 && m) atomicOr(out, m);
@@ -2401,7 +2413,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
         const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
         const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
         uint32_t cb = (qw + tw + 255) / 256;
-        k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);
+        k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);   // threads 0..2 take the tail words
     }
     G2_LAUNCH_CHECK();
     // 3. order pairs by shape (largest first) so a wave's lanes share loop bounds;
