@@ -104,6 +104,10 @@ def main():
     ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1, 2],
                     help="LOCAL WITH_TB: checkpointed traceback on the lane path (1, default), also for one-wave "
                          "pairs (2), or direction words stored by the DP (0)")
+    ap.add_argument("--tb-fused", type=int, default=0, choices=[0, 1],
+                    help="checkpointed LOCAL WITH_TB: the separate traceback kernel walks every pair (0, default) "
+                         "or the DP kernel walks each lane-path pair's traceback (1; the DP kernel time then "
+                         "includes those walks)")
     ap.add_argument("--ksw-ext", type=int, nargs=2, default=None, metavar=("W", "ZDROP"),
                     help="KSW band half-width and z-drop (seed extension as bwa; default: the reference's none, 0)")
     ap.add_argument("--cpu-pairs", type=int, default=-1,
@@ -183,6 +187,7 @@ def main():
     L = g.lib()
     g.set_ksw_extension(kw, kz)
     L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
+    L.gasal2_set_tb_fused(args.tb_fused)
     if args.wf:
         L.gasal2_set_wavefront(*args.wf)
     tuned = None
@@ -316,7 +321,7 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "mode": mode, "tb_checkpoint": args.tb_checkpoint, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
+                   "mode": mode, "tb_checkpoint": args.tb_checkpoint, "tb_fused": args.tb_fused, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
                    "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
         "roofline": ({"bound": "valu",
                       "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",

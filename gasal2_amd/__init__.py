@@ -173,6 +173,7 @@ _SIGS = {
     "gasal2_set_pk16": (None, [C.c_int32]),
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
     "gasal2_set_tb_checkpoint": (None, [C.c_int32]),
+    "gasal2_set_tb_fused": (None, [C.c_int32]),
     "gasal2_set_panels": (None, [C.c_int32]),
     "gasal2_last_paths": (C.c_int32, [C.POINTER(gasal_gpu_storage_t)]),
     "gasal2_tune_set": (None, [C.c_int32, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),

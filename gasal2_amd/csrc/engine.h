@@ -59,6 +59,7 @@ struct G2Dev {
     // column state (H, F) and tbh each strip's bottom row (H, E), from which
     // the traceback recomputes the tiles on its path
     uint32_t *tbh;
+    uint32_t tb_fused;               // with tbh: the DP kernel walks each lane-path pair's traceback itself
     // transposed sequence panels of the lane path ([group][word][lane]; null: off)
     uint32_t *qpan, *tpan;
     uint32_t qpan_w, tpan_w;

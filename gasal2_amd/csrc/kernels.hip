@@ -1017,6 +1017,9 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
     if (with_start) local_reverse_start(d, c, x, census, bv, bv > 0 ? bc : 0, bv > 0 ? bs * 8 + br : 0);
 }
 
+template <bool LOCALT>
+__device__ void tb_walk(const G2Dev &d, const G2Scores &c, uint32_t slot, const uint32_t *lut16, bool fused);
+
 // CK (WITH_TB only): checkpointed traceback.  No direction words: per 8x8
 // tile the left column state (H, F of its eight rows, 16 bits each) goes to
 // the tile region and each strip's bottom row (H, E) to a second region of
@@ -1029,6 +1032,14 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
 template <int START, bool SECOND, bool CK = false, bool WF = true>
 __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
     const uint32_t census = d.aux[0];
+    // CK with tb_fused (opt-in): each lane walks its pair's traceback after its
+    // DP (the walk's memory waits overlap other waves' DP); score tables of
+    // the 16 target nibbles for the walk's tile recompute
+    __shared__ uint32_t lut16[16];
+    if (CK && d.tb_fused) {
+        if (threadIdx.x < 16) lut16[threadIdx.x] = score_lut(c, c.has_np ? -c.np : 0, census, threadIdx.x);
+        __syncthreads();
+    }
     if (WF && blockIdx.x < d.wf_blocks) {
         if (START == S_WITHOUT_START && !SECOND) {
             if (census_fast(census)) wf_local<false, true>(d, c, census);
@@ -1192,6 +1203,10 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         d.s_qend[pr] = vbest > 0 ? 8 * QW - 1 : 0;   // compares with the best (src/kernels/local_kernel_template.h:162)
     }
     if (START == S_WITH_START) local_reverse_start(d, c, x, census, vbest, maxXY_x, maxXY_y);
+    if (CK && d.tb_fused) {
+        __threadfence();   // this lane's checkpoints and results, seen by its own loads
+        tb_walk<true>(d, c, x.slot, lut16, true);
+    }
 }
 
 // ----------------------------------------------------- LOCAL, diagonal band ----
@@ -2077,16 +2092,12 @@ __device__ __forceinline__ uint32_t seq_word4(const uint32_t *arr, uint32_t a, u
 // gasal_get_tb<T> (src/kernels/get_tb.h:4-149).  Reads the direction words the
 // DP kernel wrote for this pair's wave slot; cells outside the pair's computed
 // matrix read as 0.  Writes the pair's CIGAR into its own slot.
+// One pair's walk (the body of k_get_tb), for the pair in sorted slot `slot`.
+// fused: called by k_local_fast<.., CK> right after the pair's DP, which walks
+// exactly the pairs whose tiles it checkpointed; k_get_tb then skips those.
 template <bool LOCALT>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_get_tb(G2Dev d, G2Scores c) {
-    // score table of each target nibble for the tile recompute (one per block,
-    // built before any thread leaves)
-    __shared__ uint32_t lut16[16];
-    if (LOCALT) {
-        if (threadIdx.x < 16) lut16[threadIdx.x] = score_lut(c, c.has_np ? -c.np : 0, d.aux[0], threadIdx.x);
-        __syncthreads();
-    }
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void tb_walk(const G2Dev &d, const G2Scores &c, uint32_t slot, const uint32_t *lut16,
+                                        bool fused) {
     if (slot >= d.n) return;
     const uint32_t pr = d.perm[slot];
     // long pairs (sorted first) ran one wave per pair with per-pair tiles; the
@@ -2113,6 +2124,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     const bool ck = LOCALT && !band && !wfp && lanefast && d.tbh != nullptr;
     // long pairs of k_local_fast's one-wave blocks with checkpoints (wf_local<.., CK>)
     const bool wck = LOCALT && wfp && d.wf_tbh != nullptr && census_fast(census);
+    if (fused ? !ck : (ck && d.tb_fused)) return;   // walked by k_local_fast, or not fused there
     // recomputed tiles hold the reference's nibbles; stored sign-bit words are decoded
     const bool decode = fastfmt && !ck && !wck;
     const uint32_t qw0 = d.qoff[pr] >> 3;
@@ -2285,6 +2297,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     if (LOCALT) { d.tstart[pr] = i; d.qstart[pr] = j; }
     d.n_ops[pr] = (uint32_t)n_ops;
     if (band && bflag) d.band_flags[pr] |= (uint8_t)bflag;
+}
+
+template <bool LOCALT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_get_tb(G2Dev d, G2Scores c) {
+    // score table of each target nibble for the tile recompute (one per block,
+    // built before any thread leaves)
+    __shared__ uint32_t lut16[16];
+    if (LOCALT) {
+        if (threadIdx.x < 16) lut16[threadIdx.x] = score_lut(c, c.has_np ? -c.np : 0, d.aux[0], threadIdx.x);
+        __syncthreads();
+    }
+    tb_walk<LOCALT>(d, c, blockIdx.x * blockDim.x + threadIdx.x, lut16, false);
 }
 
 // Copies each pair's CIGAR to the reference's layout (bytes at the pair's query

@@ -39,6 +39,7 @@ constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all groups up to this many bytes
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
+int g_tb_fused = 0;  // checkpointed lane path: the DP kernel walks each pair's traceback itself
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
 int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
@@ -197,6 +198,7 @@ extern "C" int32_t gasal2_tune_count(void) {
 
 extern "C" void gasal2_set_tile_budget(uint64_t bytes) { g_tb_one_size_max = bytes; }
 extern "C" void gasal2_set_tb_checkpoint(int32_t mode) { g_tb_ckpt = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
+extern "C" void gasal2_set_tb_fused(int32_t on) { g_tb_fused = on ? 1 : 0; }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" int32_t gasal2_last_paths(gasal_gpu_storage_t *s) { return s && s->impl ? impl_of(s)->last_paths : 0; }
 extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
@@ -1241,6 +1243,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         d.tb = m->tb;
         d.tb_tw = et;
         d.tb_qw8 = 8 * eq;
+        d.tb_fused = d.tbh && g_tb_fused ? 1u : 0u;
         s->packed_tb_matrices = (uint4 *)m->tb;
         s->packed_tb_matrix_size = m->tb_bytes / 16;
     }

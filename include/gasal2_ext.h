@@ -173,6 +173,17 @@ void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
  * recompute cost about what the DP saves). */
 void gasal2_set_tb_checkpoint(int32_t mode);
 
+/* Fused traceback for checkpointed lane-path pairs (default off; no
+ * counterpart in the reference, whose gasal_get_tb runs after the DP kernel,
+ * src/gasal_align.h:50-53).  With checkpoints (mode 1 or 2 above), the DP
+ * kernel walks each lane-path pair's traceback right after the pair's DP, so
+ * the walks' memory waits overlap the other waves' DP; the separate traceback
+ * kernel then walks only the remaining pairs.  0 walks every pair in the
+ * separate kernel.  Results are identical either way.  On 1M full-feature
+ * pairs it saves about 1% of the step; on batches with one-wave pairs it
+ * costs about as much, hence off by default. */
+void gasal2_set_tb_fused(int32_t on);
+
 /* Transposed sequence panels (default on; no counterpart in the reference).
  * After the shape sort, the fast LOCAL one-lane-per-pair kernels read their
  * packed query and target words from per-group panels laid out

@@ -241,3 +241,22 @@ def test_pair_order_many_shapes(start, second, n_code):
     b = _many_shapes_batch(9000, 77, n_code)
     got, ref = run_both(b, g.LOCAL, start, second, is_rc=True, n_code=n_code)
     assert_parity(got, ref, g.LOCAL, start, second, b)
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("second", [False, True])
+def test_local_tb_fused_walk(fused, second):
+    """Checkpointed LOCAL WITH_TB with the traceback walked inside the DP kernel
+    (default) or by the separate traceback kernel: identical CIGARs and starts,
+    also on the many-shapes batch with one-wave pairs beside the lane path."""
+    L = g.lib()
+    L.gasal2_set_tb_fused(fused)
+    try:
+        b = g.synth_batch(2, 1212, 700)
+        got, ref = run_both(b, g.LOCAL, g.WITH_TB, second, is_rc=True, n_penalty=1)
+        assert_parity(got, ref, g.LOCAL, g.WITH_TB, second, b)
+        b = _many_shapes_batch(2000, 78)
+        got, ref = run_both(b, g.LOCAL, g.WITH_TB, second, is_rc=True)
+        assert_parity(got, ref, g.LOCAL, g.WITH_TB, second, b)
+    finally:
+        L.gasal2_set_tb_fused(0)
