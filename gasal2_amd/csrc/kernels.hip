@@ -27,7 +27,10 @@ enum { S_WITHOUT_START = 0, S_WITH_START = 1, S_WITH_TB = 2 };
 enum { D_NONE = 0, D_QUERY = 1, D_TARGET = 2, D_BOTH = 3 };
 
 __device__ __forceinline__ int32_t sx16(uint32_t v) { return (int32_t)(int16_t)(uint16_t)(v & 0xFFFFu); }
-__device__ __forceinline__ uint32_t pk16(int32_t h, int32_t e) { return ((uint32_t)h & 0xFFFFu) | ((uint32_t)e << 16); }
+// (h & 0xFFFF) | (e << 16): one byte select (bytes 0-1 of h, then bytes 0-1 of e)
+__device__ __forceinline__ uint32_t pk16(int32_t h, int32_t e) {
+    return __builtin_amdgcn_perm((uint32_t)e, (uint32_t)h, 0x05040100u);
+}
 __device__ __forceinline__ int32_t t16(int32_t v) { return (int32_t)(int16_t)(uint16_t)((uint32_t)v & 0xFFFFu); }
 __device__ __forceinline__ int32_t dregs(uint32_t len) { return (int32_t)((len >> 3) + ((len & 7) ? 1 : 0)); }
 
