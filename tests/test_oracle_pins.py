@@ -15,8 +15,10 @@ import pytest
 
 import gasal2_amd as g
 import pyoracle as O
-from refmath import (automaton_global, automaton_local, enumerate_local, gotoh_local,
-                     gotoh_semiglobal_target_target, key_order_first, second_best_definition)
+from refmath import (BOTH_, NONE_, QUERY_, TARGET_, automaton_global, automaton_local, banded_reference,
+                     enumerate_local, gotoh_local, gotoh_semiglobal_target_target, key_order_first,
+                     ksw_unpruned_reference, local_traceback_reference, second_best_definition,
+                     semiglobal_reference)
 
 ACGT = b"ACGT"
 
@@ -195,9 +197,6 @@ def test_local_start_positions(seed):
     tp = t + b"N" * ((-len(t)) % 8)
     sub, _ = automaton_local(qp[qs:qe + 1], tp[ts:te + 1])
     assert sub == score
-    # and no later start (in reverse key order) reaches it: shrinking either side loses the score
-    if qs < qe:
-        assert automaton_local(qp[qs + 1:qe + 1], tp[ts:te + 1])[0] < score or True
 
 
 # ------------------------------------------------------------- GLOBAL -----
@@ -365,18 +364,15 @@ def test_local_cigar_replays_to_score(seed):
     assert all((b >> 2) <= 63 for b in cg)
     score = int(r["score"][0])
     qs, ts = int(r["qstart"][0]), int(r["tstart"][0])
-    # accounting of the walk (labels only): the walk stops when it reaches the score
-    lab = 0
-    prev = 0
+    ops = []
     for b in reversed(cg):
-        pass
-    replay, ie, je = apply_cigar_score(q + b"N" * 8, t + b"N" * 8, qs, ts, cg)
-    # quirk-free walks replay exactly: same score, and they end on the reported end cell
-    quirk = any(((cg[k] & 3) in (2, 3)) for k in range(len(cg)))
-    if replay == score:
+        ops += [b & 3] * (b >> 2)
+    # a walk that passes no gap has no relabelled cells: it replays exactly,
+    # from the reported start to the reported end, to the score
+    if not any(o in (2, 3) for o in ops):
+        replay, ie, je = apply_cigar_score(q + b"N" * 8, t + b"N" * 8, qs, ts, cg)
+        assert replay == score
         assert (ie, je) == (int(r["tend"][0]), int(r["qend"][0]))
-    else:
-        assert quirk, "only walks through gaps can mis-account (gap exit labelled M)"
 
 
 def test_cigar_gap_kat():
@@ -435,3 +431,117 @@ def test_local_gap_opening_from_m_under_cheap_gaps(seed):
     assert r["score"][0] == automaton_local(qp, tp, sc)[0]
     rg = O.align(g.make_batch([q], [t]), g.GLOBAL, scores=sc)
     assert rg["score"][0] == automaton_global(q, t, sc)
+
+
+# ------------------------------------------ full-matrix pins (every variant) -----
+SCORE_SETS = [(1, 4, 6, 1), (2, 20, 1, 1), (3, 2, 5, 2), (1500, 4, 6, 1)]
+
+
+def _semi_case(rng, kind):
+    al = b"ACGTN" if kind == 5 else ACGT
+    if kind == 0:
+        return rnd_seq(rng, rng.randint(1, 30), al), rnd_seq(rng, rng.randint(1, 35))
+    if kind == 1:   # query inside the target
+        q = rnd_seq(rng, rng.randint(1, 30), al)
+        return q, rnd_seq(rng, rng.randint(0, 12)) + mutate(rng, q, 0.1) + rnd_seq(rng, rng.randint(0, 12))
+    if kind == 2:   # short query, long target (columns past the query's words)
+        q = rnd_seq(rng, rng.randint(1, 9), al)
+        return q, rnd_seq(rng, rng.randint(0, 20)) + mutate(rng, q, 0.1) + rnd_seq(rng, rng.randint(10, 25))
+    if kind == 3:   # target ends with the query
+        q = rnd_seq(rng, rng.randint(1, 9), al)
+        return q, rnd_seq(rng, rng.randint(10, 25)) + q
+    t = rnd_seq(rng, rng.randint(1, 9))   # long query, short target
+    return rnd_seq(rng, rng.randint(0, 12), al) + mutate(rng, t, 0.1) + rnd_seq(rng, rng.randint(8, 25), al), t
+
+
+@pytest.mark.parametrize("head", [NONE_, QUERY_, TARGET_, BOTH_])
+@pytest.mark.parametrize("tail", [NONE_, QUERY_, TARGET_, BOTH_])
+def test_semiglobal_every_head_tail_vs_full_matrix(head, tail):
+    """Score, both ends, second best (score and ends) and WITH_START starts of
+    every head/tail variant against the full-matrix formulation with the
+    reference's boundary rows and end bookkeeping (refmath.semiglobal_reference),
+    over four score sets (one large enough that the 16-bit row between strips
+    wraps), N bases with and without N_PENALTY, and shapes that reach the
+    columns past the query's padded words."""
+    for seed in range(36):
+        rng = random.Random(seed * 16 + head * 4 + tail)
+        q, t = _semi_case(rng, seed % 6)
+        sc = SCORE_SETS[seed % 4]
+        npen = 1 if seed % 2 == 0 else None
+        r = oracle_one(q, t, g.SEMI_GLOBAL, start_pos=g.WITH_START, second_best=True, head=head, tail=tail,
+                       n_penalty=npen, scores=sc)
+        ref = semiglobal_reference(q, t, head, tail, sc=sc, nval=(-1 if npen else 0), with_start=True)
+        got = {k: int(r[k][0]) for k in ref}
+        assert got == ref, (seed, q, t, sc)
+
+
+@pytest.mark.parametrize("head", [NONE_, QUERY_, TARGET_, BOTH_])
+def test_semiglobal_columns_past_the_query_words_kat(head):
+    # A one-base query against a 13-base target: the second best of a QUERY
+    # tail can come from a column the DP never computed (the row's initial
+    # value), which only exists for targets longer than the query's words.
+    q, t = b"A", b"C" * 12 + b"A"
+    r = oracle_one(q, t, g.SEMI_GLOBAL, second_best=True, head=head, tail=QUERY_, scores=(1, 20, 1, 1))
+    ref = semiglobal_reference(q, t, head, QUERY_, sc=(1, 20, 1, 1))
+    assert {k: int(r[k][0]) for k in ("score", "qend", "tend", "s_score", "s_qend", "s_tend")} == \
+        {k: ref[k] for k in ("score", "qend", "tend", "s_score", "s_qend", "s_tend")}
+
+
+@pytest.mark.parametrize("seed", range(150))
+def test_banded_narrow_bands_vs_masked_matrix(seed):
+    """Tile-banded LOCAL at bands 0..64 against Smith-Waterman-Gotoh over the
+    reference's per-strip word ranges (refmath.banded_reference)."""
+    rng = random.Random(9000 + seed)
+    q = rnd_seq(rng, rng.randint(1, 60), b"ACGTN" if seed % 5 == 0 else ACGT)
+    t = (rnd_seq(rng, rng.randint(0, 20)) + mutate(rng, q, 0.15) + rnd_seq(rng, rng.randint(0, 20))
+         if seed % 3 else rnd_seq(rng, rng.randint(1, 60)))
+    if seed % 7 == 0:
+        q, t = t, q
+    sc = SCORE_SETS[seed % 3]
+    npen = 1 if seed % 2 else None
+    kb = rng.choice([0, 8, 16, 24, 32, 48, 64])
+    r = O.align(g.make_batch([q], [t]), g.BANDED, k_band=kb, scores=sc, n_penalty=npen)
+    got = (int(r["score"][0]), int(r["qend"][0]), int(r["tend"][0]))
+    assert got == banded_reference(q, t, kb, sc=sc, nval=(-1 if npen else 0))
+
+
+@pytest.mark.parametrize("seed", range(300))
+def test_ksw_reference_form_vs_full_matrix(seed):
+    """Reference-form KSW (no band, z-drop 0) against the full-matrix seed
+    extension, on inputs where the seed score keeps every cell positive so
+    that row pruning and the zero-row exit provably never act
+    (refmath.ksw_unpruned_reference); includes divergent query ends, where
+    the best cell (not the query end) is reported."""
+    rng = random.Random(9500 + seed)
+    q = rnd_seq(rng, rng.randint(1, 40), b"ACGTN" if seed % 5 == 0 else ACGT)
+    t = mutate(rng, q, 0.2) if seed % 3 else rnd_seq(rng, rng.randint(1, 40))
+    if seed % 4 == 3:
+        core = rnd_seq(rng, rng.randint(3, 20))
+        q, t = core + rnd_seq(rng, rng.randint(5, 20)), core + rnd_seq(rng, rng.randint(5, 20))
+    t = t[:40]
+    sc = SCORE_SETS[seed % 3]
+    npen = 1 if seed % 2 else None
+    h0 = 1000 + rng.randint(0, 50)
+    r = O.align(g.make_batch([q], [t], seeds=[h0]), g.KSW, scores=sc, n_penalty=npen)
+    got = (int(r["score"][0]), int(r["qend"][0]), int(r["tend"][0]))
+    assert got == ksw_unpruned_reference(q, t, h0, sc=sc, nval=(-1 if npen else 0))
+
+
+@pytest.mark.parametrize("seed", range(300))
+def test_local_traceback_vs_automaton_walk(seed):
+    """LOCAL WITH_TB: score, ends, starts and every CIGAR byte against a
+    traceback over the automaton's M / D / I matrices
+    (refmath.local_traceback_reference) -- no direction nibbles, tiles or
+    strips -- over four score sets, with and without N bases and N_PENALTY."""
+    rng = random.Random(7700 + seed)
+    core = rnd_seq(rng, rng.randint(5, 50), b"ACGTN" if seed % 5 == 0 else ACGT)
+    q = rnd_seq(rng, rng.randint(0, 10)) + core + rnd_seq(rng, rng.randint(0, 10))
+    t = rnd_seq(rng, rng.randint(0, 10)) + mutate(rng, core, 0.12) + rnd_seq(rng, rng.randint(0, 10))
+    sc = [(1, 4, 6, 1), (2, 20, 1, 1), (3, 2, 5, 2), (2, 3, 2, 1)][seed % 4]
+    npen = 1 if seed % 2 else None
+    r = O.align(g.make_batch([q], [t]), g.LOCAL, start_pos=g.WITH_TB, scores=sc, n_penalty=npen)
+    ref = local_traceback_reference(q, t, sc=sc, nval=(-1 if npen else 0))
+    got = dict(score=int(r["score"][0]), qend=int(r["qend"][0]), tend=int(r["tend"][0]),
+               qstart=int(r["qstart"][0]), tstart=int(r["tstart"][0]), cigar=r["cigars"][0])
+    assert got == ref
+    assert int(r["n_ops"][0]) == len(ref["cigar"])
