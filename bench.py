@@ -114,6 +114,9 @@ def main():
                     help="oracle sample for cpu_baseline: -1 = sized to about --cpu-seconds, 0 = skip")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES bytes and exit: the reference kernel whose "
+                         "FETCH_SIZE / WRITE_SIZE counters (rocprofv3 --pmc) calibrate the DP kernels' byte counts")
     ap.add_argument("--traffic", default=None,
                     help="profile summary with a 'bytes_per_launch: N' line (measured HBM bytes per DP launch); "
                          "default: the committed one for the default workload and mode")
@@ -123,6 +126,17 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); pairs are independent and would be "
               "sharded across GPUs with no exchange -- see DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(args.copy_calibration, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration}), flush=True)
+        return
 
     import gasal2_amd as g
 

@@ -370,6 +370,21 @@ class Batch:
         return bytes(self.t[self.toff[i]:self.toff[i] + self.tlen[i]])
 
 
+def pack_bases(buf):
+    """Host form of the 4-bit packed words an isPacked=true caller supplies:
+    nibble = byte & 0xF, 8 bases per uint32, first base in bits 31:28
+    (src/kernels/pack_rc_seqs.h:24-31).  `buf` holds whole 8-base words
+    (sequences padded with N_CODE); returns their bytes as stored in memory."""
+    b = np.ascontiguousarray(buf, dtype=np.uint8)
+    if len(b) % 8:
+        raise ValueError("packing needs whole 8-base words")
+    nib = (b.reshape(-1, 8) & 0xF).astype(np.uint32)
+    w = np.zeros(len(nib), np.uint32)
+    for i in range(8):
+        w |= nib[:, i] << np.uint32(28 - 4 * i)
+    return w.view(np.uint8)
+
+
 def make_batch(queries, targets, qop=None, top=None, seeds=None, n_code=N_CODE_DEFAULT):
     def lay(seqs):
         offs, lens, chunks, o = [], [], [], 0
@@ -442,11 +457,14 @@ class Aligner:
                  n_code=N_CODE_DEFAULT, n_penalty=None):
         L = lib()
         self.params = params
+        self.n_code = n_code
         set_n_code(n_code)
         set_n_penalty(n_penalty)
         gasal_copy_subst_scores(gasal_subst_scores(*scores))
         self.v = gasal_init_gpu_storage_v(1)
-        gasal_init_streams(self.v, max_query_len + 7, max_target_len + 7, max_n_alns, params)
+        # packed per-sequence fills round each sequence to 16 bases (two words)
+        slack = 15 if params.isPacked else 7
+        gasal_init_streams(self.v, max_query_len + slack, max_target_len + slack, max_n_alns, params)
         self.s = self.v.a[0]
         self._L = L
 
@@ -467,6 +485,8 @@ class Aligner:
         n = batch.n
         if n > s.host_max_n_alns:
             gasal_host_alns_resize(s, n, self.params)
+        if self.params.isPacked:
+            return self._fill_packed(batch, per_pair)
         if per_pair:
             qi = ti = 0
             for i in range(n):
@@ -487,6 +507,43 @@ class Aligner:
                 raise ValueError("batch larger than the storage's first host page; use per_pair=True")
             C.memmove(page.data, buf.ctypes.data, len(buf))
             page.data_size = len(buf)
+            C.memmove(dst_off, offs.ctypes.data, 4 * n)
+            C.memmove(dst_len, lens.ctypes.data, 4 * n)
+        return len(batch.q), len(batch.t)
+
+    def _fill_packed(self, batch, per_pair):
+        """isPacked=true: the host pages carry packed words (4 bytes per 8
+        bases); offsets, lengths and the returned byte counts are in bases."""
+        s = self.s
+        n = batch.n
+        nc = self.n_code
+        if per_pair:
+            # each sequence packed on its own; gasal_host_batch_fill pads the
+            # packed bytes to a multiple of 8 (a word of N_CODE bytes after
+            # odd-word sequences), so every offset stays a multiple of 8 bases
+            qi = ti = 0
+            for i in range(n):
+                s.host_query_batch_offsets[i] = 2 * qi
+                s.host_target_batch_offsets[i] = 2 * ti
+                for src, seq in ((data_source.QUERY, batch.query(i)), (data_source.TARGET, batch.target(i))):
+                    words = pack_bases(seq + bytes([nc]) * ((-len(seq)) % 8)).tobytes()
+                    if src == data_source.QUERY:
+                        qi = gasal_host_batch_fill(s, qi, words, src)
+                    else:
+                        ti = gasal_host_batch_fill(s, ti, words, src)
+                s.host_query_batch_lens[i] = int(batch.qlen[i])
+                s.host_target_batch_lens[i] = int(batch.tlen[i])
+            return 2 * qi, 2 * ti
+        for src, buf, offs, lens, dst_off, dst_len in (
+                (data_source.QUERY, batch.q, batch.qoff, batch.qlen, s.host_query_batch_offsets, s.host_query_batch_lens),
+                (data_source.TARGET, batch.t, batch.toff, batch.tlen, s.host_target_batch_offsets, s.host_target_batch_lens)):
+            page = (s.extensible_host_unpacked_query_batch if src == data_source.QUERY
+                    else s.extensible_host_unpacked_target_batch).contents
+            words = pack_bases(buf)
+            if page.page_size < len(words):
+                raise ValueError("batch larger than the storage's first host page; use per_pair=True")
+            C.memmove(page.data, words.ctypes.data, len(words))
+            page.data_size = len(words)
             C.memmove(dst_off, offs.ctypes.data, 4 * n)
             C.memmove(dst_len, lens.ctypes.data, 4 * n)
         return len(batch.q), len(batch.t)

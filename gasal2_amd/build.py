@@ -4,6 +4,7 @@
                         (kernels.hip: the alignment pipeline; band.hip: learned band)
     libgasal2_synth.so  seeded synthetic pair generator (host C++/OpenMP)
     test_prog           the reference harness re-implemented on the API
+    c_host              a C99 host of the API (the public headers' C view)
 
 Run as ``python -m gasal2_amd.build``.  Every call recompiles.
 """
@@ -59,13 +60,26 @@ def build_test_prog():
     return out
 
 
+def build_c_host():
+    """c_host: the API's C view compiled by a C compiler (gcc -std=c99) and
+    linked against the library -- HIP's platform flags as `hipconfig
+    --cpp_config` gives them."""
+    out = os.path.join(HERE, "c_host")
+    _run(["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__",
+          "-I", "/opt/rocm/include", "-I", INC, os.path.join(CSRC, "c_host.c"), "-o", out,
+          "-L", HERE, "-lgasal2_amd", f"-Wl,-rpath,{HERE}"])
+    return out
+
+
 def build_all():
     build_synth()
     build_lib()
     build_test_prog()
+    build_c_host()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "synth": build_synth, "test_prog": build_test_prog}[w]()
+        {"all": build_all, "lib": build_lib, "synth": build_synth, "test_prog": build_test_prog,
+         "c_host": build_c_host}[w]()

@@ -2437,8 +2437,9 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     // aux[1..2] long-pair queue head and count
     k_zero_u32<<<1, 64, 0, s>>>(d.aux, 4);
     if (d.fast) {
-        const uint32_t qw = op.is_packed ? d.q_bytes / 4 : d.q_bytes / 8;
-        const uint32_t tw = op.is_packed ? d.t_bytes / 4 : d.t_bytes / 8;
+        // packed words of the batch: the byte counts count bases (8 per word) with or without isPacked
+        const uint32_t qw = d.q_bytes / 8;
+        const uint32_t tw = d.t_bytes / 8;
         uint32_t cb = (qw + tw + 255) / 256;
         k_census<<<cb > 2048 ? 2048 : (cb ? cb : 1), 256, 0, s>>>(d.pq, qw, d.pt, tw, d.aux);   // threads 0..2 take the tail words
     }

@@ -156,12 +156,21 @@ extern "C" void gasal2_fail(int code) {
         return;                          \
     } while (0)
 
+// A failed HIP call reports and leaves through gasal2_fail (exit in error
+// mode 0).  In error mode 1 it also raises t_failed, which the entry points
+// test between their stages so that nothing is launched or copied after a
+// failed allocation or copy.
+namespace {
+thread_local bool t_failed = false;
+}
+
 #define G2_CHECK(call)                                                                                  \
     do {                                                                                                \
         hipError_t e__ = (call);                                                                        \
         if (e__ != hipSuccess) {                                                                        \
             report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n",              \
                    hipGetErrorString(e__), (int)e__, __LINE__, __FILE__);                               \
+            t_failed = true;                                                                            \
             gasal2_fail(EXIT_FAILURE);                                                                  \
         }                                                                                               \
     } while (0)
@@ -896,66 +905,161 @@ bool validate(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n) {
     return true;
 }
 
-// Device-buffer growth of src/gasal_align.cu:70-145.
-void grow_device(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
-    if (s->gpu_max_query_batch_bytes < qb) {
-        uint32_t i = 2;
-        while (s->gpu_max_query_batch_bytes * i < qb) i++;
-        fprintf(stderr, "[GASAL WARNING:] actual_query_batch_bytes(%d) > Allocated GPU memory (gpu_max_query_batch_bytes=%d). Therefore, allocating %d bytes on GPU (gpu_max_query_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
-                qb, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes * i, s->gpu_max_query_batch_bytes * i);
-        s->gpu_max_query_batch_bytes *= i;
-        bool aliased = (void *)s->packed_query_batch == (void *)s->unpacked_query_batch;
-        if (s->unpacked_query_batch) G2_CHECK(hipFree(s->unpacked_query_batch));
-        if (!aliased && s->packed_query_batch) G2_CHECK(hipFree(s->packed_query_batch));
-        G2_CHECK(hipMalloc((void **)&(s->unpacked_query_batch), (size_t)s->gpu_max_query_batch_bytes + 16));
-        if (params->isPacked) s->packed_query_batch = (uint32_t *)s->unpacked_query_batch;
-        else G2_CHECK(hipMalloc((void **)&(s->packed_query_batch), (size_t)(s->gpu_max_query_batch_bytes / 8 + 4) * sizeof(uint32_t)));
-        if (params->start_pos == WITH_TB) {
-            fprintf(stderr, "[GASAL WARNING:] actual_query_batch_bytes(%d) > Allocated HOST memory for CIGAR (gpu_max_query_batch_bytes=%d). Therefore, allocating %d bytes on the host (gpu_max_query_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
-                    qb, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes, s->gpu_max_query_batch_bytes);
-            if (s->host_res->cigar != NULL) G2_CHECK(hipHostFree(s->host_res->cigar));
-            G2_CHECK(hipHostMalloc((void **)&(s->host_res->cigar), s->gpu_max_query_batch_bytes, hipHostMallocDefault));
+// Checks made where a batch goes to the device (gasal_aln_async,
+// gasal2_upload), beyond the reference's: the host pages fit the byte counts
+// (with isPacked the counts are in bases, 8 per packed word, and the pages
+// hold 4 bytes per 8 bases), and every pair's sequence lies inside its batch
+// at a multiple of 8 bases, so no kernel can read past the copied batch.
+bool validate_batch(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const Parameters *params) {
+    const bool packed = params && params->isPacked;
+    host_batch_t *lists[2] = {s->extensible_host_unpacked_query_batch, s->extensible_host_unpacked_target_batch};
+    const uint32_t counts[2] = {qb, tb};
+    const char *what[2] = {"query", "target"};
+    for (int k = 0; k < 2; k++) {
+        uint64_t ext = 0;
+        for (host_batch_t *c = lists[k]; c != NULL; c = c->next)
+            if (c->data_size) ext = std::max<uint64_t>(ext, (uint64_t)c->offset + c->data_size);
+        const uint64_t cap = packed ? (uint64_t)counts[k] / 2 : (uint64_t)counts[k];
+        if (ext > cap) {
+            if (packed)
+                report("[GASAL ERROR:] isPacked: the %s host batch holds %llu bytes of packed words (%llu bases) but actual_%s_batch_bytes is %u; with isPacked the batch byte counts count bases\n",
+                       what[k], (unsigned long long)ext, (unsigned long long)(2 * ext), what[k], counts[k]);
+            else
+                report("[GASAL ERROR:] the %s host batch holds %llu bytes, more than actual_%s_batch_bytes(%u)\n",
+                       what[k], (unsigned long long)ext, what[k], counts[k]);
+            gasal2_fail(EXIT_FAILURE);
+            return false;
         }
     }
-    if (s->gpu_max_target_batch_bytes < tb) {
+    const uint32_t *off[2] = {s->host_query_batch_offsets, s->host_target_batch_offsets};
+    const uint32_t *len[2] = {s->host_query_batch_lens, s->host_target_batch_lens};
+    for (int k = 0; k < 2; k++) {
+        if (!off[k] || !len[k]) continue;
+        const uint32_t *o = off[k], *l = len[k];
+        uint32_t bad_align = 0;
+        uint64_t worst = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            bad_align |= o[i] & 7u;
+            worst = std::max<uint64_t>(worst, (uint64_t)o[i] + (((uint64_t)l[i] + 7) & ~7ull));
+        }
+        if (!bad_align && worst <= counts[k]) continue;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint64_t end = (uint64_t)o[i] + (((uint64_t)l[i] + 7) & ~7ull);
+            if ((o[i] & 7u) || end > counts[k]) {
+                report("[GASAL ERROR:] pair %u: %s offset %u, length %u does not lie inside actual_%s_batch_bytes(%u) at a multiple of 8\n",
+                       i, what[k], o[i], l[i], what[k], counts[k]);
+                gasal2_fail(EXIT_FAILURE);
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
+// Device-buffer growth of src/gasal_align.cu:70-145.  The new buffers are
+// allocated before the old ones are released, so a failed allocation (error
+// mode 1) leaves the storage with its old buffers and recorded sizes.
+void hip_free_quiet(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+bool grow_device(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    struct Side {
+        uint32_t *max;
+        uint8_t **unpacked;
+        uint32_t **packed;
+        uint32_t need;
+        const char *what;
+    } sides[2] = {{&s->gpu_max_query_batch_bytes, &s->unpacked_query_batch, &s->packed_query_batch, qb, "query"},
+                  {&s->gpu_max_target_batch_bytes, &s->unpacked_target_batch, &s->packed_target_batch, tb, "target"}};
+    for (int k = 0; k < 2; k++) {
+        Side &sd = sides[k];
+        if (*sd.max >= sd.need) continue;
         uint32_t i = 2;
-        while (s->gpu_max_target_batch_bytes * i < tb) i++;
-        fprintf(stderr, "[GASAL WARNING:] actual_target_batch_bytes(%d) > Allocated GPU memory (gpu_max_target_batch_bytes=%d). Therefore, allocating %d bytes on GPU (gpu_max_target_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
-                tb, s->gpu_max_target_batch_bytes, s->gpu_max_target_batch_bytes * i, s->gpu_max_target_batch_bytes * i);
-        s->gpu_max_target_batch_bytes *= i;
-        bool aliased = (void *)s->packed_target_batch == (void *)s->unpacked_target_batch;
-        if (s->unpacked_target_batch) G2_CHECK(hipFree(s->unpacked_target_batch));
-        if (!aliased && s->packed_target_batch) G2_CHECK(hipFree(s->packed_target_batch));
-        G2_CHECK(hipMalloc((void **)&(s->unpacked_target_batch), (size_t)s->gpu_max_target_batch_bytes + 16));
-        if (params->isPacked) s->packed_target_batch = (uint32_t *)s->unpacked_target_batch;
-        else G2_CHECK(hipMalloc((void **)&(s->packed_target_batch), (size_t)(s->gpu_max_target_batch_bytes / 8 + 4) * sizeof(uint32_t)));
+        while (*sd.max * i < sd.need) i++;
+        const uint32_t nmax = *sd.max * i;
+        fprintf(stderr, "[GASAL WARNING:] actual_%s_batch_bytes(%d) > Allocated GPU memory (gpu_max_%s_batch_bytes=%d). Therefore, allocating %d bytes on GPU (gpu_max_%s_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
+                sd.what, sd.need, sd.what, *sd.max, nmax, sd.what, nmax);
+        uint8_t *nu = nullptr;
+        uint32_t *np = nullptr;
+        uint8_t *ncig = nullptr;
+        t_failed = false;
+        G2_CHECK(hipMalloc((void **)&nu, (size_t)nmax + 16));
+        if (!t_failed && !params->isPacked) G2_CHECK(hipMalloc((void **)&np, (size_t)(nmax / 8 + 4) * sizeof(uint32_t)));
+        const bool cig = k == 0 && params->start_pos == WITH_TB;
+        if (!t_failed && cig) {
+            fprintf(stderr, "[GASAL WARNING:] actual_query_batch_bytes(%d) > Allocated HOST memory for CIGAR (gpu_max_query_batch_bytes=%d). Therefore, allocating %d bytes on the host (gpu_max_query_batch_bytes=%d). Performance may be lost if this is repeated many times.\n",
+                    qb, nmax, nmax, nmax);
+            G2_CHECK(hipHostMalloc((void **)&ncig, nmax, hipHostMallocDefault));
+        }
+        if (t_failed) {
+            hip_free_quiet(nu);
+            hip_free_quiet(np);
+            if (ncig) (void)hipHostFree(ncig);
+            return false;
+        }
+        const bool aliased = (void *)*sd.packed == (void *)*sd.unpacked;
+        if (*sd.unpacked) G2_CHECK(hipFree(*sd.unpacked));
+        if (!aliased && *sd.packed) G2_CHECK(hipFree(*sd.packed));
+        *sd.unpacked = nu;
+        *sd.packed = params->isPacked ? (uint32_t *)nu : np;
+        if (cig) {
+            if (s->host_res->cigar != NULL) G2_CHECK(hipHostFree(s->host_res->cigar));
+            s->host_res->cigar = ncig;
+        }
+        *sd.max = nmax;
     }
     if (s->gpu_max_n_alns < n) {
         uint32_t i = 2;
         while (s->gpu_max_n_alns * i < n) i++;
+        const uint32_t nmax = s->gpu_max_n_alns * i;
         fprintf(stderr, "[GASAL WARNING:] actual_n_alns(%d) > gpu_max_n_alns(%d). Therefore, allocating memory for %d alignments on  GPU (gpu_max_n_alns=%d). Performance may be lost if this is repeated many times.\n",
-                n, s->gpu_max_n_alns, s->gpu_max_n_alns * i, s->gpu_max_n_alns * i);
-        s->gpu_max_n_alns *= i;
-        uint32_t **arrs[] = {&s->query_batch_offsets, &s->target_batch_offsets, &s->query_batch_lens, &s->target_batch_lens};
-        for (auto a : arrs) {
-            if (*a) G2_CHECK(hipFree(*a));
-            G2_CHECK(hipMalloc((void **)a, (size_t)s->gpu_max_n_alns * sizeof(uint32_t)));
+                n, s->gpu_max_n_alns, nmax, nmax);
+        uint32_t **arrs[] = {&s->query_batch_offsets, &s->target_batch_offsets, &s->query_batch_lens,
+                             &s->target_batch_lens, &s->seed_scores};
+        uint32_t *na[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        uint8_t *nop[2] = {nullptr, nullptr};
+        t_failed = false;
+        for (int k = 0; k < 5 && !t_failed; k++) G2_CHECK(hipMalloc((void **)&na[k], (size_t)nmax * sizeof(uint32_t)));
+        for (int k = 0; k < 2 && !t_failed; k++) G2_CHECK(hipMalloc((void **)&nop[k], nmax));
+        gasal_res_t *ncpy = nullptr, *nres = nullptr, *ncpy2 = nullptr, *nres2 = nullptr;
+        if (!t_failed) ncpy = gasal_res_new_device_cpy(nmax, params);
+        if (!t_failed) nres = gasal_res_new_device(ncpy);
+        if (!t_failed && params->secondBest) ncpy2 = gasal_res_new_device_cpy(nmax, params);
+        if (!t_failed && params->secondBest) nres2 = gasal_res_new_device(ncpy2);
+        if (t_failed) {
+            for (uint32_t *p : na) hip_free_quiet(p);
+            for (uint8_t *p : nop) hip_free_quiet(p);
+            for (gasal_res_t *c : {ncpy, ncpy2}) {
+                if (!c) continue;
+                for (void *f : {(void *)c->aln_score, (void *)c->query_batch_start, (void *)c->target_batch_start,
+                                (void *)c->query_batch_end, (void *)c->target_batch_end})
+                    hip_free_quiet(f);
+                free(c);
+            }
+            hip_free_quiet(nres);
+            hip_free_quiet(nres2);
+            return false;
         }
-        if (s->seed_scores) G2_CHECK(hipFree(s->seed_scores));
-        G2_CHECK(hipMalloc((void **)&(s->seed_scores), (size_t)s->gpu_max_n_alns * sizeof(uint32_t)));
+        for (int k = 0; k < 5; k++) {
+            if (*arrs[k]) G2_CHECK(hipFree(*arrs[k]));
+            *arrs[k] = na[k];
+        }
         if (s->query_op) G2_CHECK(hipFree(s->query_op));
         if (s->target_op) G2_CHECK(hipFree(s->target_op));
-        G2_CHECK(hipMalloc((void **)&(s->query_op), s->gpu_max_n_alns));
-        G2_CHECK(hipMalloc((void **)&(s->target_op), s->gpu_max_n_alns));
+        s->query_op = nop[0];
+        s->target_op = nop[1];
         gasal_res_destroy_device(s->device_res, s->device_cpy);
-        s->device_cpy = gasal_res_new_device_cpy(s->gpu_max_n_alns, params);
-        s->device_res = gasal_res_new_device(s->device_cpy);
+        s->device_cpy = ncpy;
+        s->device_res = nres;
         if (params->secondBest) {
             gasal_res_destroy_device(s->device_res_second, s->device_cpy_second);
-            s->device_cpy_second = gasal_res_new_device_cpy(s->gpu_max_n_alns, params);
-            s->device_res_second = gasal_res_new_device(s->device_cpy_second);
+            s->device_cpy_second = ncpy2;
+            s->device_res_second = nres2;
         }
+        s->gpu_max_n_alns = nmax;
     }
+    return true;
 }
 
 template <typename T>
@@ -963,8 +1067,14 @@ void ensure(T *&p, size_t &cap_bytes, size_t need_bytes) {
     if (cap_bytes >= need_bytes && p) return;
     if (p) G2_CHECK(hipFree(p));
     p = nullptr;
+    cap_bytes = 0;
     size_t b = need_bytes + need_bytes / 8 + 256;
-    G2_CHECK(hipMalloc((void **)&p, b));
+    hipError_t e = hipMalloc((void **)&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        G2_CHECK(e);
+        return;
+    }
     cap_bytes = b;
 }
 
@@ -1352,10 +1462,17 @@ hipError_t launch_program(Impl *m, const G2Scores &sc, const G2Opts &op, const G
     return hipGraphLaunch(m->gexec, st);
 }
 
-void run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+// Enqueues the device program; false (nothing launched) when a workspace
+// allocation failed in error mode 1.
+bool run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
     G2Dev d;
-    prepare(s, qb, tb, n, params, d);
     Impl *m = impl_of(s);
+    t_failed = false;
+    prepare(s, qb, tb, n, params, d);
+    if (t_failed) {
+        m->main_valid = false;
+        return false;
+    }
     m->main_dev = d;
     m->main_valid = true;
     G2Scores sc = scores_now();
@@ -1364,7 +1481,9 @@ void run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Par
     if (e != hipSuccess) {
         report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
         gasal2_fail(EXIT_FAILURE);
+        return false;
     }
+    return true;
 }
 
 void download_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t n, Parameters *params) {
@@ -1393,11 +1512,16 @@ void download_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t n, Parameters 
 extern "C" void gasal_aln_async(gasal_gpu_storage_t *gpu_storage, const uint32_t actual_query_batch_bytes,
                                 const uint32_t actual_target_batch_bytes, const uint32_t actual_n_alns,
                                 Parameters *params) {
+    t_failed = false;
     if (!validate(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns)) return;
-    grow_device(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params);
+    if (!validate_batch(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
+    // error mode 1: a failed allocation or copy ends the call before any later stage
+    if (!grow_device(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
     upload_async(gpu_storage, actual_n_alns, params);
-    run_async(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params);
+    if (t_failed) return;
+    if (!run_async(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
     download_async(gpu_storage, actual_query_batch_bytes, actual_n_alns, params);
+    if (t_failed) return;
     gpu_storage->is_free = 0;
 }
 
@@ -1418,14 +1542,18 @@ extern "C" int gasal_is_aln_async_done(gasal_gpu_storage_t *gpu_storage) {
 }
 
 extern "C" void gasal2_upload(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    t_failed = false;
     if (!validate(s, qb, tb, n)) return;
+    if (!validate_batch(s, qb, tb, n, params)) return;
     impl_of(s)->prepped = false;
-    grow_device(s, qb, tb, n, params);
+    if (!grow_device(s, qb, tb, n, params)) return;
     upload_async(s, n, params);
+    if (t_failed) return;
     G2_CHECK(hipStreamSynchronize(s->str));
 }
 
 extern "C" void gasal2_run_resident(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
+    t_failed = false;
     if (!validate(s, qb, tb, n)) return;
     run_async(s, qb, tb, n, params);
 }
@@ -1467,7 +1595,9 @@ extern "C" int32_t gasal2_prep_resident(gasal_gpu_storage_t *s, uint32_t qb, uin
     G2Dev d;
     PrepExtra px;
     px.banded = true;   // no direction-tile workspace for this call
+    t_failed = false;
     prepare(s, qb, tb, n, params, d, px);
+    if (t_failed) return -1;
     G2Opts op = opts_of(params);
     op.prep_only = 1;
     hipError_t e = g2_run(scores_now(), op, d, s->str, nullptr);
@@ -1524,7 +1654,12 @@ extern "C" int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32
     PrepExtra px;
     px.banded = true;
     G2Dev d;
+    t_failed = false;
     prepare(s, qb, tb, n, params, d, px);
+    if (t_failed) {
+        m->main_valid = false;
+        return -1;
+    }
     m->main_dev = d;
     m->main_valid = true;
     if (!m->prepped) {
@@ -1637,7 +1772,9 @@ extern "C" int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32
         G2Dev ds;
         PrepExtra px;
         px.hq = hq; px.ht = ht; px.sub = true;
+        t_failed = false;
         prepare(s, qb, tb, c, params, ds, px);
+        if (t_failed) return -1;
         if (m->sub_cap < c || !m->sub_u32) {
             for (void **pp : {(void **)&m->sub_u32, (void **)&m->sub_res, (void **)&m->sub_nops})
                 if (*pp) G2_CHECK(hipFree(*pp));

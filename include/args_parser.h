@@ -8,13 +8,21 @@
 #ifndef ARGS_PARSER_H
 #define ARGS_PARSER_H
 
-#include <fstream>
-#include <iostream>
-#include <string>
 #include <string.h>
 #include "gasal.h"
 
 enum fail_type { NOT_ENOUGH_ARGS, TOO_MANY_ARGS, WRONG_ARG, WRONG_FILES, WRONG_ALGO };
+
+#ifndef __cplusplus
+/* C hosts hold a Parameters only through a pointer: they create it with
+ * gasal2_parameters_new() and set its fields with gasal2_parameters_set()
+ * (gasal2_ext.h). */
+typedef enum fail_type fail_type;
+typedef struct Parameters Parameters;
+#else
+#include <fstream>
+#include <iostream>
+#include <string>
 
 class Parameters {
   public:
@@ -56,5 +64,6 @@ class Parameters {
     int argc;
     char **argv;
 };
+#endif /* __cplusplus */
 
 #endif

@@ -9,7 +9,9 @@
 #include "gasal.h"
 #include "args_parser.h"
 
+#ifdef __cplusplus
 extern "C" {
+#endif
 /* src/ctors.cpp:17-23 */
 gasal_gpu_storage_v gasal_init_gpu_storage_v(int n_streams);
 /* src/ctors.cpp:26-128: per storage, pinned host pages and arrays, device
@@ -26,6 +28,8 @@ void gasal_gpu_mem_free(gasal_gpu_storage_t *gpu_storage, Parameters *params);
 void gasal_destroy_streams(gasal_gpu_storage_v *gpu_storage_vec, Parameters *params);
 /* src/ctors.cpp:193-196 */
 void gasal_destroy_gpu_storage_v(gasal_gpu_storage_v *gpu_storage_vec);
+#ifdef __cplusplus
 }
+#endif
 
 #endif

@@ -14,7 +14,15 @@
 #include <stdint.h>
 #include <stdio.h>
 
+/* C++ hosts get the whole HIP runtime header; C hosts (cgo, a plain C
+ * program) only need its runtime API types and compile with HIP's platform
+ * flags (`hipconfig --cpp_config`: -D__HIP_PLATFORM_AMD__ -I<rocm>/include). */
+#ifdef __cplusplus
 #include <hip/hip_runtime.h>
+#else
+#include <stdbool.h>
+#include <hip/hip_runtime_api.h>
+#endif
 
 #ifndef HOST_MALLOC_SAFETY_FACTOR
 #define HOST_MALLOC_SAFETY_FACTOR 5
@@ -54,7 +62,7 @@ void gasal2_fail(int code);
 }
 #endif
 
-inline int CudaCheckKernelLaunch() {
+static inline int CudaCheckKernelLaunch(void) {
     hipError_t err = hipGetLastError();
     if (hipSuccess != err) return -1;
     return 0;
@@ -69,6 +77,15 @@ enum data_source { NONE, QUERY, TARGET, BOTH };
 enum algo_type { UNKNOWN, GLOBAL, SEMI_GLOBAL, LOCAL, MICROLOCAL, BANDED, KSW };
 
 enum operation_on_seq { FORWARD_NATURAL, REVERSE_NATURAL, FORWARD_COMPLEMENT, REVERSE_COMPLEMENT };
+
+#ifndef __cplusplus
+/* C spells the enum types without the `enum` keyword only through typedefs. */
+typedef enum comp_start comp_start;
+typedef enum Bool Bool;
+typedef enum data_source data_source;
+typedef enum algo_type algo_type;
+typedef enum operation_on_seq operation_on_seq;
+#endif
 
 /* Linked list of pinned host pages (src/gasal.h:75-83). */
 struct host_batch {

@@ -11,7 +11,9 @@
 #include "gasal.h"
 #include "args_parser.h"
 
+#ifdef __cplusplus
 extern "C" {
+#endif
 
 /* Replaces gasal_copy_subst_scores, src/gasal_align.cu:329-339 (declared
  * src/gasal_align.h:114).  Sets the process-wide match/mismatch/gap scores used
@@ -30,6 +32,8 @@ void gasal_aln_async(gasal_gpu_storage_t *gpu_storage, const uint32_t actual_que
  * batch), -1 = still running, -2 = nothing was launched on this storage. */
 int gasal_is_aln_async_done(gasal_gpu_storage_t *gpu_storage);
 
+#ifdef __cplusplus
 }
+#endif
 
 #endif

@@ -13,7 +13,9 @@
 #include <string.h>
 #include "gasal.h"
 
+#ifdef __cplusplus
 extern "C" {
+#endif
 /* src/host_batch.cpp:11-22 */
 host_batch_t *gasal_host_batch_new(uint32_t batch_bytes, uint32_t offset);
 /* src/host_batch.cpp:24-41 */
@@ -35,6 +37,8 @@ uint32_t gasal_host_batch_addbase(gasal_gpu_storage_t *gpu_storage, uint32_t idx
 void gasal_host_batch_print(host_batch_t *res);
 /* src/host_batch.cpp:235-244 */
 void gasal_host_batch_printall(host_batch_t *res);
+#ifdef __cplusplus
 }
+#endif
 
 #endif

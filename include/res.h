@@ -10,7 +10,9 @@
 #include "gasal.h"
 #include "args_parser.h"
 
+#ifdef __cplusplus
 extern "C" {
+#endif
 /* src/res.cpp:8-73 */
 gasal_res_t *gasal_res_new_host(uint32_t max_n_alns, Parameters *params);
 /* src/res.cpp:76-101 */
@@ -21,6 +23,8 @@ gasal_res_t *gasal_res_new_device_cpy(uint32_t max_n_alns, Parameters *params);
 void gasal_res_destroy_host(gasal_res_t *res);
 /* src/res.cpp:160-177 */
 void gasal_res_destroy_device(gasal_res_t *device_res, gasal_res_t *device_cpy);
+#ifdef __cplusplus
 }
+#endif
 
 #endif

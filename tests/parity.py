@@ -20,14 +20,18 @@ def fields(algo, start, second):
 
 
 def run_both(batch, algo, start=g.WITHOUT_START, second=False, head=g.TARGET, tail=g.TARGET, k_band=0,
-             is_rc=False, scores=(1, 4, 6, 1), n_penalty=None, n_code=0x4E, cigars=True):
+             is_rc=False, scores=(1, 4, 6, 1), n_penalty=None, n_code=0x4E, cigars=True, packed=False,
+             per_pair=None):
+    """packed: the host supplies 4-bit packed words (isPacked=true); the
+    oracle always reads the bases."""
     p = g.Parameters(algo=algo, start_pos=start, secondBest=g.TRUE if second else g.FALSE,
                      semiglobal_skipping_head=head, semiglobal_skipping_tail=tail, k_band=k_band,
-                     isReverseComplement=is_rc, sa=scores[0], sb=scores[1], gapo=scores[2], gape=scores[3])
+                     isReverseComplement=is_rc, sa=scores[0], sb=scores[1], gapo=scores[2], gape=scores[3],
+                     isPacked=packed)
     a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=scores, n_code=n_code,
                   n_penalty=n_penalty)
     try:
-        got = a.run(batch)
+        got = a.run(batch, per_pair=per_pair)
         if start == g.WITH_TB and cigars:
             got["cigars"] = [a.cigar(k) for k in range(batch.n)]
     finally:

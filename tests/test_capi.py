@@ -110,3 +110,96 @@ def test_cigar_text_format():
     # reverse-order bytes op | count << 2 -> merged forward text (test_prog.cpp:382-428)
     cg = bytes([(5 << 2) | 0, (3 << 2) | 2, (63 << 2) | 0, (2 << 2) | 0])
     assert g.cigar_to_string(cg) == "65M3D5M"
+
+
+def test_failed_growth_keeps_storage_and_launches_nothing():
+    """Error mode 1: a device allocation that fails while growing the storage
+    (here: no GPU in this process) ends gasal_aln_async before any copy or
+    launch, and the storage keeps its old buffers and recorded sizes."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("needs a process where device allocation fails")
+    L = g.lib()
+    L.gasal2_set_error_mode(1)
+    try:
+        s = g.gasal_gpu_storage_t()
+        s.host_max_query_batch_bytes = s.host_max_target_batch_bytes = 4096
+        s.gpu_max_query_batch_bytes = s.gpu_max_target_batch_bytes = 8
+        s.host_max_n_alns = s.gpu_max_n_alns = 16
+        s.is_free = 1
+        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB)
+        before = L.gasal2_error_count()
+        L.gasal_aln_async(C.byref(s), 64, 64, 1, p.native())
+        assert L.gasal2_error_count() > before
+        assert "CUDA error" in L.gasal2_last_error().decode()
+        assert s.gpu_max_query_batch_bytes == 8 and s.gpu_max_target_batch_bytes == 8
+        assert not s.unpacked_query_batch and not s.packed_query_batch
+        assert s.is_free == 1   # nothing enqueued
+    finally:
+        L.gasal2_set_error_mode(0)
+
+
+def _page(nbytes, data_size):
+    buf = (C.c_uint8 * nbytes)()
+    pg = g.host_batch_t()
+    pg.data = C.cast(buf, C.POINTER(C.c_uint8))
+    pg.page_size = nbytes
+    pg.data_size = data_size
+    pg.offset = 0
+    pg.next = None
+    return pg, buf
+
+
+@pytest.mark.parametrize("packed,qb,page_bytes,msg", [
+    (True, 64, 64, "isPacked: the query host batch holds 64 bytes of packed words (128 bases) but "
+                   "actual_query_batch_bytes is 64"),
+    (False, 64, 72, "the query host batch holds 72 bytes, more than actual_query_batch_bytes(64)"),
+])
+def test_batch_bytes_checked_against_host_pages(packed, qb, page_bytes, msg):
+    """With isPacked the byte counts count bases: a caller passing the packed
+    byte count is told so before anything reaches the device."""
+    L = g.lib()
+    L.gasal2_set_error_mode(1)
+    try:
+        s = g.gasal_gpu_storage_t()
+        s.host_max_query_batch_bytes = s.host_max_target_batch_bytes = 4096
+        s.host_max_n_alns = 16
+        s.is_free = 1
+        qp, qbuf = _page(256, page_bytes)
+        tp, tbuf = _page(256, 8)
+        s.extensible_host_unpacked_query_batch = C.pointer(qp)
+        s.extensible_host_unpacked_target_batch = C.pointer(tp)
+        p = g.Parameters(algo=g.LOCAL, isPacked=packed)
+        before = L.gasal2_error_count()
+        L.gasal_aln_async(C.byref(s), qb, 64, 1, p.native())
+        assert L.gasal2_error_count() == before + 1
+        assert msg in L.gasal2_last_error().decode()
+        assert s.is_free == 1
+    finally:
+        L.gasal2_set_error_mode(0)
+
+
+@pytest.mark.parametrize("off,ln,msg", [(4, 8, "pair 0: query offset 4, length 8"),
+                                        (56, 9, "pair 0: query offset 56, length 9")])
+def test_pair_outside_batch_rejected(off, ln, msg):
+    """Every pair's sequence must lie inside the batch at a multiple of 8
+    bases; otherwise the call fails before any device work."""
+    L = g.lib()
+    L.gasal2_set_error_mode(1)
+    try:
+        s = g.gasal_gpu_storage_t()
+        s.host_max_query_batch_bytes = s.host_max_target_batch_bytes = 4096
+        s.host_max_n_alns = 16
+        s.is_free = 1
+        arrs = [(C.c_uint32 * 1)(v) for v in (off, 0, ln, 8)]
+        s.host_query_batch_offsets = C.cast(arrs[0], C.POINTER(C.c_uint32))
+        s.host_target_batch_offsets = C.cast(arrs[1], C.POINTER(C.c_uint32))
+        s.host_query_batch_lens = C.cast(arrs[2], C.POINTER(C.c_uint32))
+        s.host_target_batch_lens = C.cast(arrs[3], C.POINTER(C.c_uint32))
+        p = g.Parameters(algo=g.LOCAL)
+        before = L.gasal2_error_count()
+        L.gasal_aln_async(C.byref(s), 64, 64, 1, p.native())
+        assert L.gasal2_error_count() == before + 1
+        assert msg in L.gasal2_last_error().decode()
+    finally:
+        L.gasal2_set_error_mode(0)
