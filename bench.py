@@ -9,13 +9,23 @@ pipeline (pack, reverse/complement, shape sort, DP, traceback) over the whole
 batch with the inputs already resident in HBM.
 
 Prints one JSON line.  `value` is GCUPS = sum over pairs of Lq*Lt (unpadded
-cells) / step time.  `roofline` prices the DP kernel against the integer-VALU
-peak (N_CU * 64 lanes * clock; twice that when the batch ran a packed 16-bit
-kernel, two cells per lane instruction) at the DP kernel's algorithmic int
-ops per cell (15 for the default mode: recurrence 11 + second best 4); `traffic`
-comes from rocprofv3 counter runs when given.  `cpu_baseline` times the CPU
+cells) / step time.  `roofline` prices the whole step against the
+integer-VALU peak (N_CU * 64 lanes * clock; twice that when the batch ran a
+packed 16-bit kernel, two cells per lane instruction) at the fixed
+algorithmic ops per cell of the mode (16 with traceback, 11 without):
+achieved = GCUPS * ops; `roofline.dp_kernel` prices the DP kernel alone (HIP
+events on its stream) at the ops it executes; `traffic` is the DP kernel's
+HBM bytes per launch from rocprofv3 counter runs.  `cpu_baseline` times the CPU
 oracle (an OpenMP restatement of the reference kernels) on a bounded prefix of
 the same batch on this host and reports the GPU's agreement with it there.
+
+The default run also measures every other workload of the benchmark set under
+`configs` (each with its own agreement check against the oracle): the 10k-pair
+short-read batch, the full-feature batch's GLOBAL / SEMI_GLOBAL modes and its
+packed-input form, steady-state end to end over sub-batches (host batch to
+host results), the uneven-length batch, the seed-extension batch (KSW, both
+forms, and tile-banded) and the learned band on long divergent pairs.
+`--headline-only` skips them; `--workload` / `--mode` run one configuration.
 """
 import argparse
 import json
@@ -28,13 +38,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic int ops per cell of the DP kernel: the recurrence with end
-# tracking (substitution, diagonal add, 4-way max, gap open, two gap extends,
-# two gap maxima, rank encode, running max) is 11; second best adds 4 (compare,
-# select, max, running max); direction bits stored by the DP add 8 (four
-# comparisons, four inserts).  With the checkpointed traceback the DP stores
-# no direction bits (the traceback kernel recomputes the tiles it needs).
+# Fixed algorithmic int ops per cell of the metric: the recurrence (score
+# select 2, diagonal add 1, H max 2, gap open 1, E' 2, F' 2, running max 1) is
+# 11; with traceback 16 (+ 2 source compares, 2 continuation compares, 1
+# nibble insert).  N handling and position bookkeeping are not counted.
 OPS_SCORE = 11
+OPS_TB = 16
+# What the DP kernel alone executes: second best adds 4 (compare, select, max,
+# running max); direction bits stored by the DP add 8 (four comparisons, four
+# inserts).  With the checkpointed traceback the DP stores no direction bits
+# (the traceback kernel recomputes the tiles on the path).
 OPS_SECOND = 4
 OPS_DIR = 8
 
@@ -72,23 +85,185 @@ def work_cells(batch, mode, k_band=32, ksw_w=-1):
     return 64 * tot
 
 
+def host_info(threads):
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model, "threads": threads,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
+def default_cpu_threads():
+    """The OpenMP default: OMP_NUM_THREADS when set (the host's CPU share),
+    else every CPU this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+MODES = {
+    # mode: (algo, start, second, head, tail, k_band)
+    "local_tb_second": ("LOCAL", "WITH_TB", True, None, None, 0),
+    "local_tb": ("LOCAL", "WITH_TB", False, None, None, 0),
+    "local_score": ("LOCAL", "WITHOUT_START", False, None, None, 0),
+    "local_start": ("LOCAL", "WITH_START", False, None, None, 0),
+    "global": ("GLOBAL", "WITH_START", False, None, None, 0),
+    "semi_global": ("SEMI_GLOBAL", "WITH_START", False, "TARGET", "TARGET", 0),
+    "ksw": ("KSW", "WITHOUT_START", False, None, None, 0),
+    "banded": ("BANDED", "WITHOUT_START", False, None, None, 32),
+}
+
+
+def params_for(g, mode, rc, packed=False):
+    algo, start, second, head, tail, kb = MODES[mode]
+    kw = dict(algo=getattr(g, algo), start_pos=getattr(g, start), secondBest=g.TRUE if second else g.FALSE,
+              isReverseComplement=rc, isPacked=packed, k_band=kb)
+    if head:
+        kw.update(semiglobal_skipping_head=getattr(g, head), semiglobal_skipping_tail=getattr(g, tail))
+    return g.Parameters(**kw)
+
+
+def metric_ops(mode):
+    return OPS_TB if MODES[mode][1] == "WITH_TB" else OPS_SCORE
+
+
+def kernel_ops(mode, tb_checkpoint):
+    algo, start, second = MODES[mode][:3]
+    ops = OPS_SCORE + (OPS_SECOND if second else 0)
+    if start == "WITH_TB" and not tb_checkpoint:
+        ops += OPS_DIR
+    return ops
+
+
+def oracle_run(g, batch, p, m, threads, kw, kz, with_cigar=False):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    sub = batch.subset(np.arange(m)) if m < batch.n else batch
+    t0 = time.perf_counter()
+    r = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
+                head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, k_band=p.k_band,
+                is_rc=bool(p.isReverseComplement), n_penalty=1, n_threads=threads, with_cigar=with_cigar,
+                ksw_w=kw, ksw_zdrop=kz)
+    return sub, r, time.perf_counter() - t0
+
+
+def agreement(g, p, res, ref, m, cigar_pairs=0, batch=None):
+    """Share of the first m pairs whose every defined output equals the
+    oracle's (reference src/res.cpp:8-73); CIGAR bytes in the reference layout
+    on the first cigar_pairs pairs when the oracle ran with CIGARs."""
+    keys = ["score"] if p.algo == g.GLOBAL else ["score", "qend", "tend"]
+    if (p.algo == g.LOCAL and p.start_pos in (g.WITH_START, g.WITH_TB)) or \
+            (p.algo == g.SEMI_GLOBAL and p.start_pos == g.WITH_START):
+        keys += ["qstart", "tstart"]
+    if p.secondBest and p.algo in (g.LOCAL, g.SEMI_GLOBAL):
+        keys += ["s_score", "s_qend", "s_tend"]
+    ok = np.ones(m, bool)
+    for k in keys:
+        ok &= res[k][:m] == ref[k]
+    if p.start_pos == g.WITH_TB and p.algo in (g.LOCAL, g.GLOBAL):
+        ok &= res["n_ops"][:m] == ref["n_ops"]
+        if cigar_pairs and "cigars" in ref and batch is not None:
+            ql = res["cigar_qlayout"]
+            for k in range(min(cigar_pairs, m)):
+                slot = 8 * ((int(batch.qlen[k]) + 7) // 8)
+                nb = min(int(ref["n_ops"][k]), slot)
+                o = int(batch.qoff[k])
+                if bytes(ql[o:o + nb]) != ref["cigars"][k][:nb]:
+                    ok[k] = False
+    return float(ok.mean())
+
+
+class Run:
+    """One workload resident on the device in one storage."""
+
+    def __init__(self, g, batch, p, cells, steps, warmup):
+        self.g, self.batch, self.p, self.cells = g, batch, p, cells
+        self.steps, self.warmup = steps, warmup
+        self.a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=(1, 4, 6, 1),
+                           n_penalty=1)
+        self.a.load(batch)
+
+    def timed(self, policy=0):
+        L, a = self.g.lib(), self.a
+        L.gasal2_set_dispatch(policy)
+        try:
+            for _ in range(self.warmup):
+                a.run_resident(timing=True)
+                a.sync()
+            dp, tb, dev = [], [], []
+            a.sync()
+            t0 = time.perf_counter()
+            for _ in range(self.steps):
+                a.run_resident(timing=True)
+                a.sync()
+                dp.append(a.stage_ms(3))
+                tb.append(a.stage_ms(4))
+                dev.append(a.stage_ms(0))
+            a.sync()
+            wall = time.perf_counter() - t0
+        finally:
+            L.gasal2_set_dispatch(0)
+        ms = 1e3 * wall / self.steps
+        return {"gcups": round(self.cells / (wall / self.steps) / 1e9, 3), "ms_per_step": round(ms, 3),
+                "device_ms": round(float(np.mean(dev)), 3), "dp_kernel_ms": round(float(np.mean(dp)), 3),
+                "traceback_ms": round(float(np.mean(tb)), 3), "paths": self.a.last_paths()}
+
+    def results(self):
+        return self.a.download()
+
+    def pcie_inclusive(self):
+        """One batch through gasal_aln_async, host batch already filled,
+        timed from the call to results on the host."""
+        a = self.a
+        qb, tb, n = a.fill(self.batch, per_pair=False)
+        t0 = time.perf_counter()
+        a.align_filled(qb, tb, n, poll_s=0)
+        s = time.perf_counter() - t0
+        return {"gcups": round(self.cells / s / 1e9, 3), "ms": round(1e3 * s, 3)}, a._collect(n, qb)
+
+    def close(self):
+        self.a.close()
+
+
+def check(g, run, res, m, threads, kw=-1, kz=0, cigar_pairs=0):
+    sub, ref, _ = oracle_run(g, run.batch, run.p, m, threads, kw, kz, with_cigar=cigar_pairs > 0)
+    return agreement(g, run.p, res, ref, len(sub.qlen), cigar_pairs, sub)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
-    ap.add_argument("--mode", default=None,
-                    choices=["local_tb_second", "local_tb", "local_score", "local_start", "global", "semi_global", "ksw",
-                             "banded"],
-                    help="default: local_tb_second for the full-feature batch, local_score for the uneven one")
-    ap.add_argument("--workload", default="full", choices=["full", "uneven", "seed", "band"],
-                    help="full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal lengths 50-5000 bp; "
-                         "band: learned band vs full DP on 1-10 kbp pairs (train seed 5, held-out seed 6)")
-    ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for --workload band")
-    ap.add_argument("--pipeline", type=int, default=0,
-                    help="also stream the batch in this many sub-batches through one storage and through two "
-                         "alternating storages (H2D of one overlapping the other's compute); 0 = skip")
+    ap.add_argument("--mode", default=None, choices=sorted(MODES),
+                    help="one configuration: the mode (default local_tb_second for the full-feature batch, "
+                         "local_score for the uneven one, ksw for the seed batch)")
+    ap.add_argument("--workload", default=None, choices=["full", "uneven", "seed", "band"],
+                    help="one configuration: full: 1M pairs of 100-300 bp (+flanks); uneven: 500k pairs, log-normal "
+                         "lengths 50-5000 bp; seed: 2M seed extensions; band: learned band vs full DP on 1-10 kbp "
+                         "pairs (train seed 5, held-out seed 6)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="measure the headline configuration only (no other workloads)")
+    ap.add_argument("--packed", action="store_true", help="one configuration with isPacked=true host inputs")
+    ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
+    ap.add_argument("--pipeline", type=int, default=4,
+                    help="steady state end to end: the full-feature batch in this many sub-batches through "
+                         "gasal_aln_async (host batch to host results); 0 = skip")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--check-widen", type=int, nargs="+", default=[128],
                     help="check-band widening(s) (diagonals each side) to evaluate; the first one is reported as value")
@@ -108,12 +283,15 @@ def main():
                     help="checkpointed LOCAL WITH_TB: the separate traceback kernel walks every pair (0, default) "
                          "or the DP kernel walks each lane-path pair's traceback (1; the DP kernel time then "
                          "includes those walks)")
+    ap.add_argument("--pk16", type=int, default=1, choices=[0, 1, 2],
+                    help="packed 16-bit kernels: 1 where values fit (default), 0 int32 only, 2 two pairs per lane "
+                         "for every LOCAL mode (gasal2_set_pk16)")
     ap.add_argument("--ksw-ext", type=int, nargs=2, default=None, metavar=("W", "ZDROP"),
                     help="KSW band half-width and z-drop (seed extension as bwa; default: the reference's none, 0)")
     ap.add_argument("--cpu-pairs", type=int, default=-1,
                     help="oracle sample for cpu_baseline: -1 = sized to about --cpu-seconds, 0 = skip")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=default_cpu_threads())
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES bytes and exit: the reference kernel whose "
                          "FETCH_SIZE / WRITE_SIZE counters (rocprofv3 --pmc) calibrate the DP kernels' byte counts")
@@ -141,69 +319,57 @@ def main():
     import gasal2_amd as g
 
     if args.workload == "band":
-        return band_main(args)
+        line = band_main(args)
+        print(json.dumps(line), flush=True)
+        return
+    single = args.workload is not None or args.mode is not None or args.packed
+    workload = args.workload or "full"
+    t_start = time.time()
 
     t0 = time.time()
-    if args.workload == "full":
+    if workload == "full":
         batch = g.synth_batch(2, 2, args.pairs)
         mode = args.mode or "local_tb_second"
-        workload = ("full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
-                    "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries")
-    elif args.workload == "seed":
+        wdesc = ("full-feature batch: 1M pairs Lq~U{100..300}, target = query + 0-100 bp flanks, "
+                 "5% divergence, 0.5% N (N_PENALTY 1), 50% reverse-complemented queries")
+        rc = True
+    elif workload == "seed":
         batch = g.synth_batch(4, 4, args.pairs if args.pairs != 1_000_000 else 2_000_000)
         mode = args.mode or "ksw"
-        workload = ("seed extension: 2M pairs Lq, Lt ~ U{100..250}, t = q mutated 3%, h0 ~ U{10..60}; "
-                    "KSW in the reference's form (no band, z-drop 0) or tile-banded LOCAL with k_band 32")
+        wdesc = ("seed extension: 2M pairs Lq, Lt ~ U{100..250}, t = q mutated 3%, h0 ~ U{10..60}; "
+                 "KSW in the reference's form (no band, z-drop 0) or tile-banded LOCAL with k_band 32")
+        rc = False
     else:
         batch = g.synth_batch(3, 3, args.pairs if args.pairs != 1_000_000 else 500_000)
         mode = args.mode or "local_score"
-        workload = ("uneven-length batch: 500k pairs, Lq log-normal (median 400, clipped 50-5000 bp), "
-                    "Lt = Lq * U[0.8,1.2], 10% divergence; one launch, per-pair lane/wave switch")
+        wdesc = ("uneven-length batch: 500k pairs, Lq log-normal (median 400, clipped 50-5000 bp), "
+                 "Lt = Lq * U[0.8,1.2], 10% divergence; one launch, per-pair lane/wave switch")
+        rc = False
     kw, kz = args.ksw_ext if args.ksw_ext else (-1, 0)
     if args.ksw_ext:
-        workload += f"; KSW band w={kw}, z-drop {kz}"
+        wdesc += f"; KSW band w={kw}, z-drop {kz}"
+    if args.packed:
+        wdesc += "; isPacked=true host inputs"
     cells = work_cells(batch, mode, ksw_w=kw)
     log(f"generated {batch.n} pairs, {cells / 1e9:.2f} G cells in {time.time() - t0:.1f}s")
 
-    if mode == "local_tb_second":
-        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True)
-        ops = OPS_SCORE + OPS_SECOND + (0 if args.tb_checkpoint else OPS_DIR)
-    elif mode == "local_tb":
-        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, isReverseComplement=True)
-        ops = OPS_SCORE + (0 if args.tb_checkpoint else OPS_DIR)
-    elif mode == "local_score":
-        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITHOUT_START, isReverseComplement=True)
-        ops = OPS_SCORE
-    elif mode == "local_start":   # forward pass plus the reverse pass up to the start (counted once)
-        p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_START, isReverseComplement=True)
-        ops = OPS_SCORE
-    elif mode == "global":
-        p = g.Parameters(algo=g.GLOBAL, start_pos=g.WITH_START, isReverseComplement=True)
-        ops = OPS_SCORE
-    elif mode == "ksw":
-        p = g.Parameters(algo=g.KSW, start_pos=g.WITHOUT_START)
-        ops = OPS_SCORE
-    elif mode == "banded":
-        p = g.Parameters(algo=g.BANDED, start_pos=g.WITHOUT_START, k_band=32)
-        ops = OPS_SCORE
-    else:
-        p = g.Parameters(algo=g.SEMI_GLOBAL, start_pos=g.WITH_START, isReverseComplement=True,
-                         semiglobal_skipping_head=g.TARGET, semiglobal_skipping_tail=g.TARGET)
-        ops = OPS_SCORE
+    p = params_for(g, mode, rc, args.packed)
+    ops = metric_ops(mode)
+    kops = kernel_ops(mode, args.tb_checkpoint)
 
     dev = g.device_info()
     peak_tops = dev["n_cu"] * 64 * dev["clock_mhz"] * 1e6 / 1e12
-    a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=(1, 4, 6, 1), n_penalty=1)
-    t0 = time.time()
-    a.load(batch)
-    log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
-
     L = g.lib()
     g.set_ksw_extension(kw, kz)
     L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
     L.gasal2_set_tb_fused(args.tb_fused)
+    L.gasal2_set_pk16(args.pk16)
     if args.wf:
         L.gasal2_set_wavefront(*args.wf)
+    t0 = time.time()
+    run = Run(g, batch, p, cells, args.steps, args.warmup)
+    log(f"uploaded in {time.time() - t0:.2f}s; device {dev}")
+
     tuned = None
     if args.autotune:
         from gasal2_amd import autotune
@@ -217,60 +383,26 @@ def main():
         log(f"autotune on {sample.n} pairs: best {r['best']} of {len(r['timings'])} candidates "
             f"({tuned['seconds']} s)")
 
-    dev_ms = []   # device time of each timed step (first to last event on the stream)
-
-    def timed_steps(policy):
-        L.gasal2_set_dispatch(policy)
-        for _ in range(args.warmup):
-            a.run_resident(timing=True)
-            a.sync()
-        dp, tb = [], []
-        dev_ms.clear()
-        a.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            a.run_resident(timing=True)
-            a.sync()
-            dp.append(a.stage_ms(3))
-            tb.append(a.stage_ms(4))
-            dev_ms.append(a.stage_ms(0))
-        a.sync()
-        return time.perf_counter() - t0, dp, tb
-
-    lane_only = None
-    if args.compare_lane_only:
-        w2, dp2, _ = timed_steps(2)
-        lane_only = {"gcups": round(cells / (w2 / args.steps) / 1e9, 3), "ms_per_step": round(1e3 * w2 / args.steps, 3),
-                     "dp_kernel_ms": round(float(np.mean(dp2)), 3)}
+    lane_only = run.timed(2) if args.compare_lane_only else None
+    if lane_only:
         log(f"one lane per pair only: {lane_only}")
-    wall, dp_ms, tb_ms = timed_steps(args.dispatch)
-    ms_per_step = 1e3 * wall / args.steps
-    gcups = cells / (wall / args.steps) / 1e9
-    dp_avg = float(np.mean(dp_ms))
-    achieved_tops = cells * ops / (dp_avg * 1e-3) / 1e12
+    t = run.timed(args.dispatch)
+    gcups, ms_per_step, dp_avg = t["gcups"], t["ms_per_step"], t["dp_kernel_ms"]
     # packed 16-bit kernels do two cells' ops per lane instruction: their peak is
     # twice the int32 one (only when no one-wave blocks shared the launch)
-    paths = a.last_paths()
-    packed = (paths & 1) and not (paths & 2) and mode != "ksw"
-    peak_used = peak_tops * (2 if packed else 1)
-    log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); device {np.mean(dev_ms):.2f} ms; DP kernel {dp_avg:.2f} ms; "
-        f"traceback {np.mean(tb_ms):.2f} ms")
-
-    # PCIe-inclusive rate of one batch through gasal_aln_async: host batch
-    # already filled, timed from the call to results on the host
-    qb, tb, n = a.fill(batch, per_pair=False)
-    t0 = time.perf_counter()
-    a.align_filled(qb, tb, n, poll_s=0)
-    pcie_s = time.perf_counter() - t0
-    res = a._collect(n, qb)
-    L.gasal2_set_dispatch(0)
-    log(f"gasal_aln_async incl. H2D/D2H: {pcie_s * 1e3:.1f} ms ({cells / pcie_s / 1e9:.1f} GCUPS)")
+    packed16 = bool((t["paths"] & 1) and not (t["paths"] & 2) and mode != "ksw")
+    peak_used = peak_tops * (2 if packed16 else 1)
+    log(f"step {ms_per_step:.2f} ms ({gcups:.1f} GCUPS); device {t['device_ms']:.2f} ms; DP kernel {dp_avg:.2f} ms; "
+        f"traceback {t['traceback_ms']:.2f} ms")
+    pcie, res = run.pcie_inclusive()
+    log(f"gasal_aln_async incl. H2D/D2H: {pcie}")
 
     traffic = None
     tpath = args.traffic
-    if tpath is None and args.workload == "full" and mode == "local_tb_second" and args.pairs == 1_000_000:
+    if tpath is None and workload == "full" and mode == "local_tb_second" and args.pairs == 1_000_000 \
+            and not args.packed:
         tpath = os.path.join(ROOT, "profiles", "dp_traffic_local_tb_second_1M.md")
-    if tpath is None and args.workload == "seed" and mode == "ksw" and not args.ksw_ext and args.pairs == 1_000_000:
+    if tpath is None and workload == "seed" and mode == "ksw" and not args.ksw_ext and args.pairs == 1_000_000:
         tpath = os.path.join(ROOT, "profiles", "dp_traffic_ksw_seed_2M.md")
     if tpath and os.path.exists(tpath):
         with open(tpath) as f:
@@ -280,140 +412,272 @@ def main():
 
     cpu = None
     if args.cpu_pairs != 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
-
-        def oracle_on(m):
-            sub = batch.subset(np.arange(m))
-            t0 = time.perf_counter()
-            r = O.align(sub, p.algo, start_pos=p.start_pos, second_best=bool(p.secondBest),
-                        head=p.semiglobal_skipping_head, tail=p.semiglobal_skipping_tail, k_band=p.k_band,
-                        is_rc=bool(p.isReverseComplement), n_penalty=1, n_threads=args.cpu_threads, with_cigar=False,
-                        ksw_w=kw, ksw_zdrop=kz)
-            return sub, r, time.perf_counter() - t0
-
         if args.cpu_pairs > 0:
             m = min(args.cpu_pairs, batch.n)
         else:   # pilot on 2000 pairs, then a prefix of about --cpu-seconds
             m0 = min(2000, batch.n)
-            _, _, s0 = oracle_on(m0)
+            _, _, s0 = oracle_run(g, batch, p, m0, args.cpu_threads, kw, kz)
             m = int(min(batch.n, max(m0, m0 * args.cpu_seconds / max(s0, 1e-3))))
-        sub, ref, cpu_s = oracle_on(m)
+        sub, ref, cpu_s = oracle_run(g, batch, p, m, args.cpu_threads, kw, kz)
         sub_cells = work_cells(sub, mode, ksw_w=kw)
-        # outputs the mode defines (reference src/res.cpp:8-73)
-        keys = ["score"] if p.algo == g.GLOBAL else ["score", "qend", "tend"]
-        if (p.algo == g.LOCAL and p.start_pos in (g.WITH_START, g.WITH_TB)) or \
-                (p.algo == g.SEMI_GLOBAL and p.start_pos == g.WITH_START):
-            keys += ["qstart", "tstart"]
-        if p.secondBest and p.algo in (g.LOCAL, g.SEMI_GLOBAL):
-            keys += ["s_score", "s_qend", "s_tend"]
-        ok = np.ones(m, bool)
-        for k in keys:
-            ok &= res[k][:m] == ref[k]
-        if p.start_pos == g.WITH_TB:
-            ok &= res["n_ops"][:m] == ref["n_ops"]
         cpu = {"value": round(sub_cells / cpu_s / 1e9, 4), "unit": "GCUPS", "cores": args.cpu_threads,
                "kind": "port",
                "sample": f"first {m} pairs of the same batch, same mode, oracle OpenMP schedule(dynamic) "
                          f"({cpu_s:.1f}s)",
-               "gpu_agreement_on_sample": float(ok.mean())}
+               "gpu_agreement_on_sample": agreement(g, p, res, ref, len(sub.qlen)),
+               "host": host_info(args.cpu_threads)}
         log(f"cpu oracle: {cpu}")
+    run.close()
 
-    pipe = pipeline_measure(g, batch, p, args.pipeline) if args.pipeline > 1 else None
-
+    if mode == "ksw":
+        # KSW streams each pair's (H, E) row through HBM on every target row:
+        # priced against HBM with the algorithmic bytes (8 B per nominal cell)
+        roof = {"bound": "hbm", "kernel": "k_ksw (one lane per pair)", "bytes_per_cell": 8,
+                "achieved": round(cells * 8 / (dp_avg * 1e-3) / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(cells * 8 / (dp_avg * 1e-3) / 1e9 / 8000.0, 4), "traffic": traffic,
+                "dp_kernel_ms": dp_avg}
+    else:
+        ach = gcups * ops / 1e3
+        dp_ach = cells * kops / (dp_avg * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": "whole device step (pack, reverse/complement, shape sort, DP, traceback)",
+                "ops_per_cell": ops, "packed_int16": packed16,
+                "achieved": round(ach, 3), "peak": round(peak_used, 3), "unit": "TOP/s",
+                "frac": round(ach / peak_used, 4), "traffic": traffic,
+                "dp_kernel": {"kernel": "DP (one lane per pair)" if workload == "full"
+                              else "DP (one lane or one wave per pair)",
+                              "ms": dp_avg, "ops_per_cell": kops, "achieved": round(dp_ach, 3),
+                              "frac": round(dp_ach / peak_used, 4),
+                              "note": "ops the DP kernel itself executes over its HIP-event duration"}}
     line = {
         "metric": "GCUPS + % int-VALU roofline",
-        "value": round(gcups, 3),
+        "value": gcups,
         "unit": "GCUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": workload,
-                   "mode": mode, "tb_checkpoint": args.tb_checkpoint, "tb_fused": args.tb_fused, "dispatch": args.dispatch, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells, "scores": [1, 4, 6, 1],
-                   "device": dev["name"], "n_cu": dev["n_cu"], "clock_mhz": dev["clock_mhz"]},
-        "roofline": ({"bound": "valu",
-                      "kernel": "DP (one lane per pair)" if args.workload == "full" else "DP (one lane or one wave per pair)",
-                      "ops_per_cell": ops, "packed_int16": bool(packed),
-                      "achieved": round(achieved_tops, 3), "peak": round(peak_used, 3), "unit": "TOP/s",
-                      "frac": round(achieved_tops / peak_used, 4), "traffic": traffic,
-                      "dp_kernel_ms": round(dp_avg, 3)} if mode != "ksw" else
-                     # KSW streams each pair's (H, E) row through HBM on every target row:
-                     # priced against HBM with the measured traffic when the profile holds it,
-                     # else the algorithmic upper bound (8 B per nominal cell, packed row)
-                     {"bound": "hbm", "kernel": "k_ksw (one lane per pair)",
-                      "bytes_per_cell": 8,
-                      "achieved": round((traffic if traffic else cells * 8) / (dp_avg * 1e-3) / 1e9, 1),
-                      "peak": 8000.0, "unit": "GB/s",
-                      "frac": round((traffic if traffic else cells * 8) / (dp_avg * 1e-3) / 1e9 / 8000.0, 4),
-                      "traffic": traffic, "achieved_from": "measured traffic" if traffic else "algorithmic bound",
-                      "dp_kernel_ms": round(dp_avg, 3)}),
+        "config": {"workload": wdesc, "mode": mode, "tb_checkpoint": args.tb_checkpoint, "tb_fused": args.tb_fused,
+                   "dispatch": args.dispatch, "pk16": args.pk16, "wavefront_thresholds": args.wf, "pairs": batch.n, "cells": cells,
+                   "scores": [1, 4, 6, 1], "device": dev["name"], "n_cu": dev["n_cu"],
+                   "clock_mhz": dev["clock_mhz"]},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "pcie_inclusive_gcups": round(cells / pcie_s / 1e9, 3),
+        "device_ms": t["device_ms"],
+        "traceback_ms": t["traceback_ms"],
+        "pcie_inclusive": pcie,
     }
     if lane_only:
         line["lane_only"] = lane_only
     if tuned:
         line["autotune"] = tuned
-    if pipe:
-        line["pipeline"] = pipe
+    if not single and not args.headline_only:
+        line["configs"] = other_configs(g, args, batch, peak_tops)
+    line["bench_seconds"] = round(time.time() - t_start, 1)
     print(json.dumps(line), flush=True)
-    a.close()
+
+
+def step_line(g, run, mode, peak_tops, threads, m, kw=-1, kz=0, cigar_pairs=0, policy=0):
+    t = run.timed(policy)
+    res = run.results()
+    ops = metric_ops(mode)
+    packed16 = bool((t["paths"] & 1) and not (t["paths"] & 2) and mode != "ksw")
+    peak = peak_tops * (2 if packed16 else 1)
+    out = {"gcups": t["gcups"], "ms_per_step": t["ms_per_step"], "dp_kernel_ms": t["dp_kernel_ms"],
+           "traceback_ms": t["traceback_ms"], "ops_per_cell": ops, "packed_int16": packed16,
+           "valu_frac": round(t["gcups"] * ops / 1e3 / peak, 4) if mode != "ksw" else None}
+    if m:
+        out["oracle_agreement"] = check(g, run, res, m, threads, kw, kz, cigar_pairs)
+        out["agreement_pairs"] = min(m, run.batch.n)
+    return out
+
+
+def other_configs(g, args, full, peak_tops):
+    """Every other workload of the benchmark set, each with its agreement
+    against the oracle (on all pairs where the oracle finishes in seconds,
+    else on a prefix)."""
+    L = g.lib()
+    thr = args.cpu_threads
+    out = {}
+
+    def timed(name, fn):
+        t0 = time.time()
+        try:
+            out[name] = fn()
+        except Exception as e:   # a failed extra measurement is reported, not fatal to the headline
+            out[name] = {"error": repr(e)}
+        out[name]["seconds"] = round(time.time() - t0, 1)
+        log(f"{name}: {out[name]}")
+
+    # 10k short reads, LOCAL score + ends, every pair checked
+    def c1():
+        b = g.synth_batch(1, 1, 10_000)
+        r = Run(g, b, params_for(g, "local_score", False), work_cells(b, "local_score"), 20, 3)
+        try:
+            return dict(step_line(g, r, "local_score", peak_tops, thr, b.n), pairs=b.n,
+                        workload="10k pairs, q 150 bp, t = 50 + q + 50 (2% subst, 0.5% indels); LOCAL WITHOUT_START")
+        finally:
+            r.close()
+    timed("short_reads_10k", c1)
+
+    # full-feature batch: the other modes, and the headline with packed inputs
+    fcells = work_cells(full, "global")
+    for name, mode, packed in (("full_global_start", "global", False), ("full_semi_global_start", "semi_global", False),
+                               ("full_local_tb_second_packed_inputs", "local_tb_second", True)):
+        def c2(mode=mode, packed=packed):
+            r = Run(g, full, params_for(g, mode, True, packed), fcells, 5, 1)
+            try:
+                o = step_line(g, r, mode, peak_tops, thr, 20_000, cigar_pairs=2000 if MODES[mode][1] == "WITH_TB" else 0)
+                o["pcie_inclusive"] = r.pcie_inclusive()[0]
+                return o
+            finally:
+                r.close()
+        timed(name, c2)
+
+    if args.pipeline > 1:
+        timed("full_steady_state_end_to_end", lambda: pipeline_measure(g, full, params_for(g, "local_tb_second", True),
+                                                                      args.pipeline))
+        timed("full_steady_state_end_to_end_packed_inputs",
+              lambda: pipeline_measure(g, full, params_for(g, "local_tb_second", True, True), args.pipeline))
+
+    # uneven lengths: persistent queue with the lane/wave switch vs one lane per pair
+    def c3():
+        b = g.synth_batch(3, 3, 500_000)
+        c = work_cells(b, "local_score")
+        o = {}
+        r = Run(g, b, params_for(g, "local_score", False), c, 5, 1)
+        try:
+            o["local_score"] = step_line(g, r, "local_score", peak_tops, thr, 3000)
+            lo = r.timed(2)
+            o["local_score_one_lane_per_pair"] = {"gcups": lo["gcups"], "ms_per_step": lo["ms_per_step"]}
+        finally:
+            r.close()
+        r = Run(g, b, params_for(g, "local_tb", False), c, 3, 1)
+        try:
+            o["local_tb"] = step_line(g, r, "local_tb", peak_tops, thr, 2000, cigar_pairs=500)
+        finally:
+            r.close()
+        o["pairs"] = b.n
+        o["cells"] = c
+        return o
+    timed("uneven_500k", c3)
+
+    # seed extension: KSW reference form, KSW w 50 z-drop 100, tile-banded k_band 32
+    def c4():
+        b = g.synth_batch(4, 4, 2_000_000)
+        o = {"pairs": b.n}
+        for name, mode, ext in (("ksw_reference_form", "ksw", (-1, 0)), ("ksw_w50_zdrop100", "ksw", (50, 100)),
+                                ("banded_k32", "banded", (-1, 0))):
+            g.set_ksw_extension(*ext)
+            c = work_cells(b, mode, ksw_w=ext[0])
+            r = Run(g, b, params_for(g, mode, False), c, 3, 1)
+            try:
+                o[name] = step_line(g, r, mode, peak_tops, thr, 20_000, kw=ext[0], kz=ext[1])
+                o[name]["cells"] = c
+            finally:
+                r.close()
+        g.set_ksw_extension(-1, 0)
+        return o
+    timed("seed_extension_2M", c4)
+
+    timed("learned_band", lambda: band_main(args))
+    L.gasal2_set_dispatch(0)
+    return out
 
 
 def pipeline_measure(g, batch, p, parts):
-    """Whole batch through gasal_aln_async in `parts` sub-batches, host fill to
-    results on the host: one storage (each batch waits for the previous one)
-    vs two storages used alternately (batch k+1's fill and H2D overlap batch
-    k's compute and D2H on the other storage's stream).  Outputs compared."""
+    """The batch as `parts` sub-batches through gasal_aln_async, results on the
+    host, outputs compared with one whole-batch run.
+
+    end_to_end: every sub-batch in a storage of its own (one HIP stream each,
+    as the reference's multi-storage pattern: src/ctors.cpp:118,
+    src/gasal_align.cu:152-175,266-304), host batches filled before the clock
+    starts; the clock runs from the first gasal_aln_async (its H2D) to the last
+    gasal_is_aln_async_done == 0, so one sub-batch's H2D overlaps the others'
+    device work and D2H: first H2D -> last D2H in steady state (warm: each
+    storage ran its sub-batch once before).
+    host_loop: two storages used alternately with every sub-batch's host fill
+    inside the clock (a caller whose fill is on the critical path)."""
     n = batch.n
     cuts = np.linspace(0, n, parts + 1).astype(int)
     subs = [batch.subset(np.arange(cuts[i], cuts[i + 1])) for i in range(parts)]
     mq, mt, mn = int(batch.qlen.max()), int(batch.tlen.max()), int(max(s.n for s in subs))
-    keys = ("score", "qend", "tend", "qstart", "tstart")
+    keys = ["score", "qend", "tend", "qstart", "tstart"] + (["n_ops"] if p.start_pos == g.WITH_TB else [])
+    cells = float(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
 
-    def run(nst):
+    def wait(a):
+        while True:
+            r = g.gasal_is_aln_async_done(a.s)
+            if r == 0:
+                return
+            if r != -1:
+                raise RuntimeError(f"gasal_is_aln_async_done returned {r}")
+
+    def collect(a, m, qb):
+        r = a._collect(m, qb)
+        return {k: r[k].copy() for k in keys}
+
+    def end_to_end():
+        al = [g.Aligner(p, mq, mt, mn, scores=(1, 4, 6, 1), n_penalty=1) for _ in range(parts)]
+        try:
+            for a, sb in zip(al, subs):   # warm: workspace and graph for the sub-batch's shape
+                a.align_filled(*a.fill(sb, per_pair=False), poll_s=0)
+            filled = [a.fill(sb, per_pair=False) for a, sb in zip(al, subs)]
+            t0 = time.perf_counter()
+            for a, (qb, tb, m) in zip(al, filled):
+                g.gasal_aln_async(a.s, qb, tb, m, a.params)
+            for a in al:
+                wait(a)
+            t = time.perf_counter() - t0
+            return t, [collect(a, m, qb) for a, (qb, tb, m) in zip(al, filled)]
+        finally:
+            for a in al:
+                a.close()
+
+    def host_loop(nst=2):
         al = [g.Aligner(p, mq, mt, mn, scores=(1, 4, 6, 1), n_penalty=1) for _ in range(nst)]
         out = [None] * parts
         try:
-            for a in al:   # warm: workspace and graph for the sub-batch shape
-                qb, tb, m = a.fill(subs[0], per_pair=False)
-                a.align_filled(qb, tb, m)
+            for a in al:
+                a.align_filled(*a.fill(subs[0], per_pair=False), poll_s=0)
             t0 = time.perf_counter()
             pending = [None] * nst
             for k, sb in enumerate(subs):
                 a = al[k % nst]
-                if pending[k % nst] is not None:   # this storage's previous batch: wait, collect
+                if pending[k % nst] is not None:   # this storage's previous sub-batch: wait, collect
                     j, qb0, m0 = pending[k % nst]
-                    while g.gasal_is_aln_async_done(a.s) != 0:
-                        pass
-                    out[j] = {kk: a._collect(m0, qb0)[kk].copy() for kk in keys}
+                    wait(a)
+                    out[j] = collect(a, m0, qb0)
                 qb, tb, m = a.fill(sb, per_pair=False)
                 g.gasal_aln_async(a.s, qb, tb, m, a.params)
                 pending[k % nst] = (k, qb, m)
             for i, a in enumerate(al):
                 if pending[i] is not None:
                     j, qb0, m0 = pending[i]
-                    while g.gasal_is_aln_async_done(a.s) != 0:
-                        pass
-                    out[j] = {kk: a._collect(m0, qb0)[kk].copy() for kk in keys}
+                    wait(a)
+                    out[j] = collect(a, m0, qb0)
             return time.perf_counter() - t0, out
         finally:
             for a in al:
                 a.close()
 
-    t1, o1 = run(1)
-    t2, o2 = run(2)
-    same = all(np.array_equal(o1[i][k], o2[i][k]) for i in range(parts) for k in keys)
-    cells = float(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
-    res = {"parts": parts, "one_storage_s": round(t1, 4), "two_storages_s": round(t2, 4),
-           "gain": round(t1 / t2, 3), "outputs_identical": bool(same),
-           "gcups_one": round(cells / t1 / 1e9, 2), "gcups_two": round(cells / t2 / 1e9, 2)}
+    a = g.Aligner(p, mq, mt, n, scores=(1, 4, 6, 1), n_penalty=1)
+    try:
+        qb, tb, m = a.fill(batch, per_pair=False)
+        a.align_filled(qb, tb, m, poll_s=0)
+        whole = collect(a, m, qb)
+    finally:
+        a.close()
+    t1, o1 = end_to_end()
+    t2, o2 = host_loop()
+    same = all(np.array_equal(np.concatenate([o[k] for o in oo]), whole[k]) for oo in (o1, o2) for k in keys)
+    res = {"parts": parts, "pairs": n, "end_to_end_s": round(t1, 4), "gcups": round(cells / t1 / 1e9, 2),
+           "host_loop_two_storages_s": round(t2, 4), "host_loop_gcups": round(cells / t2 / 1e9, 2),
+           "outputs_identical_to_one_batch": bool(same), "packed_inputs": bool(p.isPacked)}
     log(f"pipeline: {res}")
     return res
 
@@ -504,7 +768,7 @@ def band_main(args):
         "training": {"seconds": round(train_s, 2), "epochs": args.epochs, "loss_first": hist[0], "loss_last": hist[-1],
                      "widen_mul": mul, "widen_add": add, "peak_half_width": half},
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
 if __name__ == "__main__":

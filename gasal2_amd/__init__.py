@@ -14,7 +14,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgasal2_amd.so")
+# GASAL2_AMD_LIB selects another build of the library (e.g. one compiled with
+# different kernel options); the in-tree build is the default
+LIB_PATH = os.environ.get("GASAL2_AMD_LIB") or os.path.join(_HERE, "libgasal2_amd.so")
 SYNTH_PATH = os.path.join(_HERE, "libgasal2_synth.so")
 
 
@@ -375,7 +377,7 @@ def pack_bases(buf):
     nibble = byte & 0xF, 8 bases per uint32, first base in bits 31:28
     (src/kernels/pack_rc_seqs.h:24-31).  `buf` holds whole 8-base words
     (sequences padded with N_CODE); returns their bytes as stored in memory."""
-    b = np.ascontiguousarray(buf, dtype=np.uint8)
+    b = np.frombuffer(buf, np.uint8) if isinstance(buf, (bytes, bytearray)) else np.ascontiguousarray(buf, np.uint8)
     if len(b) % 8:
         raise ValueError("packing needs whole 8-base words")
     nib = (b.reshape(-1, 8) & 0xF).astype(np.uint32)

@@ -67,12 +67,14 @@ struct G2Dev {
     uint32_t eh_cols;
     int32_t ksw16;                   // KSW rows as packed uint16 (H, E): values fit 16 bits
     uint32_t *aux;                   // small device counters (work queue etc.)
+    uint32_t *sink;                  // 4 KiB a kernel may write discarded records to (never read)
     // score-table kernels (k_local_fast, k_global / k_semi LUT variants): 0 =
     // off, 1 = launched beside the generic kernel, the nibble census in aux[0]
     // picks which one runs.  enc_ok: the LOCAL lane-per-pair position encoding
     // fits 31 bits (otherwise k_local takes the pairs the wavefront leaves).
     int32_t fast, enc_ok;
     int32_t pk16;                    // LOCAL score and ends in packed 16-bit halves (lane_local_pk)
+    int32_t pk2;                     // LOCAL, two pairs per lane in packed 16-bit halves (k_local_pk2)
     int32_t pk16g;                   // GLOBAL score in packed 16-bit halves (lane_global_pk)
     int32_t pk16s;                   // SEMI_GLOBAL (tails none / target) in packed 16-bit halves (semi_pass_pk)
     int32_t enc_s, enc_cb;           // position encoding: h << enc_s | ~strip << (3+enc_cb) | ~col << 3 | 7-row

@@ -116,6 +116,13 @@ __device__ __forceinline__ TbView tb_view(const G2Dev &d, uint32_t grp, uint32_t
     }
     return v;
 }
+// a lane's whole record as ordinary stores: with two lane groups' records
+// in flight per wave, non-temporal stores left lines half-written at
+// eviction (1.27x the record bytes written to HBM against 1.04x, measured)
+__device__ __forceinline__ void tb_store_rec(uint32_t *p, const uint32_t *w) {
+    reinterpret_cast<u32x4 *>(p)[0] = u32x4{w[0], w[1], w[2], w[3]};
+    reinterpret_cast<u32x4 *>(p + TB_HALF)[0] = u32x4{w[4], w[5], w[6], w[7]};
+}
 __device__ __forceinline__ void tb_load(const uint32_t *p, uint32_t *w) {
     const uint4 a = reinterpret_cast<const uint4 *>(p)[0], b = reinterpret_cast<const uint4 *>(p)[1];
     w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
@@ -460,7 +467,7 @@ __device__ __forceinline__ bool census_fast(uint32_t m);
 
 template <int START, bool SECOND>
 __global__ __launch_bounds__(64) void k_local(G2Dev d, G2Scores c) {
-    if (d.fast && (d.enc_ok || ((START == S_WITHOUT_START || START == S_WITH_START) && !SECOND && d.pk16)) &&
+    if (d.fast && (d.enc_ok || d.pk2 || ((START == S_WITHOUT_START || START == S_WITH_START) && !SECOND && d.pk16)) &&
         census_fast(d.aux[0]))
         return;   // k_local_fast handles this batch
     LaneCtx x;
@@ -1210,6 +1217,271 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
         __threadfence();   // this lane's checkpoints and results, seen by its own loads
         tb_walk<true>(d, c, x.slot, lut16, true);
     }
+}
+
+// ------------------------------- LOCAL, two pairs per lane, packed 16-bit ----
+// Lane l of pair-wave w runs the pairs of sorted slots A = 128w + l and
+// B = 128w + 64 + l (lane groups 2w and 2w + 1), A in the low and B in the
+// high 16-bit half of every register (v_pk_add_u16 / v_pk_sub_u16 /
+// v_pk_max_i16 / v_pk_ashrrev_i16 / v_perm_b32).  Each half keeps the int32
+// kernel's visiting order -- strip by strip, column by column, row by row --
+// so the end (first strict maximum in (strip, column, row) order,
+// src/kernels/local_kernel_template.h:145-150,414-418), the second best (the
+// largest h below the running maximum when visited, :160-164,432-437) and the
+// checkpoints are exactly k_local_fast's.  A cell is ranked within its column
+// as h * 8 + (7 - row) (h <= 4095); after each column a half's best takes the
+// column's top rank when its h beats the best's h, together with the column's
+// index i * C + t (< 2^16).  Loop bounds are the larger pair's; a half's cells
+// past its own matrix run on zero words and reach neither its results nor its
+// checkpoints (column validity is per query word, so the masks are per word).
+// The host enables it when H <= 4095, E and F stay above -32000 and the column
+// index fits 16 bits.
+#ifndef PK2_WAVES
+#define PK2_WAVES 2
+#endif
+// 0xFFFF in each 16-bit half whose sign bit is set (v_perm sign-byte selects)
+__device__ __forceinline__ uint32_t half_sign(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x09090808u); }
+
+template <int START, bool SECOND>
+__device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t census, uint32_t wave, uint32_t slot0,
+                               const uint32_t *lut16) {
+    constexpr bool CK = START == S_WITH_TB;   // WITH_TB only with the checkpointed traceback
+    LaneCtx xa, xb;
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    auto setup = [&](LaneCtx &x, uint32_t grp) -> bool {
+        x.lane = lane;
+        x.grp = grp;
+        x.slot = slot0 + grp * WAVE + lane;
+        if (x.slot >= d.n) return false;
+        x.pair = d.perm[x.slot];
+        x.qlen = d.qlen[x.pair];
+        x.tlen = d.tlen[x.pair];
+        x.QW = dregs(x.qlen);
+        x.TW = dregs(x.tlen);
+        x.Q = d.pq + (d.qoff[x.pair] >> 3);
+        x.T = d.pt + (d.toff[x.pair] >> 3);
+        if (d.qpan) {
+            x.Qp = d.qpan + (size_t)grp * d.qpan_w * WAVE + lane;
+            x.Tp = d.tpan + (size_t)grp * d.tpan_w * WAVE + lane;
+            x.ps = WAVE;
+        } else {
+            x.Qp = x.Q;
+            x.Tp = x.T;
+            x.ps = 1;
+        }
+        x.bd = d.bnd + (size_t)grp * d.bnd_cols * WAVE + lane;
+        return true;
+    };
+    if (!setup(xa, 2 * wave)) return;
+    const bool okB = setup(xb, 2 * wave + 1);
+    if (!okB) xb = xa;   // no pair B: the high half repeats A and is never stored
+    const int32_t QWa = xa.QW, TWa = xa.TW;
+    const int32_t QWb = okB ? xb.QW : 0, TWb = okB ? xb.TW : 0;
+    const int32_t QWm = QWa > QWb ? QWa : QWb, TWm = TWa > TWb ? TWa : TWb;
+    const int32_t C = 8 * QWm;
+    const pk16x2 gapoe2 = {(short)c.gapoe, (short)c.gapoe}, gape2 = {(short)c.gape, (short)c.gape};
+    const pk16x2 zero2 = {0, 0};
+    const TbView tvA = CK ? tb_view(d, xa.grp, lane) : TbView{nullptr, 0};
+    const TbView hvA = CK ? tb_view(d, xa.grp, lane, d.tbh) : TbView{nullptr, 0};
+    const TbView tvB = CK ? tb_view(d, xb.grp, lane) : TbView{nullptr, 0};
+    const TbView hvB = CK ? tb_view(d, xb.grp, lane, d.tbh) : TbView{nullptr, 0};
+    if (!CK) {   // boundary rows (H, E) = 0 above strip 0
+        for (int32_t cc = 0; cc < 8 * QWa; cc++) xa.bd[cc * WAVE] = 0u;
+        if (okB)
+            for (int32_t cc = 0; cc < 8 * QWb; cc++) xb.bd[cc * WAVE] = 0u;
+    }
+    // A half's words, strips and checkpoint records past its own matrix are
+    // read at the nearest index inside it (the values only reach that half's
+    // discarded cells); the boundary rows span the batch's widest pair.
+    const int32_t qa1 = max(QWa - 1, 0), qb1 = max((okB ? QWb : QWa) - 1, 0);
+    const int32_t ta1 = max(TWa - 1, 0), tb1 = max((okB ? TWb : TWa) - 1, 0);
+    const bool one_size = d.tb_gbase == nullptr;
+    uint32_t best2 = 0, bpos2 = 0;   // per half: ranked best, its column index
+    uint32_t sec2 = 0, spos2 = 0;    // per half: ranked second best, its column index
+    pk16x2 pm2 = zero2;              // per half: running maximum of h (the reference's maxHH order)
+    uint32_t colid2 = 0;             // column index i * C + t in both halves
+    for (int32_t i = 0; i < TWm; i++) {
+        const bool siA = i < TWa, siB = i < TWb;
+        const uint32_t gA = xa.Tp[(uint32_t)min(i, ta1) * xa.ps], gB = xb.Tp[(uint32_t)min(i, tb1) * xb.ps];
+        uint32_t lutA[9], lutB[9];
+#pragma unroll
+        for (int m = 1; m < 9; m++) {
+            lutA[m] = lut16[(gA >> (32 - 4 * m)) & 15u];
+            lutB[m] = lut16[(gB >> (32 - 4 * m)) & 15u];
+        }
+        pk16x2 h[9], f[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { h[m] = zero2; f[m] = zero2; }
+        // the row above this strip at a word's columns, one pair each: CK: the
+        // previous strip's bottom-row checkpoint (H one column behind E; strip
+        // 0 reads strip 0's and masks it to zero); else the boundary row (H, E)
+        uint32_t nA[8], nB[8];
+        const uint32_t zmask = i > 0 ? 0xFFFFFFFFu : 0u;
+        auto fetch = [&](int32_t j) {
+            if (CK) {
+                hvA.load_nt(max(min(i - 1, ta1), 0), min(j, qa1), nA);
+                hvB.load_nt(max(min(i - 1, tb1), 0), min(j, qb1), nB);
+            } else {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) {
+                    nA[kk] = xa.bd[(j * 8 + kk) * WAVE];
+                    nB[kk] = xb.bd[(j * 8 + kk) * WAVE];
+                }
+            }
+        };
+        fetch(0);
+        uint32_t nrA = xa.Qp[0], nrB = xb.Qp[0];   // query words, one word ahead
+        pk16x2 hb2 = zero2;   // CK: bottom-row H one column back (0 left of column 0)
+        for (int32_t j = 0; j < QWm; j++) {
+            const bool vA = siA && j < QWa, vB = siB && j < QWb;
+            const uint32_t vm2 = (vA ? 0xFFFFu : 0u) | (vB ? 0xFFFF0000u : 0u);
+            const uint32_t rA = nrA, rB = nrB;
+            nrA = xa.Qp[(uint32_t)min(j + 1, qa1) * xa.ps];
+            nrB = xb.Qp[(uint32_t)min(j + 1, qb1) * xb.ps];
+            // this word's row above, packed per column: E and the diagonal H
+            uint32_t pe[8], ph[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                pe[kk] = pack_hi(nA[kk], nB[kk]);
+                ph[kk] = pack_lo(nA[kk], nB[kk]);
+                if (CK) {
+                    pe[kk] &= zmask;
+                    ph[kk] &= zmask;
+                }
+            }
+            fetch(j + 1);   // past the last word: an index clamped inside the pair (unused)
+            // CK: this tile's records, one per pair, stored without branches.
+            // A half past its matrix writes its own region's unused tiles when
+            // every group region spans the largest shape, else (per-group
+            // regions, or no pair B) the sink.
+            uint32_t *const sinkp = d.sink + lane * TB_LANE;
+            const bool okA_st = vA || one_size, okB_st = vB || (one_size && okB);
+            uint32_t *const lsA = okA_st ? tvA.chunk(i, j) : sinkp, *const lsB = okB_st ? tvB.chunk(i, j) : sinkp;
+            uint32_t *const bsA = okA_st ? hvA.chunk(i, j) : sinkp, *const bsB = okB_st ? hvB.chunk(i, j) : sinkp;
+            if (CK) {   // the tile's left column state (H, F)
+                uint32_t w[8];
+#pragma unroll
+                for (int m = 1; m < 9; m++) w[m - 1] = pack_lo(as_u32(h[m]), as_u32(f[m]));
+                tb_store_rec(lsA, w);
+#pragma unroll
+                for (int m = 1; m < 9; m++) w[m - 1] = pack_hi(as_u32(h[m]), as_u32(f[m]));
+                tb_store_rec(lsB, w);
+            }
+            const pk16x2 pm_word = pm2;
+            uint32_t wA[8], wB[8];   // CK: bottom-row records of this tile (stored whole: a record's
+                                     // two halves written apart leave each line half-written twice)
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) {
+                const uint32_t qshA = kk < 7 ? (rA >> (26 - 4 * kk)) & 0x1Cu : (rA << 2) & 0x1Cu;
+                const uint32_t qshB = kk < 7 ? (rB >> (26 - 4 * kk)) & 0x1Cu : (rB << 2) & 0x1Cu;
+                pk16x2 e = as_pk(pe[kk]);
+                pk16x2 p;
+                if (CK) {
+                    p = as_pk(ph[kk]);
+                } else {
+                    p = h[0];
+                    h[0] = as_pk(ph[kk]);
+                }
+                pk16x2 colmax = zero2, colsec = zero2;
+#pragma unroll
+                for (int m = 1; m < 9; m++) {
+                    const uint32_t sa = (uint32_t)__builtin_amdgcn_sbfe(lutA[m], qshA, 4);
+                    const uint32_t sb = (uint32_t)__builtin_amdgcn_sbfe(lutB[m], qshB, 4);
+                    const pk16x2 tmp = p + as_pk(pack_lo(sa, sb));
+                    const pk16x2 hm = pmax(pmax(pmax(tmp, f[m]), e), zero2);
+                    const pk16x2 tt = tmp - gapoe2;
+                    f[m] = pmax(tt, f[m] - gape2);
+                    e = pmax(tt, e - gape2);
+                    const pk16x2 rk = (hm << (pk16x2){3, 3}) | (pk16x2){(short)(8 - m), (short)(8 - m)};
+                    colmax = pmax(colmax, rk);
+                    if (SECOND) {   // ranked h where h < running maximum, else 0
+                        colsec = pmax(colsec, as_pk(as_u32(rk) & half_sign(as_u32(hm - pm2))));
+                        pm2 = pmax(pm2, hm);
+                    }
+                    p = h[m];
+                    h[m] = hm;
+                }
+                if (CK) {
+                    wA[kk] = pack_lo(as_u32(hb2), as_u32(e));   // H(bottom, col - 1), E(bottom, col)
+                    wB[kk] = pack_hi(as_u32(hb2), as_u32(e));
+                    hb2 = h[8];
+                    if (kk == 7) {
+                        tb_store_rec(bsA, wA);
+                        tb_store_rec(bsB, wB);
+                    }
+                } else {   // a half past its matrix writes its own (unused) boundary columns
+                    xa.bd[(j * 8 + kk) * WAVE] = pack_lo(as_u32(h[8]), as_u32(e));
+                    xb.bd[(j * 8 + kk) * WAVE] = pack_hi(as_u32(h[8]), as_u32(e));
+                }
+                // a half's best moves to this column when its h beats the best's h
+                const uint32_t up = half_sign(as_u32(as_pk(best2 | 0x00070007u) - colmax)) & vm2;
+                best2 = (as_u32(colmax) & up) | (best2 & ~up);
+                bpos2 = (colid2 & up) | (bpos2 & ~up);
+                if (SECOND) {
+                    const uint32_t us = half_sign(as_u32(as_pk(sec2 | 0x00070007u) - colsec)) & vm2;
+                    sec2 = (as_u32(colsec) & us) | (sec2 & ~us);
+                    spos2 = (colid2 & us) | (spos2 & ~us);
+                }
+                colid2 += 0x00010001u;
+                // keep each column's ranking with its DP: left to itself the
+                // compiler defers the ranking of several columns and holds every
+                // h of those columns live (an empty asm that reads and writes the
+                // running results orders the columns)
+                if (SECOND) asm volatile("" : "+v"(best2), "+v"(bpos2), "+v"(sec2), "+v"(spos2), "+v"(pm2));
+                else asm volatile("" : "+v"(best2), "+v"(bpos2));
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (SECOND) pm2 = as_pk((as_u32(pm2) & vm2) | (as_u32(pm_word) & ~vm2));   // a half past its matrix keeps its maximum
+        }
+    }
+    auto finish = [&](const LaneCtx &x, uint32_t b, uint32_t bp, uint32_t sc2, uint32_t sp) {
+        const uint32_t pr = x.pair;
+        const int32_t vbest = (int32_t)(b >> 3);
+        int32_t qe = 0, te = 0;
+        if (vbest > 0) {
+            qe = (int32_t)(bp % (uint32_t)C);
+            te = (int32_t)(bp / (uint32_t)C) * 8 + (7 - (int32_t)(b & 7u));
+        }
+        d.score[pr] = vbest;
+        d.qend[pr] = qe;
+        d.tend[pr] = te;
+        if (SECOND) {
+            const int32_t vsec = (int32_t)(sc2 >> 3);
+            d.s_score[pr] = vsec;
+            d.s_tend[pr] = vsec > 0 ? (int32_t)(sp / (uint32_t)C) * 8 + (7 - (int32_t)(sc2 & 7u)) : 0;
+            d.s_qend[pr] = vbest > 0 ? 8 * x.QW - 1 : 0;   // compares with the best (src/kernels/local_kernel_template.h:162)
+        }
+        if (START == S_WITH_START) local_reverse_start(d, c, x, census, vbest, qe, te);
+    };
+    finish(xa, best2 & 0xFFFFu, bpos2 & 0xFFFFu, sec2 & 0xFFFFu, spos2 & 0xFFFFu);
+    if (okB) finish(xb, best2 >> 16, bpos2 >> 16, sec2 >> 16, spos2 >> 16);
+}
+
+// Launch: d.wf_blocks one-wave blocks first (as in k_local_fast), then one
+// block per two lane groups.
+template <int START, bool SECOND, bool WF>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PK2_WAVES, 8))) void k_local_pk2(G2Dev d, G2Scores c) {
+    const uint32_t census = d.aux[0];
+    if (WF && blockIdx.x < d.wf_blocks) {
+        if (START == S_WITHOUT_START && !SECOND) {
+            if (census_fast(census)) wf_local<false, true>(d, c, census);
+            else wf_local<false, false>(d, c, census);
+        }
+        if (START == S_WITH_TB && !SECOND) {
+            if (census_fast(census)) {
+                if (d.wf_tbh) wf_local<true, true, true>(d, c, census);
+                else wf_local<true, true>(d, c, census);
+            } else {
+                wf_local<true, false>(d, c, census);
+            }
+        }
+        return;
+    }
+    if (!census_fast(census)) return;
+    __shared__ uint32_t lut16[16];   // score tables of the 16 target nibbles
+    if (threadIdx.x < 16) lut16[threadIdx.x] = score_lut(c, c.has_np ? -c.np : 0, census, threadIdx.x);
+    __syncthreads();
+    lane2_local_pk<START, SECOND>(d, c, census, blockIdx.x - (WF ? d.wf_blocks : 0u), d.wf_blocks ? d.aux[2] : 0u, lut16);
 }
 
 // ----------------------------------------------------- LOCAL, diagonal band ----
@@ -2465,7 +2737,21 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             else k_local_band<false, false><<<grid, blk, 0, s>>>(d, sc);
             break;
         case A_LOCAL:
-            if (d.fast) {
+            if (d.fast && d.pk2) {   // two pairs per lane: half the lane-path blocks
+                const dim3 g2((groups + 1) / 2), g2w((groups + 1) / 2 + d.wf_blocks);
+                if (op.start == S_WITH_TB) {
+                    if (second) k_local_pk2<S_WITH_TB, true, false><<<g2, blk, 0, s>>>(d, sc);
+                    else if (d.wf_blocks) k_local_pk2<S_WITH_TB, false, true><<<g2w, blk, 0, s>>>(d, sc);
+                    else k_local_pk2<S_WITH_TB, false, false><<<g2, blk, 0, s>>>(d, sc);
+                } else if (op.start == S_WITH_START) {
+                    if (second) k_local_pk2<S_WITH_START, true, false><<<g2, blk, 0, s>>>(d, sc);
+                    else k_local_pk2<S_WITH_START, false, false><<<g2, blk, 0, s>>>(d, sc);
+                } else {
+                    if (second) k_local_pk2<S_WITHOUT_START, true, false><<<g2, blk, 0, s>>>(d, sc);
+                    else if (d.wf_blocks) k_local_pk2<S_WITHOUT_START, false, true><<<g2w, blk, 0, s>>>(d, sc);
+                    else k_local_pk2<S_WITHOUT_START, false, false><<<g2, blk, 0, s>>>(d, sc);
+                }
+            } else if (d.fast) {
                 if (op.start == S_WITH_TB) {
                     if (d.tbh) {
                         if (second) k_local_fast<S_WITH_TB, true, true, false><<<grid, blk, 0, s>>>(d, sc);

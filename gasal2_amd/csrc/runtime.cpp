@@ -41,7 +41,7 @@ uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
 int g_tb_fused = 0;  // checkpointed lane path: the DP kernel walks each pair's traceback itself
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
-int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16
+int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16; 2: two pairs per lane for every LOCAL mode
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
@@ -210,7 +210,7 @@ extern "C" void gasal2_set_tb_checkpoint(int32_t mode) { g_tb_ckpt = mode < 0 ? 
 extern "C" void gasal2_set_tb_fused(int32_t on) { g_tb_fused = on ? 1 : 0; }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" int32_t gasal2_last_paths(gasal_gpu_storage_t *s) { return s && s->impl ? impl_of(s)->last_paths : 0; }
-extern "C" void gasal2_set_pk16(int32_t on) { g_pk16_mode = on ? 1 : 0; }
+extern "C" void gasal2_set_pk16(int32_t mode) { g_pk16_mode = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
     g_ksw_zdrop = zdrop > 0 ? zdrop : 0;
@@ -1187,8 +1187,9 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                                                 (uint64_t)std::max(g_scores.match, 0) * std::min(hq[k], ht[k]));
         if (hmax <= 65535 && g_pk16_mode) d.ksw16 = 1;
     }
-    if (!m->aux) G2_CHECK(hipMalloc((void **)&m->aux, 256));
+    if (!m->aux) G2_CHECK(hipMalloc((void **)&m->aux, 256 + 4096));
     d.aux = m->aux;
+    d.sink = m->aux ? m->aux + 64 : nullptr;
 
     // Score-table kernels: 4-bit signed substitution scores.
     d.fast = 0;
@@ -1367,6 +1368,22 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
         if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
     }
+    // Two pairs per lane in packed 16-bit halves (k_local_pk2): LOCAL with
+    // second best or the checkpointed traceback (policy 2: every LOCAL mode).
+    // Cells are ranked as h * 8 + (7 - row) in 16 bits, so H <= match *
+    // min(8 eq, 8 et) must stay <= 4095; column indices i * 8 eq + t must fit
+    // 16 bits; E and F stay above -32000 as for pk16.
+    d.pk2 = 0;
+    if (d.fast && op.algo == LOCAL && !px.banded && g_pk16_mode) {
+        const bool tb_ok = op.start != WITH_TB || (d.tbh && !d.tb_fused);
+        const bool want = g_pk16_mode == 2 || op.second || op.start == WITH_TB;
+        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
+        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
+        if (tb_ok && want && hmax <= 4095 && lowest >= -32000 && (uint64_t)et * 8u * eq <= 65535u) {
+            d.pk2 = 1;
+            d.pk16 = 0;
+        }
+    }
     // Packed 16-bit GLOBAL score: every value of the lane path fits int16 with
     // room below for the packed "minus infinity" (-30000): |H| <= match and
     // penalty times the shorter side plus open and extend over both sides,
@@ -1436,7 +1453,7 @@ void upload_async(gasal_gpu_storage_t *s, uint32_t n, Parameters *params) {
 // trigger a new capture.  The program holds kernels only (no memset or copy
 // nodes).  Timed runs (per-stage events) launch eagerly.
 hipError_t launch_program(Impl *m, const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStream_t st) {
-    m->last_paths = ((d.pk16 || d.pk16g || d.pk16s || d.ksw16) ? 1 : 0) | (d.wf_blocks ? 2 : 0) | (d.tbh ? 4 : 0);
+    m->last_paths = ((d.pk16 || d.pk2 || d.pk16g || d.pk16s || d.ksw16) ? 1 : 0) | (d.wf_blocks ? 2 : 0) | (d.tbh ? 4 : 0);
     if (!g_graph_mode || m->tm.enabled) return g2_run(sc, op, d, st, &m->tm);
     const bool same = m->g_valid && memcmp(&m->g_dev, &d, sizeof(d)) == 0 && memcmp(&m->g_sc, &sc, sizeof(sc)) == 0 &&
                       memcmp(&m->g_op, &op, sizeof(op)) == 0;

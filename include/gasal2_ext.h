@@ -146,10 +146,14 @@ int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint
  * (src/ctors.cpp:48-53 sizes for the maximum lengths). */
 void gasal2_set_tile_budget(uint64_t bytes);
 
-/* Packed 16-bit DP (default on): LOCAL score-and-ends batches whose values fit
- * int16 run two 8-row strips per lane in the halves of each register; 0 keeps
- * the int32 kernels.  Results are identical.  No reference counterpart. */
-void gasal2_set_pk16(int32_t on);
+/* Packed 16-bit DP, where every value of the batch fits (default mode 1):
+ * LOCAL score-and-ends batches run two 8-row strips of a pair per lane in the
+ * halves of each register; LOCAL with second best or the checkpointed
+ * traceback runs two pairs per lane, one per half; GLOBAL and SEMI_GLOBAL
+ * scores and KSW rows are packed too.  Mode 2 runs every LOCAL mode two pairs
+ * per lane; 0 keeps the int32 kernels.  Results are identical.  No reference
+ * counterpart. */
+void gasal2_set_pk16(int32_t mode);
 
 /* KSW extension (seed extension as bwa's ksw_extend2, which the reference's
  * KSW kernel transcribes with no band and z-drop 0,

@@ -203,3 +203,17 @@ def test_pair_outside_batch_rejected(off, ln, msg):
         assert msg in L.gasal2_last_error().decode()
     finally:
         L.gasal2_set_error_mode(0)
+
+
+def test_host_packing_matches_the_pack_kernel_oracle():
+    """pack_bases (what an isPacked=true caller hands over) is the oracle's
+    pack of the same bytes (src/kernels/pack_rc_seqs.h:24-31), for bytes and
+    arrays, every byte value."""
+    import pyoracle as O
+    import numpy as np
+    data = np.arange(256, dtype=np.uint8).repeat(3)[:760]
+    want = O.pack(data).view(np.uint8)
+    assert bytes(g.pack_bases(data)) == bytes(want)
+    assert bytes(g.pack_bases(data.tobytes())) == bytes(want)
+    with pytest.raises(ValueError):
+        g.pack_bases(b"ACGT")
