@@ -171,6 +171,7 @@ _SIGS = {
     "gasal2_set_dispatch": (None, [C.c_int32]),
     "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_set_graph_mode": (None, [C.c_int32]),
+    "gasal2_set_batch_order": (None, [C.c_int32]),
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
     "gasal2_set_pk16": (None, [C.c_int32]),
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
@@ -359,6 +360,14 @@ class Batch:
     def n(self):
         return len(self.qlen)
 
+    def packed_words(self, query=True):
+        """The query (or target) batch as 4-bit packed words (pack_bases),
+        computed once per batch: what an isPacked=true caller holds."""
+        key = "_pq" if query else "_pt"
+        if getattr(self, key, None) is None:
+            setattr(self, key, pack_bases(self.q if query else self.t))
+        return getattr(self, key)
+
     def subset(self, idx):
         """A new batch of the selected pairs, re-laid-out contiguously."""
         qs = [self.query(i) for i in idx]
@@ -444,9 +453,12 @@ class Results(dict):
 
 
 def _arr(ptr, n, dtype):
+    """A copy of n elements of a host result array (one memcpy)."""
     if not ptr:
         return None
-    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+    addr = C.cast(ptr, C.c_void_p).value
+    raw = (C.c_uint8 * (n * C.sizeof(ptr._type_))).from_address(addr)
+    return np.frombuffer(raw, dtype=ptr._type_).astype(dtype, copy=True)
 
 
 class Aligner:
@@ -487,6 +499,9 @@ class Aligner:
         n = batch.n
         if n > s.host_max_n_alns:
             gasal_host_alns_resize(s, n, self.params)
+        # a new batch starts on empty pages (gasal_is_aln_async_done resets them
+        # after gasal_aln_async; the resident path never calls it)
+        self._L.gasal_host_batch_reset(C.byref(s))
         if self.params.isPacked:
             return self._fill_packed(batch, per_pair)
         if per_pair:
@@ -541,7 +556,7 @@ class Aligner:
                 (data_source.TARGET, batch.t, batch.toff, batch.tlen, s.host_target_batch_offsets, s.host_target_batch_lens)):
             page = (s.extensible_host_unpacked_query_batch if src == data_source.QUERY
                     else s.extensible_host_unpacked_target_batch).contents
-            words = pack_bases(buf)
+            words = batch.packed_words(src == data_source.QUERY)
             if page.page_size < len(words):
                 raise ValueError("batch larger than the storage's first host page; use per_pair=True")
             C.memmove(page.data, words.ctypes.data, len(words))

@@ -41,6 +41,14 @@ uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
 int g_tb_fused = 0;  // checkpointed lane path: the DP kernel walks each pair's traceback itself
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
+// Batch order across storages (gasal2_set_batch_order; default off, the
+// reference's independent streams): 1 makes each gasal_aln_async's H2D wait
+// for the previously submitted batch's H2D, 2 also its device program for the
+// previous batch's program.  The events belong to the storages; the last
+// ones are remembered here.
+int g_batch_order = 0;
+std::mutex g_order_mu;
+hipEvent_t g_last_h2d = nullptr, g_last_dev = nullptr;
 int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16; 2: two pairs per lane for every LOCAL mode
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
 
@@ -82,6 +90,9 @@ struct Impl {
     bool last_tb = false;
     G2Timing tm;
     bool tm_init = false;
+    // batch order across storages (gasal2_set_batch_order): this storage's
+    // last H2D and last device program, for the next submitted batch to wait on
+    hipEvent_t ev_h2d = nullptr, ev_dev = nullptr;
     // captured device program of the last batch shape (see run_async)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -210,6 +221,11 @@ extern "C" void gasal2_set_tb_checkpoint(int32_t mode) { g_tb_ckpt = mode < 0 ? 
 extern "C" void gasal2_set_tb_fused(int32_t on) { g_tb_fused = on ? 1 : 0; }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" int32_t gasal2_last_paths(gasal_gpu_storage_t *s) { return s && s->impl ? impl_of(s)->last_paths : 0; }
+extern "C" void gasal2_set_batch_order(int32_t on) {
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    g_batch_order = on < 0 ? 0 : (on > 2 ? 2 : on);
+    g_last_h2d = g_last_dev = nullptr;
+}
 extern "C" void gasal2_set_pk16(int32_t mode) { g_pk16_mode = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
@@ -779,8 +795,13 @@ extern "C" void gasal_init_streams(gasal_gpu_storage_v *gpu_storage_vec, int max
     }
 }
 
+static void order_forget(Impl *m);
+
 static void impl_free(Impl *m) {
     if (!m) return;
+    order_forget(m);
+    if (m->ev_h2d) (void)hipEventDestroy(m->ev_h2d);
+    if (m->ev_dev) (void)hipEventDestroy(m->ev_dev);
     void *ptrs[] = {m->qpan, m->tpan, m->tb_gbase, m->tb_gqw, m->wf_tb, m->wf_tbo, m->band_wres, m->band_lo, m->band_hi, m->band_bw, m->band_tbo, m->band_flags, m->sub_u32, m->sub_res,
                     m->sub_nops, m->sub_cigar,
                     m->keys, m->perm, m->hist, m->bnd, m->tb, m->eh,
@@ -792,6 +813,12 @@ static void impl_free(Impl *m) {
     if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
     delete m;
+}
+
+static void order_forget(Impl *m) {
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    if (g_last_h2d && g_last_h2d == m->ev_h2d) g_last_h2d = nullptr;
+    if (g_last_dev && g_last_dev == m->ev_dev) g_last_dev = nullptr;
 }
 
 // src/ctors.cpp:130-190
@@ -1481,6 +1508,11 @@ hipError_t launch_program(Impl *m, const G2Scores &sc, const G2Opts &op, const G
 
 // Enqueues the device program; false (nothing launched) when a workspace
 // allocation failed in error mode 1.
+//
+// With isPacked the caller's words are the packed buffer itself, which the
+// reverse/complement step rewrites in place (the reference's aliasing,
+// src/ctors.cpp:64-67); a rerun of the same upload (gasal2_run_resident)
+// therefore skips that step.
 bool run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Parameters *params) {
     G2Dev d;
     Impl *m = impl_of(s);
@@ -1494,12 +1526,15 @@ bool run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Par
     m->main_valid = true;
     G2Scores sc = scores_now();
     G2Opts op = opts_of(params);
+    const bool in_place = params->isPacked && params->isReverseComplement;
+    if (in_place && m->prepped) op.skip_prep = 1;
     hipError_t e = launch_program(m, sc, op, d, s->str);
     if (e != hipSuccess) {
         report("[GASAL CUDA ERROR:] %s(CUDA error no.=%d). Line no. %d in file %s\n", hipGetErrorString(e), (int)e, __LINE__, __FILE__);
         gasal2_fail(EXIT_FAILURE);
         return false;
     }
+    if (in_place) m->prepped = true;   // the resident words are reverse-complemented now
     return true;
 }
 
@@ -1534,9 +1569,29 @@ extern "C" void gasal_aln_async(gasal_gpu_storage_t *gpu_storage, const uint32_t
     if (!validate_batch(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
     // error mode 1: a failed allocation or copy ends the call before any later stage
     if (!grow_device(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
+    Impl *m = impl_of(gpu_storage);
+    m->prepped = false;   // fresh words
+    std::unique_lock<std::mutex> lk(g_order_mu, std::defer_lock);
+    if (g_batch_order) {
+        lk.lock();
+        if (!m->ev_h2d) G2_CHECK(hipEventCreateWithFlags(&m->ev_h2d, hipEventDisableTiming));
+        if (!m->ev_dev) G2_CHECK(hipEventCreateWithFlags(&m->ev_dev, hipEventDisableTiming));
+        if (t_failed) return;
+        if (g_last_h2d) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, g_last_h2d, 0));
+    }
     upload_async(gpu_storage, actual_n_alns, params);
     if (t_failed) return;
+    if (g_batch_order) {
+        G2_CHECK(hipEventRecord(m->ev_h2d, gpu_storage->str));
+        g_last_h2d = m->ev_h2d;
+        if (g_batch_order == 2 && g_last_dev) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, g_last_dev, 0));
+    }
     if (!run_async(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
+    if (g_batch_order) {
+        G2_CHECK(hipEventRecord(m->ev_dev, gpu_storage->str));
+        g_last_dev = m->ev_dev;
+        lk.unlock();
+    }
     download_async(gpu_storage, actual_query_batch_bytes, actual_n_alns, params);
     if (t_failed) return;
     gpu_storage->is_free = 0;

@@ -210,6 +210,15 @@ int32_t gasal2_last_paths(gasal_gpu_storage_t *storage);
  * reference counterpart (the reference launches each kernel,
  * src/gasal_align.cu:180-264). */
 void gasal2_set_graph_mode(int32_t on);
+
+/* Batch order across storages.  0 (default): every storage's stream is
+ * independent, as in the reference (src/ctors.cpp:118; each stream starts its
+ * copies at once, src/gasal_align.cu:152-175).  1: each gasal_aln_async
+ * enqueues its H2D copies after the previously submitted batch's (on any
+ * storage), so batches take PCIe one after another; 2: also its device
+ * program after that batch's program.  Results are identical.  No reference
+ * counterpart. */
+void gasal2_set_batch_order(int32_t mode);
 int32_t gasal2_get_graph_mode(void);
 void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *launches);
 

@@ -52,3 +52,27 @@ def test_packed_local_tb_reference_layout_per_pair():
         slot = 8 * ((int(b.qlen[k]) + 7) // 8)
         nb = min(int(got["n_ops"][k]), slot)
         assert bytes(ql[offs[k]:offs[k] + nb]) == cig[k][:nb], k
+
+
+@pytest.mark.parametrize("packed", [False, True])
+def test_resident_reruns_keep_inputs(packed):
+    """gasal2_run_resident three times on one upload: with isPacked the
+    reverse/complement step rewrites the caller's words in place, so reruns
+    must not apply it again; every run equals the oracle."""
+    import pyoracle as O
+    b = g.synth_batch(2, 1310, 700)
+    p = g.Parameters(algo=g.LOCAL, start_pos=g.WITH_TB, secondBest=g.TRUE, isReverseComplement=True, isPacked=packed)
+    a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n, n_penalty=1)
+    try:
+        a.load(b)
+        outs = []
+        for _ in range(3):
+            a.run_resident()
+            outs.append(a.download())
+    finally:
+        a.close()
+    ref = O.align(b, g.LOCAL, start_pos=g.WITH_TB, second_best=True, is_rc=True, n_penalty=1)
+    for r in outs:
+        for f in fields(g.LOCAL, g.WITH_TB, True):
+            assert np.array_equal(r[f], ref[f]), f
+        assert np.array_equal(r["n_ops"], ref["n_ops"])
