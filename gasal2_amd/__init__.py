@@ -172,6 +172,8 @@ _SIGS = {
     "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_set_graph_mode": (None, [C.c_int32]),
     "gasal2_set_batch_order": (None, [C.c_int32]),
+    "gasal2_aln_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32] + [C.c_void_p] * 6 +
+                           [C.c_uint32] * 3 + [C.c_void_p] * 9 + [C.c_void_p, C.c_void_p]),
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
     "gasal2_set_pk16": (None, [C.c_int32]),
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),

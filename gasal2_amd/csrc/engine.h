@@ -118,6 +118,13 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
 // batch's result arrays and CIGAR slots.
 hipError_t g2_scatter_subset(const G2Dev &sub, const G2Dev &main, bool tb, hipStream_t stream);
 
+// Kernel choices for a batch that depend only on the mode, the scores and the
+// largest shape the lane path runs (runtime.cpp): score-table kernels, then
+// the packed 16-bit kernels and the LOCAL position encoding.
+void g2_choose_fast(G2Dev &d, const G2Opts &op, const G2Scores &sc, uint32_t qw, uint32_t tw);
+void g2_choose_packed(G2Dev &d, const G2Opts &op, const G2Scores &sc, uint32_t eq, uint32_t et, bool banded);
+G2Scores g2_scores();
+
 // Dispatch policy for LOCAL (0 auto, 1 reference-order kernel only).
 void g2_set_dispatch(int policy);
 int g2_get_dispatch();

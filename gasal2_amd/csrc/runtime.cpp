@@ -899,6 +899,93 @@ extern "C" void gasal_gpu_mem_free(gasal_gpu_storage_t *gpu_storage, Parameters 
     if (params->secondBest) gasal_res_destroy_device(gpu_storage->device_res_second, gpu_storage->device_cpy_second);
 }
 
+// Packed and table kernels for a batch: the choices depend only on the
+// mode, the scores and the largest shape the lane path runs (eq query words,
+// et target words); shared by the storage path and gasal2_aln_noalloc.
+void g2_choose_packed(G2Dev &d, const G2Opts &op, const G2Scores &sc, uint32_t eq, uint32_t et, bool banded) {
+    auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
+    // Packed 16-bit LOCAL (score and ends; WITH_START then runs the int32
+    // reverse pass on the packed forward's end): every value of the pairs the lane
+    // path runs fits int16 -- H <= match * min(8 eq, 8 et); E, F >= -(mismatch,
+    // N penalty) - open - 9 extend -- and the policy allows it.
+    d.pk16 = 0;
+    if (d.fast && op.algo == LOCAL && (op.start == WITHOUT_START || op.start == WITH_START) && !op.second && g_pk16_mode) {
+        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
+        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
+        if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
+    }
+    // Two pairs per lane in packed 16-bit halves (k_local_pk2): LOCAL with
+    // second best or the checkpointed traceback (policy 2: every LOCAL mode).
+    // Cells are ranked as h * 8 + (7 - row) in 16 bits, so H <= match *
+    // min(8 eq, 8 et) must stay <= 4095; column indices i * 8 eq + t must fit
+    // 16 bits; E and F stay above -32000 as for pk16.
+    d.pk2 = 0;
+    if (d.fast && op.algo == LOCAL && !banded && g_pk16_mode) {
+        const bool tb_ok = op.start != WITH_TB || (d.tbh && !d.tb_fused);
+        const bool want = g_pk16_mode == 2 || op.second || op.start == WITH_TB;
+        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
+        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
+        if (tb_ok && want && hmax <= 4095 && lowest >= -32000 && (uint64_t)et * 8u * eq <= 65535u) {
+            d.pk2 = 1;
+            d.pk16 = 0;
+        }
+    }
+    // Packed 16-bit GLOBAL score: every value of the lane path fits int16 with
+    // room below for the packed "minus infinity" (-30000): |H| <= match and
+    // penalty times the shorter side plus open and extend over both sides,
+    // E and F at most one open and one penalty lower.
+    d.pk16g = 0;
+    if (d.fast && op.algo == GLOBAL && op.start != WITH_TB && g_pk16_mode && sc.gape <= 255) {
+        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
+        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
+        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
+        if (bound <= 16000) d.pk16g = 1;
+    }
+    // Packed 16-bit SEMI_GLOBAL: the same bound (head gaps as GLOBAL's
+    // boundaries, classic Gotoh) for tails that end on the target or nowhere.
+    d.pk16s = 0;
+    if (d.fast && op.algo == SEMI_GLOBAL && op.start != WITH_TB && (op.tail == NONE || op.tail == TARGET) &&
+        g_pk16_mode && sc.gape <= 255) {
+        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
+        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
+        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
+        if (bound <= 16000) d.pk16s = 1;
+    }
+    // LOCAL lane-per-pair score-table kernel: position encoding
+    // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
+    // (all pairs, or those the one-wave path leaves).
+    if (d.fast && op.algo == LOCAL) {
+        const int cb = std::max(1, bl(8ull * eq - 1)), sbits = std::max(1, bl(et - 1));
+        const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * eq, 8ull * et);
+        const int hb = bl(maxh) + 1;
+        const int S = 3 + cb + sbits;
+        if (S + hb <= 31) {
+            d.enc_ok = 1;
+            d.enc_s = S;
+            d.enc_cb = cb;
+            d.enc_cm = (1u << cb) - 1u;
+            d.enc_sm = (1u << sbits) - 1u;
+        }
+    }
+}
+
+// The process-wide scores (gasal_copy_subst_scores, gasal2_set_n_code /
+// _n_penalty) as the kernels take them.
+G2Scores g2_scores() { return g_scores; }
+
+// Score-table kernels (4-bit signed substitution scores) for the batch.
+void g2_choose_fast(G2Dev &d, const G2Opts &op, const G2Scores &sc, uint32_t qw, uint32_t tw) {
+    d.fast = 0;
+    d.enc_ok = 0;
+    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL || op.algo == KSW;
+    if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
+        auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
+        // the N score of the substitution macros (src/gasal_kernels.h:37-56)
+        const int32_t nval = sc.has_np ? -sc.np : 0;
+        if (fits4(sc.match) && fits4(-sc.mismatch) && fits4(nval)) d.fast = 1;
+    }
+}
+
 // ------------------------------------------------------------ launcher -----
 extern "C" void gasal_copy_subst_scores(gasal_subst_scores *subst) {                 // src/gasal_align.cu:329-339
     g_scores.gapo = subst->gap_open;
@@ -1218,18 +1305,8 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     d.aux = m->aux;
     d.sink = m->aux ? m->aux + 64 : nullptr;
 
-    // Score-table kernels: 4-bit signed substitution scores.
-    d.fast = 0;
-    d.enc_ok = 0;
-    auto bl = [](uint64_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; };
     const G2Scores &sc = g_scores;
-    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL || op.algo == KSW;
-    if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
-        auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
-        // the N score of the substitution macros (src/gasal_kernels.h:37-56)
-        const int32_t nval = sc.has_np ? -sc.np : 0;
-        if (fits4(sc.match) && fits4(-sc.mismatch) && fits4(nval)) d.fast = 1;
-    }
+    g2_choose_fast(d, op, sc, qw, tw);
     // Long LOCAL pairs (score and ends only) run one wave per pair.  The
     // device applies the same shape test in k_keys and also needs the nibble
     // census to allow the score tables; the host count only sizes the launch.
@@ -1385,69 +1462,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         s->packed_tb_matrices = (uint4 *)m->tb;
         s->packed_tb_matrix_size = m->tb_bytes / 16;
     }
-    // Packed 16-bit LOCAL (score and ends; WITH_START then runs the int32
-    // reverse pass on the packed forward's end): every value of the pairs the lane
-    // path runs fits int16 -- H <= match * min(8 eq, 8 et); E, F >= -(mismatch,
-    // N penalty) - open - 9 extend -- and the policy allows it.
-    d.pk16 = 0;
-    if (d.fast && op.algo == LOCAL && (op.start == WITHOUT_START || op.start == WITH_START) && !op.second && g_pk16_mode) {
-        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
-        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
-        if (hmax <= 32000 && lowest >= -32000) d.pk16 = 1;
-    }
-    // Two pairs per lane in packed 16-bit halves (k_local_pk2): LOCAL with
-    // second best or the checkpointed traceback (policy 2: every LOCAL mode).
-    // Cells are ranked as h * 8 + (7 - row) in 16 bits, so H <= match *
-    // min(8 eq, 8 et) must stay <= 4095; column indices i * 8 eq + t must fit
-    // 16 bits; E and F stay above -32000 as for pk16.
-    d.pk2 = 0;
-    if (d.fast && op.algo == LOCAL && !px.banded && g_pk16_mode) {
-        const bool tb_ok = op.start != WITH_TB || (d.tbh && !d.tb_fused);
-        const bool want = g_pk16_mode == 2 || op.second || op.start == WITH_TB;
-        const int64_t hmax = (int64_t)std::max(sc.match, 0) * std::min<int64_t>(8ll * eq, 8ll * et);
-        const int64_t lowest = -(int64_t)std::max(sc.mismatch, sc.has_np ? sc.np : 0) - sc.gapoe - 9ll * sc.gape - 8;
-        if (tb_ok && want && hmax <= 4095 && lowest >= -32000 && (uint64_t)et * 8u * eq <= 65535u) {
-            d.pk2 = 1;
-            d.pk16 = 0;
-        }
-    }
-    // Packed 16-bit GLOBAL score: every value of the lane path fits int16 with
-    // room below for the packed "minus infinity" (-30000): |H| <= match and
-    // penalty times the shorter side plus open and extend over both sides,
-    // E and F at most one open and one penalty lower.
-    d.pk16g = 0;
-    if (d.fast && op.algo == GLOBAL && op.start != WITH_TB && g_pk16_mode && sc.gape <= 255) {
-        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
-        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
-        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
-        if (bound <= 16000) d.pk16g = 1;
-    }
-    // Packed 16-bit SEMI_GLOBAL: the same bound (head gaps as GLOBAL's
-    // boundaries, classic Gotoh) for tails that end on the target or nowhere.
-    d.pk16s = 0;
-    if (d.fast && op.algo == SEMI_GLOBAL && op.start != WITH_TB && (op.tail == NONE || op.tail == TARGET) &&
-        g_pk16_mode && sc.gape <= 255) {
-        const int64_t pen = std::max<int64_t>(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0);
-        const int64_t sh = 8ll * std::min(eq, et), lg = 8ll * ((int64_t)eq + et);
-        const int64_t bound = (pen + std::max(sc.match, 0)) * sh + 2ll * sc.gapoe + (int64_t)sc.gape * lg + pen;
-        if (bound <= 16000) d.pk16s = 1;
-    }
-    // LOCAL lane-per-pair score-table kernel: position encoding
-    // h << S | ~strip | ~column | row must fit 31 bits for the pairs it runs
-    // (all pairs, or those the one-wave path leaves).
-    if (d.fast && op.algo == LOCAL) {
-        const int cb = std::max(1, bl(8ull * eq - 1)), sbits = std::max(1, bl(et - 1));
-        const uint64_t maxh = (uint64_t)std::max(sc.match, 0) * std::min<uint64_t>(8ull * eq, 8ull * et);
-        const int hb = bl(maxh) + 1;
-        const int S = 3 + cb + sbits;
-        if (S + hb <= 31) {
-            d.enc_ok = 1;
-            d.enc_s = S;
-            d.enc_cb = cb;
-            d.enc_cm = (1u << cb) - 1u;
-            d.enc_sm = (1u << sbits) - 1u;
-        }
-    }
+    g2_choose_packed(d, op, sc, eq, et, px.banded);
 }
 
 G2Scores scores_now() { return g_scores; }

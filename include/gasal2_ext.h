@@ -211,6 +211,35 @@ int32_t gasal2_last_paths(gasal_gpu_storage_t *storage);
  * src/gasal_align.cu:180-264). */
 void gasal2_set_graph_mode(int32_t on);
 
+/* The device program of gasal2_run_resident (pack, reverse/complement,
+ * shape sort, DP; LOCAL, GLOBAL and SEMI_GLOBAL, WITHOUT_START or WITH_START,
+ * any second best) on caller-owned device arrays, calling no allocator: its
+ * scratch is one file-scope device array of 32 MiB per GPU, held until the
+ * process ends.  No counterpart in the reference; for callers that cannot
+ * allocate device memory (HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).
+ *   query_batch / target_batch: the batch as gasal_aln_async would upload it
+ *     (bytes padded to multiples of 8, or with params->isPacked the packed
+ *     words; the byte counts count bases either way); offsets, lengths and,
+ *     with isReverseComplement, the per-pair operations as device arrays;
+ *   max_query_len / max_target_len bound every pair's lengths;
+ *   outputs: device arrays of n_alns int32 (score, query_end, target_end
+ *     required; starts and second-best arrays may be NULL); bad_pairs, if not
+ *     NULL, receives the count of pairs outside their batch or longer than the
+ *     declared maxima (those pairs' outputs are undefined).
+ * Enqueued on `stream`; returns 0, or -1 with a message on stderr (never
+ * exits) for WITH_TB, KSW, BANDED, or a batch whose scratch would exceed
+ * 32 MiB.  The first call on a device must not be under graph capture (it
+ * looks up the scratch's address).  Calls that overlap on one GPU share the
+ * scratch and race; inputs are never written. */
+int32_t gasal2_aln_noalloc(const void *query_batch, const void *target_batch, uint32_t query_batch_bytes,
+                           uint32_t target_batch_bytes, const uint32_t *query_offsets, const uint32_t *target_offsets,
+                           const uint32_t *query_lens, const uint32_t *target_lens, const uint8_t *query_ops,
+                           const uint8_t *target_ops, uint32_t n_alns, uint32_t max_query_len, uint32_t max_target_len,
+                           int32_t *score, int32_t *query_end, int32_t *target_end, int32_t *query_start,
+                           int32_t *target_start, int32_t *second_score, int32_t *second_query_end,
+                           int32_t *second_target_end, uint32_t *bad_pairs, Parameters *params, hipStream_t stream);
+
 /* Batch order across storages.  0 (default): every storage's stream is
  * independent, as in the reference (src/ctors.cpp:118; each stream starts its
  * copies at once, src/gasal_align.cu:152-175).  1: each gasal_aln_async
