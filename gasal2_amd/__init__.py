@@ -109,6 +109,25 @@ class gasal_subst_scores(C.Structure):    # src/gasal.h:160-165
     _fields_ = [("match", C.c_int32), ("mismatch", C.c_int32), ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
 
 
+class gasal2_pairs_t(C.Structure):        # include/gasal2_stream.h
+    _fields_ = [("query", _P8), ("target", _P8), ("query_bytes", C.c_uint64), ("target_bytes", C.c_uint64),
+                ("query_words", _P32), ("target_words", _P32),
+                ("query_offsets", _P32), ("target_offsets", _P32), ("query_lens", _P32), ("target_lens", _P32),
+                ("query_ops", _P8), ("target_ops", _P8),
+                ("query_names", C.POINTER(C.c_char_p)), ("target_names", C.POINTER(C.c_char_p)), ("n", C.c_uint32)]
+
+
+class gasal2_results_t(C.Structure):      # include/gasal2_stream.h
+    _fields_ = [(k, C.POINTER(C.c_int32)) for k in ("score", "query_end", "target_end", "query_start", "target_start",
+                                                      "second_score", "second_query_end", "second_target_end")] + \
+               [("n_cigar_ops", _P32), ("cigar", _P8), ("cigar_cap", C.c_uint32)]
+
+
+class gasal2_stream_stats_t(C.Structure):  # include/gasal2_stream.h
+    _fields_ = [("seconds", C.c_double), ("seconds_from_fill", C.c_double), ("sub_batches", C.c_uint32),
+                ("cells", C.c_uint64)]
+
+
 class gasal2_params_t(C.Structure):       # include/gasal2_ext.h
     _fields_ = [(n, C.c_int32) for n in ("sa", "sb", "gapo", "gape", "start_pos", "print_out", "n_threads", "k_band",
                                          "secondBest", "isPacked", "isReverseComplement",
@@ -172,6 +191,14 @@ _SIGS = {
     "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_set_graph_mode": (None, [C.c_int32]),
     "gasal2_set_batch_order": (None, [C.c_int32]),
+    "gasal2_pairs_read": (C.POINTER(gasal2_pairs_t), [C.c_char_p, C.c_char_p, C.c_uint32]),
+    "gasal2_pairs_free": (None, [C.POINTER(gasal2_pairs_t)]),
+    "gasal2_pairs_align": (C.c_int32, [C.POINTER(gasal2_pairs_t), C.POINTER(gasal2_results_t), C.c_void_p, C.c_uint32,
+                                       C.c_int32, C.POINTER(gasal2_stream_stats_t)]),
+    "gasal2_stream_new": (C.c_void_p, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
+    "gasal2_stream_free": (None, [C.c_void_p]),
+    "gasal2_stream_align": (C.c_int32, [C.c_void_p, C.POINTER(gasal2_pairs_t), C.POINTER(gasal2_results_t),
+                                        C.POINTER(gasal2_stream_stats_t)]),
     "gasal2_aln_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32] + [C.c_void_p] * 6 +
                            [C.c_uint32] * 3 + [C.c_void_p] * 9 + [C.c_void_p, C.c_void_p]),
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
@@ -713,6 +740,146 @@ class Aligner:
         """gasal2_last_paths: bit 0 packed 16-bit lane kernel, bit 1 one-wave
         blocks, bit 2 checkpointed traceback."""
         return int(self._L.gasal2_last_paths(C.byref(self.s)))
+
+
+class PairSet:
+    """Pairs for the multi-storage driver (gasal2_pairs_align,
+    include/gasal2_stream.h): read from paired FASTA/FASTQ files by the
+    library's reader (gasal2_pairs_read) or wrapping a Batch's arrays."""
+
+    def __init__(self, ptr=None, batch=None, pack=False):
+        self._ptr = ptr          # owned by the library (gasal2_pairs_read)
+        self._keep = []
+        if ptr is not None:
+            self.p = ptr.contents
+        else:
+            P = gasal2_pairs_t()
+            keep = self._keep
+            def arr(a, dt, ct):
+                a = np.ascontiguousarray(a, dtype=dt)
+                keep.append(a)
+                return a.ctypes.data_as(C.POINTER(ct))
+            P.query, P.target = arr(batch.q, np.uint8, C.c_uint8), arr(batch.t, np.uint8, C.c_uint8)
+            P.query_bytes, P.target_bytes = len(batch.q), len(batch.t)
+            if pack:
+                P.query_words = arr(batch.packed_words(True).view(np.uint32), np.uint32, C.c_uint32)
+                P.target_words = arr(batch.packed_words(False).view(np.uint32), np.uint32, C.c_uint32)
+            P.query_offsets, P.target_offsets = arr(batch.qoff, np.uint32, C.c_uint32), arr(batch.toff, np.uint32, C.c_uint32)
+            P.query_lens, P.target_lens = arr(batch.qlen, np.uint32, C.c_uint32), arr(batch.tlen, np.uint32, C.c_uint32)
+            P.query_ops, P.target_ops = arr(batch.qop, np.uint8, C.c_uint8), arr(batch.top, np.uint8, C.c_uint8)
+            P.n = batch.n
+            self.p = P
+
+    @classmethod
+    def read(cls, query_path, target_path, names=False, pack=False):
+        flags = (1 if names else 0) | (2 if pack else 0)
+        ptr = lib().gasal2_pairs_read(os.fsencode(query_path), os.fsencode(target_path), flags)
+        if not ptr:
+            raise RuntimeError(f"gasal2_pairs_read failed on {query_path}, {target_path}")
+        return cls(ptr=ptr)
+
+    @classmethod
+    def from_batch(cls, batch, pack=False):
+        return cls(batch=batch, pack=pack)
+
+    @property
+    def n(self):
+        return int(self.p.n)
+
+    def names(self):
+        if not self.p.query_names:
+            return None, None
+        return ([self.p.query_names[k].decode() for k in range(self.n)],
+                [self.p.target_names[k].decode() for k in range(self.n)])
+
+    def to_batch(self):
+        """The pairs as a Batch (host arrays copied)."""
+        n = self.n
+        a = lambda ptr, cnt, dt: np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True) if cnt else np.zeros(0, dt)
+        return Batch(a(self.p.query, self.p.query_bytes, np.uint8), a(self.p.target, self.p.target_bytes, np.uint8),
+                     a(self.p.query_offsets, n, np.uint32), a(self.p.target_offsets, n, np.uint32),
+                     a(self.p.query_lens, n, np.uint32), a(self.p.target_lens, n, np.uint32),
+                     a(self.p.query_ops, n, np.uint8), a(self.p.target_ops, n, np.uint8))
+
+    def align(self, params, batch_pairs=250_000, storages=3, scores=(1, 4, 6, 1), n_code=N_CODE_DEFAULT,
+              n_penalty=None, cigar_cap=0, stream=None):
+        """gasal2_pairs_align (or gasal2_stream_align on `stream`, a Stream
+        whose storages stay allocated across calls); returns (Results, stats)."""
+        L = lib()
+        set_n_code(n_code)
+        set_n_penalty(n_penalty)
+        gasal_copy_subst_scores(gasal_subst_scores(*scores))
+        n = self.n
+        out = Results()
+        R = gasal2_results_t()
+        keep = []
+        def slot(name, dt, ct, field):
+            a = np.zeros(n, dt)
+            keep.append(a)
+            out[name] = a
+            setattr(R, field, a.ctypes.data_as(C.POINTER(ct)))
+        slot("score", np.int32, C.c_int32, "score")
+        if params.algo != algo_type.GLOBAL:
+            slot("qend", np.int32, C.c_int32, "query_end")
+            slot("tend", np.int32, C.c_int32, "target_end")
+            if params.start_pos in (comp_start.WITH_START, comp_start.WITH_TB):
+                slot("qstart", np.int32, C.c_int32, "query_start")
+                slot("tstart", np.int32, C.c_int32, "target_start")
+        if params.secondBest:
+            slot("s_score", np.int32, C.c_int32, "second_score")
+            slot("s_qend", np.int32, C.c_int32, "second_query_end")
+            slot("s_tend", np.int32, C.c_int32, "second_target_end")
+        if params.start_pos == comp_start.WITH_TB:
+            slot("n_ops", np.uint32, C.c_uint32, "n_cigar_ops")
+            if cigar_cap:
+                cg = np.zeros(n * cigar_cap, np.uint8)
+                keep.append(cg)
+                R.cigar = cg.ctypes.data_as(C.POINTER(C.c_uint8))
+                R.cigar_cap = cigar_cap
+                out["cigar_slots"] = cg.reshape(n, cigar_cap)
+        st = gasal2_stream_stats_t()
+        if stream is not None:
+            r = L.gasal2_stream_align(stream.h, C.byref(self.p), C.byref(R), C.byref(st))
+        else:
+            r = L.gasal2_pairs_align(C.byref(self.p), C.byref(R), params.native(), batch_pairs, storages, C.byref(st))
+        if r != 0:
+            raise RuntimeError("gasal2_stream_align failed")
+        return out, {"seconds": st.seconds, "seconds_from_fill": st.seconds_from_fill,
+                     "sub_batches": st.sub_batches, "cells": st.cells}
+
+    def close(self):
+        if self._ptr is not None:
+            lib().gasal2_pairs_free(self._ptr)
+            self._ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Stream:
+    """gasal2_stream_new: storages (one HIP stream and host thread each) kept
+    across PairSet.align(..., stream=this) calls, so their workspace is
+    allocated once."""
+
+    def __init__(self, params, batch_pairs, storages, max_query_len, max_target_len):
+        self.params = params
+        self.h = lib().gasal2_stream_new(params.native(), batch_pairs, storages, max_query_len, max_target_len)
+        if not self.h:
+            raise RuntimeError("gasal2_stream_new failed")
+
+    def close(self):
+        if self.h:
+            lib().gasal2_stream_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def cigar_to_string(cg):

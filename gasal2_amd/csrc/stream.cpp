@@ -1,0 +1,434 @@
+// stream.cpp -- host input path (include/gasal2_stream.h): paired FASTA/FASTQ
+// files read into padded host batches, then streamed through several storages
+// by one host thread each.  Built only on the public API; the reference does
+// the same work inline in its example program (test_prog/test_prog.cpp:79-137
+// reading, :237-330 threads and batches; src/host_batch.cpp:79-153 paging).
+#include "gasal_header.h"
+#include "gasal2_stream.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+int fail(const char *msg, const char *arg = "") {
+    fprintf(stderr, "[GASAL ERROR:] %s%s\n", msg, arg);
+    return -1;
+}
+
+// One file's records, padded with N_CODE to multiples of 8 bases.
+struct Side {
+    std::vector<uint8_t> bytes;
+    std::vector<uint32_t> off, len;
+    std::vector<uint8_t> op;
+    std::vector<std::string> names;
+    bool ok = false;
+};
+
+// Buffered line reader over a FILE (lines without their '\n' / "\r\n").
+struct Lines {
+    FILE *f;
+    std::vector<char> buf;
+    size_t pos = 0, end = 0;
+    bool eof = false;
+    explicit Lines(FILE *fp) : f(fp), buf(1 << 22) {}
+    bool next(std::string &line) {
+        line.clear();
+        for (;;) {
+            if (pos == end) {
+                if (eof) return !line.empty();
+                end = fread(buf.data(), 1, buf.size(), f);
+                pos = 0;
+                if (end == 0) {
+                    eof = true;
+                    return !line.empty();
+                }
+            }
+            const char *s = buf.data() + pos;
+            const char *nl = (const char *)memchr(s, '\n', end - pos);
+            if (nl) {
+                line.append(s, nl - s);
+                pos += (nl - s) + 1;
+                if (!line.empty() && line.back() == '\r') line.pop_back();
+                return true;
+            }
+            line.append(s, end - pos);
+            pos = end;
+        }
+    }
+};
+
+void close_record(Side &sd, uint8_t n_code) {
+    const uint32_t start = sd.off.back();
+    const uint32_t l = (uint32_t)(sd.bytes.size() - start);
+    sd.len.push_back(l);
+    while (sd.bytes.size() % 8) sd.bytes.push_back(n_code);   // src/host_batch.cpp:100-102
+}
+
+void read_side(const char *path, bool names, uint8_t n_code, Side &sd) {
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        fail("cannot open ", path);
+        return;
+    }
+    Lines in(f);
+    std::string line;
+    static const char marks[] = "></+";   // test_prog/test_prog.cpp:79-137
+    bool fastq = false, first = true, open = false;
+    while (in.next(line)) {
+        if (line.empty()) continue;
+        if (first) {
+            fastq = line[0] == '@';
+            first = false;
+        }
+        if (fastq) {
+            if (line[0] != '@') {
+                fail("malformed FASTQ record in ", path);
+                fclose(f);
+                return;
+            }
+            std::string seq, plus, qual;
+            if (!in.next(seq) || !in.next(plus) || !in.next(qual) || plus.empty() || plus[0] != '+') {
+                fail("truncated FASTQ record in ", path);
+                fclose(f);
+                return;
+            }
+            sd.off.push_back((uint32_t)sd.bytes.size());
+            sd.op.push_back(0);
+            if (names) sd.names.push_back(line.substr(1));
+            sd.bytes.insert(sd.bytes.end(), seq.begin(), seq.end());
+            close_record(sd, n_code);
+            continue;
+        }
+        const char *m = strchr(marks, line[0]);
+        if (m && line[0]) {
+            if (open) close_record(sd, n_code);
+            sd.off.push_back((uint32_t)sd.bytes.size());
+            sd.op.push_back((uint8_t)(m - marks));
+            if (names) sd.names.push_back(line.substr(1));
+            open = true;
+        } else if (open) {
+            sd.bytes.insert(sd.bytes.end(), line.begin(), line.end());
+        } else {
+            fail("FASTA sequence before any header in ", path);
+            fclose(f);
+            return;
+        }
+        if (sd.bytes.size() > 0xFFFFFFF0ull) {
+            fail("batch of more than 4 G bases in ", path);
+            fclose(f);
+            return;
+        }
+    }
+    if (open) close_record(sd, n_code);
+    fclose(f);
+    sd.ok = true;
+}
+
+// 4-bit packed words of padded bytes (src/kernels/pack_rc_seqs.h:24-31).
+void pack_words(const uint8_t *b, uint64_t bytes, uint32_t *w) {
+    for (uint64_t i = 0; i < bytes / 8; i++) {
+        const uint8_t *p = b + 8 * i;
+        uint32_t v = 0;
+        for (int k = 0; k < 8; k++) v |= (uint32_t)(p[k] & 15u) << (28 - 4 * k);
+        w[i] = v;
+    }
+}
+
+template <typename T>
+T *copy_out(const std::vector<T> &v) {
+    T *p = (T *)malloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (p && !v.empty()) memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+}
+
+}  // namespace
+
+extern "C" gasal2_pairs_t *gasal2_pairs_read(const char *query_path, const char *target_path, uint32_t flags) {
+    if (!query_path || !target_path) {
+        fail("gasal2_pairs_read: no file name");
+        return nullptr;
+    }
+    const bool names = (flags & GASAL2_READ_NAMES) != 0;
+    const uint8_t n_code = (uint8_t)gasal2_get_n_code();
+    Side q, t;
+    std::thread tq([&] { read_side(query_path, names, n_code, q); });
+    read_side(target_path, names, n_code, t);
+    tq.join();
+    if (!q.ok || !t.ok) return nullptr;
+    if (q.off.size() != t.off.size()) {
+        fail("query and target files hold different numbers of sequences");
+        return nullptr;
+    }
+    gasal2_pairs_t *p = (gasal2_pairs_t *)calloc(1, sizeof(gasal2_pairs_t));
+    if (!p) return nullptr;
+    p->n = (uint32_t)q.off.size();
+    p->query_bytes = q.bytes.size();
+    p->target_bytes = t.bytes.size();
+    p->query = copy_out(q.bytes);
+    p->target = copy_out(t.bytes);
+    p->query_offsets = copy_out(q.off);
+    p->target_offsets = copy_out(t.off);
+    p->query_lens = copy_out(q.len);
+    p->target_lens = copy_out(t.len);
+    p->query_ops = copy_out(q.op);
+    p->target_ops = copy_out(t.op);
+    if (flags & GASAL2_READ_PACK) {
+        p->query_words = (uint32_t *)malloc(std::max<uint64_t>(p->query_bytes / 2, 4));
+        p->target_words = (uint32_t *)malloc(std::max<uint64_t>(p->target_bytes / 2, 4));
+        if (p->query_words && p->target_words) {
+            std::thread pq([&] { pack_words(p->query, p->query_bytes, p->query_words); });
+            pack_words(p->target, p->target_bytes, p->target_words);
+            pq.join();
+        }
+    }
+    if (names) {
+        p->query_names = (char **)calloc(p->n ? p->n : 1, sizeof(char *));
+        p->target_names = (char **)calloc(p->n ? p->n : 1, sizeof(char *));
+        for (uint32_t k = 0; k < p->n && p->query_names && p->target_names; k++) {
+            p->query_names[k] = strdup(q.names[k].c_str());
+            p->target_names[k] = strdup(t.names[k].c_str());
+        }
+    }
+    if (!p->query || !p->target || !p->query_offsets || !p->target_offsets || !p->query_lens || !p->target_lens ||
+        !p->query_ops || !p->target_ops || ((flags & GASAL2_READ_PACK) && (!p->query_words || !p->target_words))) {
+        gasal2_pairs_free(p);
+        fail("gasal2_pairs_read: out of host memory");
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void gasal2_pairs_free(gasal2_pairs_t *p) {
+    if (!p) return;
+    if (p->query_names)
+        for (uint32_t k = 0; k < p->n; k++) free(p->query_names[k]);
+    if (p->target_names)
+        for (uint32_t k = 0; k < p->n; k++) free(p->target_names[k]);
+    void *ptrs[] = {p->query, p->target, p->query_words, p->target_words, p->query_offsets, p->target_offsets,
+                    p->query_lens, p->target_lens, p->query_ops, p->target_ops, p->query_names, p->target_names};
+    for (void *q : ptrs) free(q);
+    free(p);
+}
+
+struct gasal2_stream {
+    Parameters *params;
+    uint32_t batch_pairs;
+    int dev;
+    std::vector<gasal_gpu_storage_v> storages;   // one storage each
+};
+
+extern "C" gasal2_stream_t *gasal2_stream_new(Parameters *params, uint32_t batch_pairs, int32_t n_storages,
+                                              uint32_t max_query_len, uint32_t max_target_len) {
+    if (!params || batch_pairs == 0 || n_storages < 1 || n_storages > 16) {
+        fail("gasal2_stream_new: no Parameters, zero batch size or n_storages outside 1..16");
+        return nullptr;
+    }
+    gasal2_stream_t *st = new gasal2_stream_t();
+    st->params = params;
+    st->batch_pairs = batch_pairs;
+    if (hipGetDevice(&st->dev) != hipSuccess) {
+        delete st;
+        fail("gasal2_stream_new: no current device");
+        return nullptr;
+    }
+    const uint32_t per_q = (max_query_len + 7) & ~7u, per_t = (max_target_len + 7) & ~7u;
+    for (int k = 0; k < n_storages; k++) {
+        gasal_gpu_storage_v v = gasal_init_gpu_storage_v(1);
+        gasal_init_streams(&v, (int)per_q + 8, (int)per_t + 8, (int)batch_pairs, params);
+        st->storages.push_back(v);
+    }
+    return st;
+}
+
+extern "C" void gasal2_stream_free(gasal2_stream_t *st) {
+    if (!st) return;
+    for (gasal_gpu_storage_v &v : st->storages) {
+        gasal_destroy_streams(&v, st->params);
+        gasal_destroy_gpu_storage_v(&v);
+    }
+    delete st;
+}
+
+extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_t *pairs, gasal2_results_t *out,
+                                       gasal2_stream_stats_t *stats) {
+    if (!stp || !pairs || !out) return fail("gasal2_stream_align: NULL argument");
+    Parameters *params = stp->params;
+    const uint32_t batch_pairs = stp->batch_pairs;
+    const int32_t n_storages = (int32_t)stp->storages.size();
+    if (pairs->n == 0) return fail("gasal2_stream_align: no pairs");
+    if (params->algo == KSW) return fail("gasal2_stream_align: KSW needs seed scores (use gasal_aln_async)");
+    const bool packed = params->isPacked;
+    if (packed && (!pairs->query_words || !pairs->target_words))
+        return fail("gasal2_stream_align: isPacked needs the packed words (GASAL2_READ_PACK)");
+    const uint32_t n = pairs->n;
+    uint64_t cells = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        cells += (uint64_t)pairs->query_lens[k] * pairs->target_lens[k];
+        const uint64_t qe = (uint64_t)pairs->query_offsets[k] + ((pairs->query_lens[k] + 7ull) & ~7ull);
+        const uint64_t te = (uint64_t)pairs->target_offsets[k] + ((pairs->target_lens[k] + 7ull) & ~7ull);
+        if ((pairs->query_offsets[k] & 7u) || (pairs->target_offsets[k] & 7u) || qe > pairs->query_bytes ||
+            te > pairs->target_bytes)
+            return fail("gasal2_stream_align: a pair lies outside its batch or off a multiple of 8");
+    }
+    const uint32_t nb = (n + batch_pairs - 1) / batch_pairs;
+    const int dev = stp->dev;
+
+    std::mutex mu;
+    std::condition_variable cv;
+    int ready = 0;
+    bool go = false;
+    std::atomic<int> errors{0};
+    std::chrono::steady_clock::time_point t0, t_end;
+    std::vector<std::chrono::steady_clock::time_point> ends(n_storages), firsts(n_storages);
+
+    auto worker = [&](int st) {
+        (void)hipSetDevice(dev);
+        gasal_gpu_storage_t *s = &stp->storages[st].a[0];
+        bool first = true;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            ready++;
+            cv.notify_all();
+            cv.wait(lk, [&] { return go; });
+        }
+        for (uint32_t b = (uint32_t)st; b < nb && !errors.load(); b += (uint32_t)n_storages) {
+            const uint32_t k0 = b * batch_pairs, k1 = std::min(n, k0 + batch_pairs), m = k1 - k0;
+            // one contiguous span per side when the pairs are in order
+            // (gasal2_pairs_read lays them out so), else per-sequence fills
+            bool contiguous = true;
+            for (uint32_t k = k0 + 1; k < k1 && contiguous; k++)
+                contiguous = pairs->query_offsets[k] == pairs->query_offsets[k - 1] + ((pairs->query_lens[k - 1] + 7) & ~7u) &&
+                             pairs->target_offsets[k] == pairs->target_offsets[k - 1] + ((pairs->target_lens[k - 1] + 7) & ~7u);
+            gasal_host_batch_reset(s);
+            const uint32_t q0 = pairs->query_offsets[k0], t0w = pairs->target_offsets[k0];
+            uint32_t qb = 0, tb = 0;
+            if (contiguous) {
+                qb = pairs->query_offsets[k1 - 1] + ((pairs->query_lens[k1 - 1] + 7) & ~7u) - q0;
+                tb = pairs->target_offsets[k1 - 1] + ((pairs->target_lens[k1 - 1] + 7) & ~7u) - t0w;
+                host_batch_t *pq = s->extensible_host_unpacked_query_batch, *pt = s->extensible_host_unpacked_target_batch;
+                const uint32_t qcopy = packed ? qb / 2 : qb, tcopy = packed ? tb / 2 : tb;
+                if (qcopy > pq->page_size || tcopy > pt->page_size) contiguous = false;
+                else {
+                    if (packed) {
+                        memcpy(pq->data, pairs->query_words + q0 / 8, qcopy);
+                        memcpy(pt->data, pairs->target_words + t0w / 8, tcopy);
+                    } else {
+                        memcpy(pq->data, pairs->query + q0, qcopy);
+                        memcpy(pt->data, pairs->target + t0w, tcopy);
+                    }
+                    pq->data_size = qcopy;
+                    pt->data_size = tcopy;
+                    for (uint32_t k = 0; k < m; k++) {
+                        s->host_query_batch_offsets[k] = pairs->query_offsets[k0 + k] - q0;
+                        s->host_target_batch_offsets[k] = pairs->target_offsets[k0 + k] - t0w;
+                    }
+                }
+            }
+            if (!contiguous) {
+                gasal_host_batch_reset(s);
+                uint32_t qi = 0, ti = 0;
+                for (uint32_t k = 0; k < m; k++) {
+                    const uint32_t p = k0 + k;
+                    const uint32_t lq = (pairs->query_lens[p] + 7) & ~7u, lt = (pairs->target_lens[p] + 7) & ~7u;
+                    s->host_query_batch_offsets[k] = packed ? 2 * qi : qi;
+                    s->host_target_batch_offsets[k] = packed ? 2 * ti : ti;
+                    if (packed) {   // whole words, padded to 8 bytes by the fill as the offsets expect
+                        qi = gasal_host_batch_fill(s, qi, (const char *)(pairs->query_words + pairs->query_offsets[p] / 8), lq / 2, QUERY);
+                        ti = gasal_host_batch_fill(s, ti, (const char *)(pairs->target_words + pairs->target_offsets[p] / 8), lt / 2, TARGET);
+                    } else {
+                        qi = gasal_host_batch_fill(s, qi, (const char *)(pairs->query + pairs->query_offsets[p]), pairs->query_lens[p], QUERY);
+                        ti = gasal_host_batch_fill(s, ti, (const char *)(pairs->target + pairs->target_offsets[p]), pairs->target_lens[p], TARGET);
+                    }
+                }
+                qb = packed ? 2 * qi : qi;
+                tb = packed ? 2 * ti : ti;
+            }
+            memcpy(s->host_query_batch_lens, pairs->query_lens + k0, (size_t)m * 4);
+            memcpy(s->host_target_batch_lens, pairs->target_lens + k0, (size_t)m * 4);
+            if (params->isReverseComplement) {
+                gasal_op_fill(s, pairs->query_ops + k0, m, QUERY);
+                gasal_op_fill(s, pairs->target_ops + k0, m, TARGET);
+            }
+            s->current_n_alns = m;
+            if (first) {
+                firsts[st] = std::chrono::steady_clock::now();
+                first = false;
+            }
+            gasal_aln_async(s, qb, tb, m, params);
+            int r;
+            while ((r = gasal_is_aln_async_done(s)) == -1) std::this_thread::yield();
+            if (r != 0) {
+                errors++;
+                break;
+            }
+            const gasal_res_t *h = s->host_res, *h2 = s->host_res_second;
+            auto put = [&](int32_t *dst, const int32_t *src) {
+                if (dst && src) memcpy(dst + k0, src, (size_t)m * 4);
+            };
+            put(out->score, h->aln_score);
+            put(out->query_end, h->query_batch_end);
+            put(out->target_end, h->target_batch_end);
+            put(out->query_start, h->query_batch_start);
+            put(out->target_start, h->target_batch_start);
+            if (h2) {
+                put(out->second_score, h2->aln_score);
+                put(out->second_query_end, h2->query_batch_end);
+                put(out->second_target_end, h2->target_batch_end);
+            }
+            if (params->start_pos == WITH_TB && h->n_cigar_ops) {
+                if (out->n_cigar_ops) memcpy(out->n_cigar_ops + k0, h->n_cigar_ops, (size_t)m * 4);
+                if (out->cigar && out->cigar_cap)
+                    for (uint32_t k = 0; k < m; k++) {
+                        const uint32_t slot = (pairs->query_lens[k0 + k] + 7) & ~7u;
+                        const uint32_t nb_ = std::min(std::min(h->n_cigar_ops[k], slot), out->cigar_cap);
+                        memcpy(out->cigar + (size_t)(k0 + k) * out->cigar_cap, h->cigar + s->host_query_batch_offsets[k], nb_);
+                    }
+            }
+        }
+        ends[st] = std::chrono::steady_clock::now();
+        if (first) firsts[st] = ends[st];
+    };
+
+    std::vector<std::thread> th;
+    for (int st = 0; st < n_storages; st++) th.emplace_back(worker, st);
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return ready == n_storages; });
+        t0 = std::chrono::steady_clock::now();
+        go = true;
+        cv.notify_all();
+    }
+    for (auto &t : th) t.join();
+    t_end = *std::max_element(ends.begin(), ends.end());
+    const auto t_h2d = *std::min_element(firsts.begin(), firsts.end());
+    if (stats) {
+        stats->seconds = std::chrono::duration<double>(t_end - t_h2d).count();
+        stats->seconds_from_fill = std::chrono::duration<double>(t_end - t0).count();
+        stats->sub_batches = nb;
+        stats->cells = cells;
+    }
+    return errors.load() ? fail("gasal2_stream_align: a sub-batch failed") : 0;
+}
+
+extern "C" int32_t gasal2_pairs_align(const gasal2_pairs_t *pairs, gasal2_results_t *out, Parameters *params,
+                                      uint32_t batch_pairs, int32_t n_storages, gasal2_stream_stats_t *stats) {
+    if (!pairs || !out || !params) return fail("gasal2_pairs_align: NULL argument");
+    uint32_t maxq = 0, maxt = 0;
+    for (uint32_t k = 0; k < pairs->n; k++) {
+        maxq = std::max(maxq, pairs->query_lens[k]);
+        maxt = std::max(maxt, pairs->target_lens[k]);
+    }
+    gasal2_stream_t *st = gasal2_stream_new(params, batch_pairs, n_storages, maxq, maxt);
+    if (!st) return -1;
+    const int32_t r = gasal2_stream_align(st, pairs, out, stats);
+    gasal2_stream_free(st);
+    return r;
+}

@@ -1,0 +1,100 @@
+/*
+ * gasal2_stream.h -- the host input path of the batched pipeline: paired
+ * FASTA / FASTQ files read into padded host batches, streamed through several
+ * storages by one host thread each (fill the storage's pinned pages,
+ * gasal_aln_async, wait, copy the results out), results gathered per pair.
+ *
+ * The reference has no library counterpart: its example program does this
+ * work inline (reading test_prog/test_prog.cpp:79-137, batching and threads
+ * :237-330, paging src/host_batch.cpp:79-153).  Everything here is built on the
+ * public API of gasal_align.h / ctors.h / host_batch.h.
+ */
+#ifndef GASAL2_STREAM_H
+#define GASAL2_STREAM_H
+
+#include "gasal.h"
+#include "gasal2_ext.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Pairs in host memory, laid out as the reference's host batches: every
+ * sequence padded with N_CODE to a multiple of 8 bases, offsets and lengths
+ * in bases.  query_words / target_words, when not NULL, hold the same batch as
+ * 4-bit packed words (what an isPacked caller hands over).  Names are
+ * optional. */
+typedef struct {
+    uint8_t *query, *target;
+    uint64_t query_bytes, target_bytes;
+    uint32_t *query_words, *target_words;
+    uint32_t *query_offsets, *target_offsets;
+    uint32_t *query_lens, *target_lens;
+    uint8_t *query_ops, *target_ops; /* operation_on_seq per pair */
+    char **query_names, **target_names;
+    uint32_t n;
+} gasal2_pairs_t;
+
+/* gasal2_pairs_read flags */
+#define GASAL2_READ_NAMES 1u  /* keep the header names */
+#define GASAL2_READ_PACK 2u   /* also fill query_words / target_words */
+
+/* Reads paired files of the same number of records.  FASTA ('>' headers; as
+ * in the reference's example program a header starting with '<', '/' or '+'
+ * instead selects REVERSE_NATURAL, FORWARD_COMPLEMENT or REVERSE_COMPLEMENT
+ * for that sequence, test_prog/test_prog.cpp:79-137) or FASTQ ('@' records of
+ * four lines; qualities are skipped).  Sequence lines may wrap.  Returns NULL
+ * after a message on stderr if a file cannot be read or the record counts
+ * differ.  Free with gasal2_pairs_free. */
+gasal2_pairs_t *gasal2_pairs_read(const char *query_path, const char *target_path, uint32_t flags);
+void gasal2_pairs_free(gasal2_pairs_t *pairs);
+
+/* Host result arrays of pairs->n entries each; NULL = not wanted.  cigar:
+ * per pair a slot of cigar_cap bytes holding its CIGAR bytes in the
+ * reference's format (op | count << 2, reverse order), n_cigar_ops[k] of
+ * them, clipped to the slot. */
+typedef struct {
+    int32_t *score, *query_end, *target_end, *query_start, *target_start;
+    int32_t *second_score, *second_query_end, *second_target_end;
+    uint32_t *n_cigar_ops;
+    uint8_t *cigar;
+    uint32_t cigar_cap;
+} gasal2_results_t;
+
+typedef struct {
+    double seconds;            /* first gasal_aln_async (its H2D) to the last results copied out */
+    double seconds_from_fill;  /* first host fill to the last results copied out */
+    uint32_t sub_batches;
+    uint64_t cells;            /* sum of query length x target length */
+} gasal2_stream_stats_t;
+
+/* A set of storages on the caller's current device, kept across calls so
+ * their device workspace is allocated once: n_storages storages (each a HIP
+ * stream) sized for sub-batches of batch_pairs pairs of at most the given
+ * lengths (longer batches grow them, as gasal_aln_async does).  n_storages 3
+ * uses the four hardware queues a process gets without two streams sharing
+ * one.  The Parameters must outlive the stream. */
+typedef struct gasal2_stream gasal2_stream_t;
+gasal2_stream_t *gasal2_stream_new(Parameters *params, uint32_t batch_pairs, int32_t n_storages,
+                                   uint32_t max_query_len, uint32_t max_target_len);
+void gasal2_stream_free(gasal2_stream_t *stream);
+
+/* Aligns every pair: sub-batches of the stream's batch_pairs go round robin
+ * to its storages, each driven by its own host thread (fill the pinned pages,
+ * gasal_aln_async, wait, copy the results out).  With params->isPacked the
+ * packed words are handed over (pairs->query_words / target_words must be
+ * set).  KSW needs seed scores and is not streamed here.  Returns 0, or -1
+ * after a message on stderr. */
+int32_t gasal2_stream_align(gasal2_stream_t *stream, const gasal2_pairs_t *pairs, gasal2_results_t *out,
+                            gasal2_stream_stats_t *stats);
+
+/* gasal2_stream_new + gasal2_stream_align + gasal2_stream_free in one call
+ * (the workspace is allocated inside, so the first sub-batches pay for it). */
+int32_t gasal2_pairs_align(const gasal2_pairs_t *pairs, gasal2_results_t *out, Parameters *params,
+                           uint32_t batch_pairs, int32_t n_storages, gasal2_stream_stats_t *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -12,5 +12,6 @@
 #include "ctors.h"
 #include "interfaces.h"
 #include "gasal2_ext.h"
+#include "gasal2_stream.h"
 
 #endif
