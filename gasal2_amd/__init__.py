@@ -59,6 +59,8 @@ class operation_on_seq(enum.IntEnum):    # src/gasal.h:67-72
 globals().update({m.name: m for e in (comp_start, Bool, data_source, algo_type, operation_on_seq) for m in e})
 
 N_CODE_DEFAULT = 0x4E
+CIGAR_HARNESS, CIGAR_SAM = 0, 1          # include/gasal2_sam.h
+SAM_EQX, SAM_NO_SEQ, SAM_HEADER = 1, 2, 4
 
 
 # --------------------------------------------------------------- structs ---
@@ -125,7 +127,15 @@ class gasal2_results_t(C.Structure):      # include/gasal2_stream.h
 
 class gasal2_stream_stats_t(C.Structure):  # include/gasal2_stream.h
     _fields_ = [("seconds", C.c_double), ("seconds_from_fill", C.c_double), ("sub_batches", C.c_uint32),
-                ("cells", C.c_uint64)]
+                ("cells", C.c_uint64), ("fill_seconds", C.c_double), ("launch_seconds", C.c_double),
+                ("wait_seconds", C.c_double), ("copy_seconds", C.c_double)]
+
+
+class gasal2_sam_in_t(C.Structure):      # include/gasal2_sam.h
+    _fields_ = [("qname", C.c_char_p), ("rname", C.c_char_p), ("query", _P8), ("target", _P8),
+                ("query_len", C.c_uint32), ("target_len", C.c_uint32), ("query_op", C.c_uint8), ("target_op", C.c_uint8),
+                ("score", C.c_int32), ("query_end", C.c_int32), ("target_end", C.c_int32),
+                ("has_second", C.c_int32), ("second_score", C.c_int32), ("cigar", _P8), ("n_cigar_ops", C.c_uint32)]
 
 
 class gasal2_params_t(C.Structure):       # include/gasal2_ext.h
@@ -197,8 +207,17 @@ _SIGS = {
                                        C.c_int32, C.POINTER(gasal2_stream_stats_t)]),
     "gasal2_stream_new": (C.c_void_p, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
     "gasal2_stream_free": (None, [C.c_void_p]),
+    "gasal2_stream_set_order": (None, [C.c_void_p, C.c_int32]),
+    "gasal2_order_new": (C.c_void_p, [C.c_int32]),
+    "gasal2_order_free": (None, [C.c_void_p]),
+    "gasal2_storage_set_order": (None, [C.c_void_p, C.c_void_p]),
     "gasal2_stream_align": (C.c_int32, [C.c_void_p, C.POINTER(gasal2_pairs_t), C.POINTER(gasal2_results_t),
                                         C.POINTER(gasal2_stream_stats_t)]),
+    "gasal2_cigar_text": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_int32, C.c_char_p, C.c_uint32]),
+    "gasal2_sam_record": (C.c_int32, [C.POINTER(gasal2_sam_in_t), C.c_int32, C.c_int32, C.c_uint32, C.c_char_p,
+                                      C.c_uint32]),
+    "gasal2_sam_write": (C.c_int32, [C.c_char_p, C.POINTER(gasal2_pairs_t), C.POINTER(gasal2_results_t), C.c_int32,
+                                     C.c_uint32, C.c_int32]),
     "gasal2_aln_noalloc": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32] + [C.c_void_p] * 6 +
                            [C.c_uint32] * 3 + [C.c_void_p] * 9 + [C.c_void_p, C.c_void_p]),
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
@@ -844,8 +863,15 @@ class PairSet:
             r = L.gasal2_pairs_align(C.byref(self.p), C.byref(R), params.native(), batch_pairs, storages, C.byref(st))
         if r != 0:
             raise RuntimeError("gasal2_stream_align failed")
-        return out, {"seconds": st.seconds, "seconds_from_fill": st.seconds_from_fill,
-                     "sub_batches": st.sub_batches, "cells": st.cells}
+        out.native = (R, keep, bool(params.isReverseComplement))
+        return out, {k: getattr(st, k) for k, _ in gasal2_stream_stats_t._fields_}
+
+    def write_sam(self, path, results, flags=SAM_HEADER, threads=0):
+        """gasal2_sam_write: the SAM records of `results` (what align()
+        returned, with cigar_cap > 0 for CIGARs) into `path`."""
+        R, _, rc = results.native
+        if lib().gasal2_sam_write(os.fsencode(str(path)), C.byref(self.p), C.byref(R), int(rc), flags, threads) != 0:
+            raise RuntimeError(f"gasal2_sam_write failed on {path}")
 
     def close(self):
         if self._ptr is not None:
@@ -870,6 +896,11 @@ class Stream:
         if not self.h:
             raise RuntimeError("gasal2_stream_new failed")
 
+    def set_order(self, mode):
+        """gasal2_stream_set_order: 2 (default) sub-batches' H2D and device
+        programs in submission order, 1 H2D only, 0 independent streams."""
+        lib().gasal2_stream_set_order(self.h, mode)
+
     def close(self):
         if self.h:
             lib().gasal2_stream_free(self.h)
@@ -880,6 +911,40 @@ class Stream:
             self.close()
         except Exception:
             pass
+
+
+def cigar_text(cg, style=CIGAR_SAM):
+    """gasal2_cigar_text: forward CIGAR text of the reverse-order CIGAR bytes."""
+    cg = bytes(cg)
+    n = lib().gasal2_cigar_text(cg, len(cg), style, None, 0)
+    if n < 0:
+        raise ValueError(f"unknown CIGAR style {style}")
+    buf = C.create_string_buffer(n + 1)
+    lib().gasal2_cigar_text(cg, len(cg), style, buf, n + 1)
+    return buf.value.decode()
+
+
+def sam_record(query, target, score, query_end=-1, target_end=-1, cigar=b"", qname="q", rname="t", query_op=0,
+               target_op=0, second_score=None, rc_enabled=False, n_code=N_CODE_DEFAULT, flags=0):
+    """gasal2_sam_record: one SAM line (with its newline) for one pair."""
+    q, t, cg = bytes(query), bytes(target), bytes(cigar)
+    I = gasal2_sam_in_t()
+    I.qname, I.rname = (qname.encode() if qname is not None else None), (rname.encode() if rname is not None else None)
+    I.query = C.cast(C.c_char_p(q), _P8)
+    I.target = C.cast(C.c_char_p(t), _P8)
+    I.query_len, I.target_len = len(q), len(t)
+    I.query_op, I.target_op = query_op, target_op
+    I.score, I.query_end, I.target_end = score, query_end, target_end
+    I.has_second, I.second_score = (0, 0) if second_score is None else (1, second_score)
+    I.cigar = C.cast(C.c_char_p(cg), _P8) if cg else None
+    I.n_cigar_ops = len(cg)
+    L = lib()
+    n = L.gasal2_sam_record(C.byref(I), int(rc_enabled), n_code, flags, None, 0)
+    if n < 0:
+        raise ValueError("gasal2_sam_record rejected its input")
+    buf = C.create_string_buffer(n + 1)
+    L.gasal2_sam_record(C.byref(I), int(rc_enabled), n_code, flags, buf, n + 1)
+    return buf.value.decode()
 
 
 def cigar_to_string(cg):

@@ -3,7 +3,8 @@
     libgasal2_amd.so    HIP kernels (gfx950) + the host runtime of the C/C++ API
                         (kernels.hip: the alignment pipeline; band.hip: learned band;
                         noalloc.hip: the entry point that allocates nothing;
-                        stream.cpp: FASTA/FASTQ reader and the multi-storage driver)
+                        stream.cpp: FASTA/FASTQ reader and the multi-storage driver,
+                        sam.cpp: CIGAR text and SAM records)
     libgasal2_synth.so  seeded synthetic pair generator (host C++/OpenMP)
     test_prog           the reference harness re-implemented on the API
     c_host              a C99 host of the API (the public headers' C view)
@@ -42,7 +43,7 @@ def build_synth():
 def build_lib():
     out = os.path.join(HERE, "libgasal2_amd.so")
     objs = []
-    for src in ("kernels.hip", "band.hip", "noalloc.hip", "runtime.cpp", "stream.cpp"):
+    for src in ("kernels.hip", "band.hip", "noalloc.hip", "runtime.cpp", "stream.cpp", "sam.cpp"):
         obj = os.path.join(CSRC, "_build", src + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         lang = ["-x", "hip"] if src.endswith(".hip") else []

@@ -14,10 +14,18 @@
 #include <math.h>
 #include <string>
 #include <vector>
+#include <atomic>
 #include <mutex>
 #include <algorithm>
 
 // ------------------------------------------------------------ globals ------
+// An order chain (the batch order below).
+struct gasal2_order {
+    std::mutex mu;
+    std::atomic<int> mode{0};
+    hipEvent_t last_h2d = nullptr, last_dev = nullptr;
+};
+
 namespace {
 
 G2Scores g_scores = {0, 0, 0, 0, 0, (uint32_t)(N_CODE & 0xF), N_CODE,
@@ -41,14 +49,14 @@ uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
 int g_tb_fused = 0;  // checkpointed lane path: the DP kernel walks each pair's traceback itself
 int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
-// Batch order across storages (gasal2_set_batch_order; default off, the
-// reference's independent streams): 1 makes each gasal_aln_async's H2D wait
-// for the previously submitted batch's H2D, 2 also its device program for the
-// previous batch's program.  The events belong to the storages; the last
-// ones are remembered here.
-int g_batch_order = 0;
-std::mutex g_order_mu;
-hipEvent_t g_last_h2d = nullptr, g_last_dev = nullptr;
+// Batch order across storages (default off, the reference's independent
+// streams).  The storages on one order chain submit in order: mode 1 makes
+// each gasal_aln_async's H2D wait for the previously submitted batch's H2D,
+// mode 2 also its device program for that batch's program.  The events belong
+// to the storages; a chain remembers the last ones.  Storages not attached to
+// a chain of their own (gasal2_storage_set_order) use the process-wide chain
+// (gasal2_set_batch_order).
+gasal2_order g_chain;
 int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16; 2: two pairs per lane for every LOCAL mode
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
 
@@ -93,6 +101,7 @@ struct Impl {
     // batch order across storages (gasal2_set_batch_order): this storage's
     // last H2D and last device program, for the next submitted batch to wait on
     hipEvent_t ev_h2d = nullptr, ev_dev = nullptr;
+    gasal2_order *chain = nullptr;   // nullptr: the process-wide chain
     // captured device program of the last batch shape (see run_async)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -222,9 +231,14 @@ extern "C" void gasal2_set_tb_fused(int32_t on) { g_tb_fused = on ? 1 : 0; }
 extern "C" void gasal2_set_panels(int32_t on) { g_panels = on ? 1 : 0; }
 extern "C" int32_t gasal2_last_paths(gasal_gpu_storage_t *s) { return s && s->impl ? impl_of(s)->last_paths : 0; }
 extern "C" void gasal2_set_batch_order(int32_t on) {
-    std::lock_guard<std::mutex> lk(g_order_mu);
-    g_batch_order = on < 0 ? 0 : (on > 2 ? 2 : on);
-    g_last_h2d = g_last_dev = nullptr;
+    std::lock_guard<std::mutex> lk(g_chain.mu);
+    g_chain.mode = on < 0 ? 0 : (on > 2 ? 2 : on);
+    g_chain.last_h2d = g_chain.last_dev = nullptr;
+}
+extern "C" gasal2_order_t *gasal2_order_new(int32_t mode) {
+    gasal2_order *o = new (std::nothrow) gasal2_order();
+    if (o) o->mode = mode < 0 ? 0 : (mode > 2 ? 2 : mode);
+    return o;
 }
 extern "C" void gasal2_set_pk16(int32_t mode) { g_pk16_mode = mode < 0 ? 0 : (mode > 2 ? 2 : mode); }
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
@@ -816,10 +830,21 @@ static void impl_free(Impl *m) {
 }
 
 static void order_forget(Impl *m) {
-    std::lock_guard<std::mutex> lk(g_order_mu);
-    if (g_last_h2d && g_last_h2d == m->ev_h2d) g_last_h2d = nullptr;
-    if (g_last_dev && g_last_dev == m->ev_dev) g_last_dev = nullptr;
+    gasal2_order *o = m->chain ? m->chain : &g_chain;
+    std::lock_guard<std::mutex> lk(o->mu);
+    if (o->last_h2d && o->last_h2d == m->ev_h2d) o->last_h2d = nullptr;
+    if (o->last_dev && o->last_dev == m->ev_dev) o->last_dev = nullptr;
 }
+
+extern "C" void gasal2_storage_set_order(gasal_gpu_storage_t *s, gasal2_order_t *o) {
+    if (!s || !s->impl) return;
+    Impl *m = impl_of(s);
+    if (m->chain == o) return;
+    order_forget(m);
+    m->chain = o;
+}
+
+extern "C" void gasal2_order_free(gasal2_order_t *o) { delete o; }
 
 // src/ctors.cpp:130-190
 extern "C" void gasal_destroy_streams(gasal_gpu_storage_v *gpu_storage_vec, Parameters *params) {
@@ -1586,25 +1611,27 @@ extern "C" void gasal_aln_async(gasal_gpu_storage_t *gpu_storage, const uint32_t
     if (!grow_device(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
     Impl *m = impl_of(gpu_storage);
     m->prepped = false;   // fresh words
-    std::unique_lock<std::mutex> lk(g_order_mu, std::defer_lock);
-    if (g_batch_order) {
+    gasal2_order *oc = m->chain ? m->chain : &g_chain;
+    const int order = oc->mode.load();
+    std::unique_lock<std::mutex> lk(oc->mu, std::defer_lock);
+    if (order) {
         lk.lock();
         if (!m->ev_h2d) G2_CHECK(hipEventCreateWithFlags(&m->ev_h2d, hipEventDisableTiming));
         if (!m->ev_dev) G2_CHECK(hipEventCreateWithFlags(&m->ev_dev, hipEventDisableTiming));
         if (t_failed) return;
-        if (g_last_h2d) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, g_last_h2d, 0));
+        if (oc->last_h2d) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, oc->last_h2d, 0));
     }
     upload_async(gpu_storage, actual_n_alns, params);
     if (t_failed) return;
-    if (g_batch_order) {
+    if (order) {
         G2_CHECK(hipEventRecord(m->ev_h2d, gpu_storage->str));
-        g_last_h2d = m->ev_h2d;
-        if (g_batch_order == 2 && g_last_dev) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, g_last_dev, 0));
+        oc->last_h2d = m->ev_h2d;
+        if (order == 2 && oc->last_dev) G2_CHECK(hipStreamWaitEvent(gpu_storage->str, oc->last_dev, 0));
     }
     if (!run_async(gpu_storage, actual_query_batch_bytes, actual_target_batch_bytes, actual_n_alns, params)) return;
-    if (g_batch_order) {
+    if (order) {
         G2_CHECK(hipEventRecord(m->ev_dev, gpu_storage->str));
-        g_last_dev = m->ev_dev;
+        oc->last_dev = m->ev_dev;
         lk.unlock();
     }
     download_async(gpu_storage, actual_query_batch_bytes, actual_n_alns, params);

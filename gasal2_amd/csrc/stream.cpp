@@ -223,6 +223,7 @@ struct gasal2_stream {
     uint32_t batch_pairs;
     int dev;
     std::vector<gasal_gpu_storage_v> storages;   // one storage each
+    gasal2_order_t *order = nullptr;             // the storages' order chain
 };
 
 extern "C" gasal2_stream_t *gasal2_stream_new(Parameters *params, uint32_t batch_pairs, int32_t n_storages,
@@ -239,13 +240,33 @@ extern "C" gasal2_stream_t *gasal2_stream_new(Parameters *params, uint32_t batch
         fail("gasal2_stream_new: no current device");
         return nullptr;
     }
+    st->order = gasal2_order_new(GASAL2_STREAM_ORDER_DEFAULT);
+    if (!st->order) {
+        delete st;
+        fail("gasal2_stream_new: out of host memory");
+        return nullptr;
+    }
     const uint32_t per_q = (max_query_len + 7) & ~7u, per_t = (max_target_len + 7) & ~7u;
     for (int k = 0; k < n_storages; k++) {
         gasal_gpu_storage_v v = gasal_init_gpu_storage_v(1);
         gasal_init_streams(&v, (int)per_q + 8, (int)per_t + 8, (int)batch_pairs, params);
+        gasal2_storage_set_order(&v.a[0], st->order);
         st->storages.push_back(v);
     }
     return st;
+}
+
+extern "C" void gasal2_stream_set_order(gasal2_stream_t *st, int32_t mode) {
+    if (!st) return;
+    // a fresh chain: the old one's last events may belong to batches in flight
+    gasal2_order_t *o = gasal2_order_new(mode);
+    if (!o) return;
+    for (gasal_gpu_storage_v &v : st->storages) {
+        (void)hipStreamSynchronize(v.a[0].str);
+        gasal2_storage_set_order(&v.a[0], o);
+    }
+    gasal2_order_free(st->order);
+    st->order = o;
 }
 
 extern "C" void gasal2_stream_free(gasal2_stream_t *st) {
@@ -254,6 +275,7 @@ extern "C" void gasal2_stream_free(gasal2_stream_t *st) {
         gasal_destroy_streams(&v, st->params);
         gasal_destroy_gpu_storage_v(&v);
     }
+    gasal2_order_free(st->order);
     delete st;
 }
 
@@ -288,6 +310,10 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
     std::atomic<int> errors{0};
     std::chrono::steady_clock::time_point t0, t_end;
     std::vector<std::chrono::steady_clock::time_point> ends(n_storages), firsts(n_storages);
+    std::vector<double> t_fill(n_storages, 0.0), t_launch(n_storages, 0.0), t_wait(n_storages, 0.0),
+        t_copy(n_storages, 0.0);
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
 
     auto worker = [&](int st) {
         (void)hipSetDevice(dev);
@@ -301,6 +327,7 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
         }
         for (uint32_t b = (uint32_t)st; b < nb && !errors.load(); b += (uint32_t)n_storages) {
             const uint32_t k0 = b * batch_pairs, k1 = std::min(n, k0 + batch_pairs), m = k1 - k0;
+            const clk::time_point c0 = clk::now();
             // one contiguous span per side when the pairs are in order
             // (gasal2_pairs_read lays them out so), else per-sequence fills
             bool contiguous = true;
@@ -358,13 +385,19 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
                 gasal_op_fill(s, pairs->target_ops + k0, m, TARGET);
             }
             s->current_n_alns = m;
+            const clk::time_point c1 = clk::now();
             if (first) {
-                firsts[st] = std::chrono::steady_clock::now();
+                firsts[st] = c1;
                 first = false;
             }
             gasal_aln_async(s, qb, tb, m, params);
+            const clk::time_point c2 = clk::now();
             int r;
             while ((r = gasal_is_aln_async_done(s)) == -1) std::this_thread::yield();
+            const clk::time_point c3 = clk::now();
+            t_fill[st] += secs(c0, c1);
+            t_launch[st] += secs(c1, c2);
+            t_wait[st] += secs(c2, c3);
             if (r != 0) {
                 errors++;
                 break;
@@ -388,10 +421,14 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
                 if (out->cigar && out->cigar_cap)
                     for (uint32_t k = 0; k < m; k++) {
                         const uint32_t slot = (pairs->query_lens[k0 + k] + 7) & ~7u;
-                        const uint32_t nb_ = std::min(std::min(h->n_cigar_ops[k], slot), out->cigar_cap);
-                        memcpy(out->cigar + (size_t)(k0 + k) * out->cigar_cap, h->cigar + s->host_query_batch_offsets[k], nb_);
+                        uint8_t *dst = out->cigar + (size_t)(k0 + k) * out->cigar_cap;
+                        if (h->n_cigar_ops[k] > slot && out->cigar_cap > slot)   // clipped in host_res: the whole CIGAR
+                            gasal2_get_cigar(s, k, dst, out->cigar_cap);
+                        else
+                            memcpy(dst, h->cigar + s->host_query_batch_offsets[k], std::min(std::min(h->n_cigar_ops[k], slot), out->cigar_cap));
                     }
             }
+            t_copy[st] += secs(c3, clk::now());
         }
         ends[st] = std::chrono::steady_clock::now();
         if (first) firsts[st] = ends[st];
@@ -414,6 +451,13 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
         stats->seconds_from_fill = std::chrono::duration<double>(t_end - t0).count();
         stats->sub_batches = nb;
         stats->cells = cells;
+        stats->fill_seconds = stats->launch_seconds = stats->wait_seconds = stats->copy_seconds = 0.0;
+        for (int k = 0; k < n_storages; k++) {
+            stats->fill_seconds += t_fill[k];
+            stats->launch_seconds += t_launch[k];
+            stats->wait_seconds += t_wait[k];
+            stats->copy_seconds += t_copy[k];
+        }
     }
     return errors.load() ? fail("gasal2_stream_align: a sub-batch failed") : 0;
 }

@@ -248,6 +248,15 @@ int32_t gasal2_aln_noalloc(const void *query_batch, const void *target_batch, ui
  * program after that batch's program.  Results are identical.  No reference
  * counterpart. */
 void gasal2_set_batch_order(int32_t mode);
+/* An order chain of its own for a set of storages (the same modes as
+ * gasal2_set_batch_order, applied among the storages attached to it instead of
+ * process-wide); NULL detaches a storage back to the process-wide chain.  Free
+ * a chain after its storages are destroyed or detached.  No reference
+ * counterpart. */
+typedef struct gasal2_order gasal2_order_t;
+gasal2_order_t *gasal2_order_new(int32_t mode);
+void gasal2_storage_set_order(gasal_gpu_storage_t *s, gasal2_order_t *order);
+void gasal2_order_free(gasal2_order_t *order);
 int32_t gasal2_get_graph_mode(void);
 void gasal2_graph_stats(gasal_gpu_storage_t *s, uint64_t *captures, uint64_t *launches);
 

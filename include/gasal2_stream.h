@@ -52,7 +52,8 @@ void gasal2_pairs_free(gasal2_pairs_t *pairs);
 /* Host result arrays of pairs->n entries each; NULL = not wanted.  cigar:
  * per pair a slot of cigar_cap bytes holding its CIGAR bytes in the
  * reference's format (op | count << 2, reverse order), n_cigar_ops[k] of
- * them, clipped to the slot. */
+ * them, clipped to the slot (whole when they fit, also when they exceed the
+ * pair's padded query length, where host_res clips them). */
 typedef struct {
     int32_t *score, *query_end, *target_end, *query_start, *target_start;
     int32_t *second_score, *second_query_end, *second_target_end;
@@ -66,6 +67,10 @@ typedef struct {
     double seconds_from_fill;  /* first host fill to the last results copied out */
     uint32_t sub_batches;
     uint64_t cells;            /* sum of query length x target length */
+    /* host time of the worker threads, summed over them: filling pages,
+     * inside gasal_aln_async (enqueueing), waiting for the storage, copying
+     * results out */
+    double fill_seconds, launch_seconds, wait_seconds, copy_seconds;
 } gasal2_stream_stats_t;
 
 /* A set of storages on the caller's current device, kept across calls so
@@ -78,6 +83,15 @@ typedef struct gasal2_stream gasal2_stream_t;
 gasal2_stream_t *gasal2_stream_new(Parameters *params, uint32_t batch_pairs, int32_t n_storages,
                                    uint32_t max_query_len, uint32_t max_target_len);
 void gasal2_stream_free(gasal2_stream_t *stream);
+
+/* The stream's storages share an order chain (gasal2_order_new): by default
+ * mode 2, so sub-batches take PCIe and run their device programs one after
+ * another in submission order (each program fills the device; the next
+ * sub-batch's H2D and the last one's D2H overlap it).  0 lets the storages'
+ * streams run independently.  Waits for work in flight.  Results are
+ * identical in every mode. */
+#define GASAL2_STREAM_ORDER_DEFAULT 2
+void gasal2_stream_set_order(gasal2_stream_t *stream, int32_t mode);
 
 /* Aligns every pair: sub-batches of the stream's batch_pairs go round robin
  * to its storages, each driven by its own host thread (fill the pinned pages,
