@@ -22,8 +22,9 @@ the same batch on this host and reports the GPU's agreement with it there.
 The default run also measures every other workload of the benchmark set under
 `configs` (each with its own agreement check against the oracle): the 10k-pair
 short-read batch, the full-feature batch's GLOBAL / SEMI_GLOBAL modes and its
-packed-input form, steady-state end to end over sub-batches (host batch to
-host results), the uneven-length batch, the seed-extension batch (KSW, both
+packed-input form, steady state end to end (four full-feature batches
+streamed through the library's stream driver, first H2D to the last results
+on the host; and FASTA files in, SAM out), the uneven-length batch, the seed-extension batch (KSW, both
 forms, and tile-banded) and the learned band on long divergent pairs.
 `--headline-only` skips them; `--workload` / `--mode` run one configuration.
 """
@@ -261,9 +262,10 @@ def main():
                     help="measure the headline configuration only (no other workloads)")
     ap.add_argument("--packed", action="store_true", help="one configuration with isPacked=true host inputs")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
-    ap.add_argument("--pipeline", type=int, default=4,
-                    help="steady state end to end: the full-feature batch in this many sub-batches through "
-                         "gasal_aln_async (host batch to host results); 0 = skip")
+    ap.add_argument("--stream-batches", type=int, default=4,
+                    help="steady state: full-feature batches streamed end to end (0: skip)")
+    ap.add_argument("--stream-sub", type=int, default=1_000_000, help="steady state: pairs per sub-batch")
+    ap.add_argument("--stream-storages", type=int, default=3, help="steady state: storages (host threads)")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--check-widen", type=int, nargs="+", default=[128],
                     help="check-band widening(s) (diagonals each side) to evaluate; the first one is reported as value")
@@ -476,7 +478,9 @@ def main():
     if tuned:
         line["autotune"] = tuned
     if not single and not args.headline_only:
-        line["configs"] = other_configs(g, args, batch, peak_tops)
+        line["configs"] = other_configs(g, args, batch, peak_tops,
+                                        res if workload == "full" and mode == "local_tb_second" and not args.packed
+                                        else None)
     line["bench_seconds"] = round(time.time() - t_start, 1)
     print(json.dumps(line), flush=True)
 
@@ -496,7 +500,7 @@ def step_line(g, run, mode, peak_tops, threads, m, kw=-1, kz=0, cigar_pairs=0, p
     return out
 
 
-def other_configs(g, args, full, peak_tops):
+def other_configs(g, args, full, peak_tops, headline_res=None):
     """Every other workload of the benchmark set, each with its agreement
     against the oracle (on all pairs where the oracle finishes in seconds,
     else on a prefix)."""
@@ -538,11 +542,19 @@ def other_configs(g, args, full, peak_tops):
                 r.close()
         timed(name, c2)
 
-    if args.pipeline > 1:
-        timed("full_steady_state_end_to_end", lambda: pipeline_measure(g, full, params_for(g, "local_tb_second", True),
-                                                                      args.pipeline))
+    if args.stream_batches > 0 and headline_res is not None:
+        sp = dict(batches=args.stream_batches, sub_pairs=args.stream_sub, storages=args.stream_storages, threads=thr)
+        timed("full_steady_state_end_to_end",
+              lambda: stream_measure(g, full, headline_res, params_for(g, "local_tb_second", True), **sp))
         timed("full_steady_state_end_to_end_packed_inputs",
-              lambda: pipeline_measure(g, full, params_for(g, "local_tb_second", True, True), args.pipeline))
+              lambda: stream_measure(g, full, headline_res, params_for(g, "local_tb_second", True, True), **sp))
+
+        def files():
+            import tempfile
+            with tempfile.TemporaryDirectory() as d:
+                return files_measure(g, full, params_for(g, "local_tb_second", True), args.stream_sub,
+                                     args.stream_storages, d)
+        timed("full_fasta_to_sam", files)
 
     # uneven lengths: persistent queue with the lane/wave switch vs one lane per pair
     def c3():
@@ -589,88 +601,106 @@ def other_configs(g, args, full, peak_tops):
     return out
 
 
-def pipeline_measure(g, batch, p, parts, storages=3, passes=2):
-    """Steady-state end to end: the batch cut into `parts` sub-batches and
-    streamed `passes` times through gasal_aln_async by `storages` host
-    threads, each owning one storage (one HIP stream) and taking every
-    `storages`-th sub-batch: fill the host batch, gasal_aln_async, poll
-    gasal_is_aln_async_done, copy the results out -- the reference's test
-    program pattern (test_prog/test_prog.cpp:237-330; one stream per storage,
-    src/ctors.cpp:118).  The clock runs from the first fill to the last
-    results on the host, so one sub-batch's fill, H2D and D2H overlap the
-    others' device work.  Three storages: the box gives a process four
-    hardware queues and a fourth stream would share one.  Every sub-batch's
-    outputs are compared with one whole-batch run."""
-    import threading
-    n = batch.n
-    cuts = np.linspace(0, n, parts + 1).astype(int)
-    subs = [batch.subset(np.arange(cuts[i], cuts[i + 1])) for i in range(parts)]
-    for sb in subs:   # a packed-input caller holds its words already
-        if p.isPacked:
-            sb.packed_words(True), sb.packed_words(False)
-    mq, mt, mn = int(batch.qlen.max()), int(batch.tlen.max()), int(max(s.n for s in subs))
-    keys = ["score", "qend", "tend", "qstart", "tstart"] + (["n_ops"] if p.start_pos == g.WITH_TB else [])
-    cells = float(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+def concat_batches(g, bs):
+    """One Batch holding the pairs of several, in order."""
+    qo, to = np.cumsum([0] + [len(b.q) for b in bs]), np.cumsum([0] + [len(b.t) for b in bs])
+    cat = lambda f: np.concatenate([getattr(b, f) for b in bs])
+    return g.Batch(cat("q"), cat("t"), np.concatenate([b.qoff + qo[i] for i, b in enumerate(bs)]),
+                   np.concatenate([b.toff + to[i] for i, b in enumerate(bs)]), cat("qlen"), cat("tlen"),
+                   cat("qop"), cat("top"))
 
-    def wait(a):
-        while True:
-            r = g.gasal_is_aln_async_done(a.s)
-            if r == 0:
-                return
-            if r != -1:
-                raise RuntimeError(f"gasal_is_aln_async_done returned {r}")
-            time.sleep(0.0002)   # let the other host threads run
 
-    def collect(a, m, qb):
-        r = a._collect(m, qb)
-        return {k: r[k] for k in keys}
-
-    a = g.Aligner(p, mq, mt, n, scores=(1, 4, 6, 1), n_penalty=1)
+def stream_measure(g, full, headline_res, p, batches, sub_pairs, storages, threads, reps=3, sample=96):
+    """Steady state end to end through the library's stream driver
+    (gasal2_stream_align, include/gasal2_stream.h): `batches` full-feature
+    batches -- the headline batch, then the same generator with other seeds --
+    streamed as sub-batches of `sub_pairs` pairs through `storages` storages,
+    each driven by its own host thread (fill the pinned pages, gasal_aln_async,
+    wait, copy the results out; the reference's example-program pattern,
+    test_prog/test_prog.cpp:237-330, one stream per storage src/ctors.cpp:118)
+    with the sub-batches' H2D and device programs in submission order, so one
+    sub-batch's H2D and D2H overlap another's device work.  Clock: first
+    gasal_aln_async (its H2D) to the last results on the host (also reported
+    from the first host fill).  Warm storages, median of `reps` runs.  The
+    headline batch's outputs are compared with its single-batch run on every
+    pair, the other batches' with the oracle on their first `sample` pairs."""
+    bs = [full] + [g.synth_batch(2, 20000 + i, full.n) for i in range(1, batches)]
+    big = concat_batches(g, bs)
+    ps = g.PairSet.from_batch(big, pack=bool(p.isPacked))
+    st = g.Stream(p, sub_pairs, storages, int(big.qlen.max()), int(big.tlen.max()))
+    runs = []
     try:
-        qb, tb, m = a.fill(batch, per_pair=False)
-        a.align_filled(qb, tb, m, poll_s=0)
-        whole = collect(a, m, qb)
+        r, _ = ps.align(p, n_penalty=1, stream=st)   # warm: workspace, graphs, first touch
+        for _ in range(reps):
+            r, stats = ps.align(p, n_penalty=1, stream=st)
+            runs.append(stats)
     finally:
-        a.close()
+        st.close()
+    keys = ["score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend", "s_tend", "n_ops"]
+    n = full.n
+    same = all(np.array_equal(r[k][:n], headline_res[k]) for k in keys if k in r and k in headline_res)
+    agree = []
+    for i in range(1, batches):
+        sub, ref, _ = oracle_run(g, bs[i], p, sample, threads, -1, 0)
+        agree.append(agreement(g, p, {k: v[i * n:(i + 1) * n] for k, v in r.items()}, ref, len(sub.qlen)))
+    secs = sorted(x["seconds"] for x in runs)
+    med = secs[len(secs) // 2]
+    best = min(runs, key=lambda x: abs(x["seconds"] - med))
+    cells = best["cells"]
+    res = {"pairs": big.n, "batches": batches, "sub_batch_pairs": sub_pairs, "storages": storages,
+           "sub_batches": best["sub_batches"], "packed_inputs": bool(p.isPacked),
+           "gcups": round(cells / med / 1e9, 1), "seconds": round(med, 4),
+           "gcups_runs": [round(cells / x / 1e9, 1) for x in secs],
+           "gcups_from_fill": round(cells / best["seconds_from_fill"] / 1e9, 1),
+           "host_seconds": {k: round(best[k], 4) for k in ("fill_seconds", "launch_seconds", "wait_seconds",
+                                                            "copy_seconds")},
+           "headline_batch_identical_to_single_batch": bool(same),
+           "other_batches_oracle_agreement": agree, "oracle_sample_pairs": sample,
+           "clock": "first gasal_aln_async (H2D) to the last results on the host; median of runs"}
+    log(f"stream: {res}")
+    return res
 
-    al = [g.Aligner(p, mq, mt, mn, scores=(1, 4, 6, 1), n_penalty=1) for _ in range(storages)]
-    seq = [k % parts for k in range(parts * passes)]
-    out = {}
-    errors = []
 
-    def worker(st):
-        try:
-            a = al[st]
-            for k in range(st, len(seq), storages):
-                qb, tb, m = a.fill(subs[seq[k]], per_pair=False)
-                g.gasal_aln_async(a.s, qb, tb, m, a.params)
-                wait(a)
-                out[k] = collect(a, m, qb)
-        except Exception as e:   # reported after the join
-            errors.append(e)
-
+def files_measure(g, full, p, sub_pairs, storages, tmpdir):
+    """The host input path from files: the headline batch written as two
+    FASTA files (op markers in the headers), read back by gasal2_pairs_read,
+    aligned through the stream driver and written as SAM (gasal2_sam_write);
+    each stage timed."""
+    qp, tp, sp = (os.path.join(tmpdir, x) for x in ("q.fa", "t.fa", "out.sam"))
+    for path, data, offs, lens, ops in ((qp, full.q, full.qoff, full.qlen, full.qop),
+                                        (tp, full.t, full.toff, full.tlen, full.top)):
+        with open(path, "wb") as f:
+            mk = b"></+"
+            raw = data.tobytes()
+            f.write(b"".join(mk[o:o + 1] + b"p%d\n" % k + raw[a:a + l] + b"\n"
+                             for k, (a, l, o) in enumerate(zip(offs.tolist(), lens.tolist(), ops.tolist()))))
+    fasta_bytes = os.path.getsize(qp) + os.path.getsize(tp)
+    t0 = time.perf_counter()
+    ps = g.PairSet.read(qp, tp, names=True, pack=bool(p.isPacked))
+    t_read = time.perf_counter() - t0
+    st = g.Stream(p, sub_pairs, storages, int(full.qlen.max()), int(full.tlen.max()))
     try:
-        for i, a in enumerate(al):   # warm: workspace and graph per storage
-            a.align_filled(*a.fill(subs[i % parts], per_pair=False), poll_s=0)
-        th = [threading.Thread(target=worker, args=(st,)) for st in range(storages)]
+        cap = int(full.qlen.max() + full.tlen.max()) + 8
+        ps.align(p, n_penalty=1, stream=st, cigar_cap=cap)   # warm
         t0 = time.perf_counter()
-        for t_ in th:
-            t_.start()
-        for t_ in th:
-            t_.join()
-        t = time.perf_counter() - t0
+        r, stats = ps.align(p, n_penalty=1, stream=st, cigar_cap=cap)
+        t_align = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ps.write_sam(sp, r)
+        t_sam = time.perf_counter() - t0
     finally:
-        for a in al:
-            a.close()
-    if errors:
-        raise errors[0]
-    same = all(np.array_equal(np.concatenate([out[k][kk] for k in range(parts * pp, parts * (pp + 1))]), whole[kk])
-               for pp in range(passes) for kk in keys)
-    res = {"sub_batches": len(seq), "sub_batch_pairs": int(mn), "storages": storages, "host_threads": storages,
-           "pairs_streamed": n * passes, "seconds": round(t, 4), "gcups": round(passes * cells / t / 1e9, 2),
-           "outputs_identical_to_one_batch": bool(same), "packed_inputs": bool(p.isPacked),
-           "clock": "first host fill to last results on the host"}
-    log(f"pipeline: {res}")
+        st.close()
+        ps.close()
+    sam_bytes = os.path.getsize(sp)
+    for x in (qp, tp, sp):
+        os.remove(x)
+    cells = stats["cells"]
+    res = {"pairs": full.n, "fasta_bytes": fasta_bytes, "read_s": round(t_read, 3),
+           "read_gb_s": round(fasta_bytes / t_read / 1e9, 2), "align_s": round(t_align, 4),
+           "align_gcups": round(cells / t_align / 1e9, 1), "sam_bytes": sam_bytes, "sam_write_s": round(t_sam, 3),
+           "sam_gb_s": round(sam_bytes / t_sam / 1e9, 2),
+           "files_to_sam_gcups": round(cells / (t_read + t_align + t_sam) / 1e9, 1)}
+    log(f"files: {res}")
     return res
 
 

@@ -116,7 +116,21 @@ class gasal2_pairs_t(C.Structure):        # include/gasal2_stream.h
                 ("query_words", _P32), ("target_words", _P32),
                 ("query_offsets", _P32), ("target_offsets", _P32), ("query_lens", _P32), ("target_lens", _P32),
                 ("query_ops", _P8), ("target_ops", _P8),
-                ("query_names", C.POINTER(C.c_char_p)), ("target_names", C.POINTER(C.c_char_p)), ("n", C.c_uint32)]
+                ("query_names", C.POINTER(C.c_char_p)), ("target_names", C.POINTER(C.c_char_p)), ("n", C.c_uint32),
+                ("seed_scores", _P32)]
+
+
+class gasal2_seed_t(C.Structure):         # include/gasal2_stream.h
+    _fields_ = [("query_beg", C.c_uint32), ("target_beg", C.c_uint32), ("len", C.c_uint32), ("score", C.c_int32)]
+
+
+class gasal2_extend_opts_t(C.Structure):  # include/gasal2_stream.h
+    _fields_ = [(k, C.c_int32) for k in ("w", "zdrop", "end_bonus_left", "end_bonus_right")]
+
+
+class gasal2_extend_out_t(C.Structure):   # include/gasal2_stream.h
+    _fields_ = [(k, C.POINTER(C.c_int32)) for k in ("score", "query_beg", "query_end", "target_beg", "target_end",
+                                                      "left_score", "right_score")]
 
 
 class gasal2_results_t(C.Structure):      # include/gasal2_stream.h
@@ -208,6 +222,9 @@ _SIGS = {
     "gasal2_stream_new": (C.c_void_p, [C.c_void_p, C.c_uint32, C.c_int32, C.c_uint32, C.c_uint32]),
     "gasal2_stream_free": (None, [C.c_void_p]),
     "gasal2_stream_set_order": (None, [C.c_void_p, C.c_int32]),
+    "gasal2_extend_seeds": (C.c_int32, [C.c_void_p, C.POINTER(gasal2_pairs_t), C.POINTER(gasal2_seed_t),
+                                        C.POINTER(gasal2_extend_opts_t), C.POINTER(gasal2_extend_out_t),
+                                        C.POINTER(gasal2_stream_stats_t)]),
     "gasal2_order_new": (C.c_void_p, [C.c_int32]),
     "gasal2_order_free": (None, [C.c_void_p]),
     "gasal2_storage_set_order": (None, [C.c_void_p, C.c_void_p]),
@@ -223,6 +240,8 @@ _SIGS = {
     "gasal2_set_tile_budget": (None, [C.c_uint64]),
     "gasal2_set_pk16": (None, [C.c_int32]),
     "gasal2_set_ksw_extension": (None, [C.c_int32, C.c_int32]),
+    "gasal2_set_ksw_end_bonus": (None, [C.c_int32]),
+    "gasal2_storage_set_ksw": (None, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "gasal2_set_tb_checkpoint": (None, [C.c_int32]),
     "gasal2_set_tb_fused": (None, [C.c_int32]),
     "gasal2_set_panels": (None, [C.c_int32]),
@@ -375,11 +394,13 @@ def set_n_code(n_code):
     lib().gasal2_set_n_code(n_code)
 
 
-def set_ksw_extension(w=-1, zdrop=0):
-    """KSW seed-extension settings: band half-width w (< 0: none) and z-drop
-    (0: off).  The defaults reproduce the reference's KSW kernel, which has
-    neither (src/kernels/ksw_kernel_template.h:27-29)."""
+def set_ksw_extension(w=-1, zdrop=0, end_bonus=5):
+    """KSW seed-extension settings: band half-width w (< 0: none), z-drop
+    (0: off) and end bonus.  The defaults reproduce the reference's KSW
+    kernel: no band, z-drop 0, PEN_CLIP5 = 5
+    (src/kernels/ksw_kernel_template.h:20,27-29)."""
     lib().gasal2_set_ksw_extension(int(w), int(zdrop))
+    lib().gasal2_set_ksw_end_bonus(int(end_bonus))
 
 
 def set_n_penalty(penalty):
@@ -786,6 +807,7 @@ class PairSet:
             P.query_offsets, P.target_offsets = arr(batch.qoff, np.uint32, C.c_uint32), arr(batch.toff, np.uint32, C.c_uint32)
             P.query_lens, P.target_lens = arr(batch.qlen, np.uint32, C.c_uint32), arr(batch.tlen, np.uint32, C.c_uint32)
             P.query_ops, P.target_ops = arr(batch.qop, np.uint8, C.c_uint8), arr(batch.top, np.uint8, C.c_uint8)
+            P.seed_scores = arr(batch.seeds, np.uint32, C.c_uint32)
             P.n = batch.n
             self.p = P
 
@@ -872,6 +894,30 @@ class PairSet:
         R, _, rc = results.native
         if lib().gasal2_sam_write(os.fsencode(str(path)), C.byref(self.p), C.byref(R), int(rc), flags, threads) != 0:
             raise RuntimeError(f"gasal2_sam_write failed on {path}")
+
+    def extend(self, stream, query_beg, target_beg, length, score, w=-1, zdrop=0, end_bonus=(5, 5), scores=(1, 4, 6, 1),
+               n_code=N_CODE_DEFAULT, n_penalty=None):
+        """gasal2_extend_seeds: two-sided extension of one seed per pair on
+        `stream` (a Stream made with algo = KSW); returns (dict, stats)."""
+        set_n_code(n_code)
+        set_n_penalty(n_penalty)
+        gasal_copy_subst_scores(gasal_subst_scores(*scores))
+        n = self.n
+        seeds = (gasal2_seed_t * max(n, 1))()
+        for k, (a, b, l, sc) in enumerate(zip(query_beg, target_beg, length, score)):
+            seeds[k] = gasal2_seed_t(int(a), int(b), int(l), int(sc))
+        out, keep = {}, []
+        O = gasal2_extend_out_t()
+        for name in ("score", "query_beg", "query_end", "target_beg", "target_end", "left_score", "right_score"):
+            a = np.zeros(n, np.int32)
+            out[name] = a
+            keep.append(a)
+            setattr(O, name, a.ctypes.data_as(C.POINTER(C.c_int32)))
+        opts = gasal2_extend_opts_t(int(w), int(zdrop), int(end_bonus[0]), int(end_bonus[1]))
+        st = gasal2_stream_stats_t()
+        if lib().gasal2_extend_seeds(stream.h, C.byref(self.p), seeds, C.byref(opts), C.byref(O), C.byref(st)) != 0:
+            raise RuntimeError("gasal2_extend_seeds failed")
+        return out, {k: getattr(st, k) for k, _ in gasal2_stream_stats_t._fields_}
 
     def close(self):
         if self._ptr is not None:

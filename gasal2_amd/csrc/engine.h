@@ -24,6 +24,7 @@ struct G2Opts {
     int32_t skip_prep;   // batch already packed and reverse-complemented on the device
     int32_t prep_only;   // pack and reverse-complement only
     int32_t ksw_w, ksw_zdrop;   // KSW band half-width (< 0: none) and z-drop (0: off), the reference's defaults
+    int32_t ksw_eb;             // KSW end bonus (the reference: PEN_CLIP5 = 5)
 };
 
 // Device view of one batch plus the workspace it runs in.

@@ -2178,7 +2178,7 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // the host checked h0 + match * min(Lq, Lt) <= 65535), halving the row's traffic.
 // LUT: scores from the row's table (batch census allows it), else per cell.
 template <bool EH16, bool LUT>
-__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop) {
+__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop, int32_t end_bonus) {
     const uint32_t census = d.aux[0];
     if (LUT != (d.fast && census_fast(census))) return;
     LaneCtx x;
@@ -2191,13 +2191,21 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
     const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
     int2 *__restrict__ eh2 = reinterpret_cast<int2 *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
     uint32_t *__restrict__ eh1 = reinterpret_cast<uint32_t *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
+    // The row streams through L2 with non-temporal hints: the rows of the
+    // resident waves exceed it, and cached they would evict the query words
+    // every lane re-reads on each row.
+    uint64_t *__restrict__ eh8 = reinterpret_cast<uint64_t *>(eh2);
     auto ld = [&](int k) -> int2 {
-        if (EH16) { const uint32_t u = eh1[(size_t)k * WAVE]; return make_int2((int)(u & 0xFFFFu), (int)(u >> 16)); }
-        return eh2[(size_t)k * WAVE];
+        if (EH16) {
+            const uint32_t u = __builtin_nontemporal_load(eh1 + (size_t)k * WAVE);
+            return make_int2((int)(u & 0xFFFFu), (int)(u >> 16));
+        }
+        const uint64_t u = __builtin_nontemporal_load(eh8 + (size_t)k * WAVE);
+        return make_int2((int)(uint32_t)u, (int)(uint32_t)(u >> 32));
     };
     auto st = [&](int k, int2 v) {
-        if (EH16) eh1[(size_t)k * WAVE] = (uint32_t)v.x | ((uint32_t)v.y << 16);
-        else eh2[(size_t)k * WAVE] = v;
+        if (EH16) __builtin_nontemporal_store((uint32_t)v.x | ((uint32_t)v.y << 16), eh1 + (size_t)k * WAVE);
+        else __builtin_nontemporal_store((uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32), eh8 + (size_t)k * WAVE);
     };
     for (uint32_t jj = 0; jj <= qlen + 1; jj++) st((int)jj, make_int2(0, 0));
     int j;
@@ -2207,7 +2215,7 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
     int max = (int32_t)h0, max_i = -1, max_j = -1, max_ie = -1, gscore = -1;
     int beg = 0, end = (int)qlen;
     if (w >= 0) {   // band extension: w clamped as bwa's ksw_extend2 does (the reference has no band)
-        int mx = (int)((double)((int)qlen * (c.match > 0 ? c.match : 0) - c.gapo) / c.gape + 1.0);
+        int mx = (int)((double)((int)qlen * (c.match > 0 ? c.match : 0) + end_bonus - c.gapo) / c.gape + 1.0);
         mx = mx > 1 ? mx : 1;
         w = w < mx ? w : mx;
     }
@@ -2245,12 +2253,13 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // row (coalesced).
             uint32_t rpac = 0;
             // the row entry of the next column is loaded one step ahead (it was
-            // written by the previous row; this row writes it only at its step)
-            int2 nx = jlo < end ? ld(jlo) : make_int2(0, 0);
+            // written by the previous row; this row writes it only at its
+            // step); a lane loads only its own columns [beg, end)
+            int2 nx = jlo < end && jlo >= beg ? ld(jlo) : make_int2(0, 0);
             for (int jj = jlo; jj < end; jj++) {
                 if ((jj & 7) == 0) rpac = x.Q[jj >> 3];
                 int2 pv = nx;
-                if (jj + 1 < end) nx = ld(jj + 1);
+                if (jj + 1 < end && jj + 1 >= beg) nx = ld(jj + 1);
                 if (jj < beg) continue;
                 const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
                 int h, M = pv.x, e = pv.y;
@@ -2297,7 +2306,7 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
         }
     }
     const uint32_t pr = x.pair;
-    if (gscore <= 0 || gscore <= max - 5) {                                                // :188-197
+    if (gscore <= 0 || gscore <= max - end_bonus) {                                        // :188-197 (PEN_CLIP5 there)
         d.score[pr] = max; d.qend[pr] = max_j + 1; d.tend[pr] = max_i + 1;
     } else {
         d.score[pr] = gscore; d.qend[pr] = (int32_t)qlen; d.tend[pr] = max_ie + 1;
@@ -2810,11 +2819,11 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             break;
         case A_KSW:
             if (d.ksw16) {
-                k_ksw<true, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
-                k_ksw<true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+                k_ksw<true, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
+                k_ksw<true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
             } else {
-                k_ksw<false, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
-                k_ksw<false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop);
+                k_ksw<false, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
+                k_ksw<false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
             }
             break;
         default:

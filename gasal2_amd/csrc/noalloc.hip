@@ -176,6 +176,7 @@ extern "C" int32_t gasal2_aln_noalloc(const void *query_batch, const void *targe
     op.k_band = params->k_band;
     op.is_rc = params->isReverseComplement ? 1 : 0;
     op.ksw_w = -1;
+    op.ksw_eb = 5;
     // packed inputs are copied into the scratch, so reverse/complement never
     // rewrites the caller's words
     op.is_packed = 1;

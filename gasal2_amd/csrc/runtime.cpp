@@ -59,6 +59,7 @@ int g_panels = 1;    // transposed sequence panels for the fast LOCAL lane path
 gasal2_order g_chain;
 int g_pk16_mode = 1;    // packed 16-bit kernels (and KSW's packed row) when values fit int16; 2: two pairs per lane for every LOCAL mode
 int32_t g_ksw_w = -1, g_ksw_zdrop = 0;   // KSW extension (reference: no band, z-drop 0)
+int32_t g_ksw_eb = 5;                    // KSW end bonus (reference: PEN_CLIP5)
 
 // Tuned switch thresholds per (algo, start, second best, mean query length
 // range); filled by the autotuner (gasal2_tune_set), consulted per batch.
@@ -102,6 +103,9 @@ struct Impl {
     // last H2D and last device program, for the next submitted batch to wait on
     hipEvent_t ev_h2d = nullptr, ev_dev = nullptr;
     gasal2_order *chain = nullptr;   // nullptr: the process-wide chain
+    // KSW settings of this storage (gasal2_storage_set_ksw); off: the process-wide ones
+    bool ksw_own = false;
+    int32_t ksw_w = -1, ksw_zdrop = 0, ksw_eb = 5;
     // captured device program of the last batch shape (see run_async)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -244,6 +248,9 @@ extern "C" void gasal2_set_pk16(int32_t mode) { g_pk16_mode = mode < 0 ? 0 : (mo
 extern "C" void gasal2_set_ksw_extension(int32_t w, int32_t zdrop) {
     g_ksw_w = w;
     g_ksw_zdrop = zdrop > 0 ? zdrop : 0;
+}
+extern "C" void gasal2_set_ksw_end_bonus(int32_t end_bonus) {
+    g_ksw_eb = end_bonus;
 }
 
 extern "C" void gasal2_set_graph_mode(int32_t on) { g_graph_mode = on ? 1 : 0; }
@@ -557,6 +564,10 @@ extern "C" uint32_t gasal_host_batch_add(gasal_gpu_storage_t *gpu_storage, uint3
             *p_batch_bytes += *p_batch_bytes;
             while (*p_batch_bytes < size) *p_batch_bytes += *p_batch_bytes;
             host_batch_t *res = gasal_host_batch_new(*p_batch_bytes, idx);
+            // appended after the last page: the reference links it after the
+            // page the walk stopped at (src/host_batch.cpp:211-213), which
+            // drops the pages after it and the bases written there
+            while (cur_page->next != NULL) cur_page = cur_page->next;
             cur_page->next = res;
             cur_page = cur_page->next;
         }
@@ -845,6 +856,15 @@ extern "C" void gasal2_storage_set_order(gasal_gpu_storage_t *s, gasal2_order_t 
 }
 
 extern "C" void gasal2_order_free(gasal2_order_t *o) { delete o; }
+
+extern "C" void gasal2_storage_set_ksw(gasal_gpu_storage_t *s, int32_t own, int32_t w, int32_t zdrop, int32_t end_bonus) {
+    if (!s || !s->impl) return;
+    Impl *m = impl_of(s);
+    m->ksw_own = own != 0;
+    m->ksw_w = w;
+    m->ksw_zdrop = zdrop > 0 ? zdrop : 0;
+    m->ksw_eb = end_bonus;
+}
 
 // src/ctors.cpp:130-190
 extern "C" void gasal_destroy_streams(gasal_gpu_storage_v *gpu_storage_vec, Parameters *params) {
@@ -1217,7 +1237,7 @@ void ensure(T *&p, size_t &cap_bytes, size_t need_bytes) {
     cap_bytes = b;
 }
 
-G2Opts opts_of(const Parameters *p) {
+G2Opts opts_of(const Parameters *p, const gasal_gpu_storage_t *s) {
     G2Opts o;
     o.algo = p->algo;
     o.start = p->start_pos;
@@ -1232,6 +1252,13 @@ G2Opts opts_of(const Parameters *p) {
     o.prep_only = 0;
     o.ksw_w = g_ksw_w;
     o.ksw_zdrop = g_ksw_zdrop;
+    o.ksw_eb = g_ksw_eb;
+    const Impl *m = s && s->impl ? static_cast<const Impl *>(s->impl) : nullptr;
+    if (m && m->ksw_own) {
+        o.ksw_w = m->ksw_w;
+        o.ksw_zdrop = m->ksw_zdrop;
+        o.ksw_eb = m->ksw_eb;
+    }
     return o;
 }
 
@@ -1253,7 +1280,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
         mt = std::max(mt, ht[k]);
     }
     if (!px.sub) { m->last_max_q = mq; m->last_max_t = mt; m->last_n = n; }
-    const G2Opts op = opts_of(params);
+    const G2Opts op = opts_of(params, s);
     const uint32_t qw = g2_regs(mq), tw = g2_regs(mt);
     const uint32_t groups = (n + 63) / 64;
     size_t b = 0;
@@ -1565,7 +1592,7 @@ bool run_async(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, Par
     m->main_dev = d;
     m->main_valid = true;
     G2Scores sc = scores_now();
-    G2Opts op = opts_of(params);
+    G2Opts op = opts_of(params, s);
     const bool in_place = params->isPacked && params->isReverseComplement;
     if (in_place && m->prepped) op.skip_prep = 1;
     hipError_t e = launch_program(m, sc, op, d, s->str);
@@ -1712,7 +1739,7 @@ extern "C" int32_t gasal2_prep_resident(gasal_gpu_storage_t *s, uint32_t qb, uin
     t_failed = false;
     prepare(s, qb, tb, n, params, d, px);
     if (t_failed) return -1;
-    G2Opts op = opts_of(params);
+    G2Opts op = opts_of(params, s);
     op.prep_only = 1;
     hipError_t e = g2_run(scores_now(), op, d, s->str, nullptr);
     if (e != hipSuccess) {
@@ -1827,7 +1854,7 @@ extern "C" int32_t gasal2_run_banded(gasal_gpu_storage_t *s, uint32_t qb, uint32
         d.band_widen = check_widen;
     }
     m->main_dev = d;
-    G2Opts op = opts_of(params);
+    G2Opts op = opts_of(params, s);
     op.banded = 1;
     op.skip_prep = 1;
     hipError_t e = launch_program(m, scores_now(), op, d, s->str);
@@ -1859,7 +1886,7 @@ extern "C" int32_t gasal2_run_subset(gasal_gpu_storage_t *s, uint32_t qb, uint32
     if (!max_tb_bytes) max_tb_bytes = 32ull << 30;
     const bool tb_on = params->start_pos == WITH_TB;
     const G2Dev main = m->main_dev;
-    G2Opts op = opts_of(params);
+    G2Opts op = opts_of(params, s);
     op.skip_prep = 1;
     uint32_t k0 = 0;
     while (k0 < cnt) {

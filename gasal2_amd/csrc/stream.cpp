@@ -286,7 +286,7 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
     const uint32_t batch_pairs = stp->batch_pairs;
     const int32_t n_storages = (int32_t)stp->storages.size();
     if (pairs->n == 0) return fail("gasal2_stream_align: no pairs");
-    if (params->algo == KSW) return fail("gasal2_stream_align: KSW needs seed scores (use gasal_aln_async)");
+    if (params->algo == KSW && !pairs->seed_scores) return fail("gasal2_stream_align: KSW needs pairs->seed_scores");
     const bool packed = params->isPacked;
     if (packed && (!pairs->query_words || !pairs->target_words))
         return fail("gasal2_stream_align: isPacked needs the packed words (GASAL2_READ_PACK)");
@@ -384,6 +384,7 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
                 gasal_op_fill(s, pairs->query_ops + k0, m, QUERY);
                 gasal_op_fill(s, pairs->target_ops + k0, m, TARGET);
             }
+            if (params->algo == KSW) memcpy(s->host_seed_scores, pairs->seed_scores + k0, (size_t)m * 4);
             s->current_n_alns = m;
             const clk::time_point c1 = clk::now();
             if (first) {
@@ -460,6 +461,154 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
         }
     }
     return errors.load() ? fail("gasal2_stream_align: a sub-batch failed") : 0;
+}
+
+namespace {
+
+// A derived pair set (one side of the seed extension) owning its arrays.
+struct OwnedPairs {
+    std::vector<uint8_t> q, t, qop, top;
+    std::vector<uint32_t> qw, tw, qoff, toff, qlen, tlen, seeds;
+    std::vector<uint32_t> index;   // pair of the original set
+    gasal2_pairs_t view() {
+        gasal2_pairs_t p;
+        memset(&p, 0, sizeof p);
+        p.query = q.data();
+        p.target = t.data();
+        p.query_bytes = q.size();
+        p.target_bytes = t.size();
+        p.query_words = qw.empty() ? nullptr : qw.data();
+        p.target_words = tw.empty() ? nullptr : tw.data();
+        p.query_offsets = qoff.data();
+        p.target_offsets = toff.data();
+        p.query_lens = qlen.data();
+        p.target_lens = tlen.data();
+        p.query_ops = qop.data();
+        p.target_ops = top.data();
+        p.n = (uint32_t)index.size();
+        p.seed_scores = seeds.data();
+        return p;
+    }
+    // appends src[0, len) (reversed when rev) padded with N_CODE to 8 bases
+    static uint32_t put(std::vector<uint8_t> &dst, const uint8_t *src, uint32_t len, bool rev, uint8_t n_code) {
+        const uint32_t off = (uint32_t)dst.size();
+        if (rev)
+            for (uint32_t i = len; i-- > 0;) dst.push_back(src[i]);
+        else
+            dst.insert(dst.end(), src, src + len);
+        while (dst.size() % 8) dst.push_back(n_code);
+        return off;
+    }
+    void add(uint32_t k, const uint8_t *qs, uint32_t ql, const uint8_t *ts, uint32_t tl, bool rev, uint32_t h0,
+             uint8_t n_code) {
+        index.push_back(k);
+        qoff.push_back(put(q, qs, ql, rev, n_code));
+        toff.push_back(put(t, ts, tl, rev, n_code));
+        qlen.push_back(ql);
+        tlen.push_back(tl);
+        seeds.push_back(h0);
+    }
+    void finish(bool packed) {
+        qop.assign(index.size() + 1, 0);
+        top.assign(index.size() + 1, 0);
+        if (q.empty()) q.assign(8, 0);
+        if (t.empty()) t.assign(8, 0);
+        if (packed) {
+            qw.resize(q.size() / 8 + 1);
+            tw.resize(t.size() / 8 + 1);
+            pack_words(q.data(), q.size(), qw.data());
+            pack_words(t.data(), t.size(), tw.data());
+        }
+    }
+};
+
+}  // namespace
+
+extern "C" int32_t gasal2_extend_seeds(gasal2_stream_t *st, const gasal2_pairs_t *pairs, const gasal2_seed_t *seeds,
+                                       const gasal2_extend_opts_t *opts, gasal2_extend_out_t *out,
+                                       gasal2_stream_stats_t *stats) {
+    if (!st || !pairs || !seeds || !opts || !out) return fail("gasal2_extend_seeds: NULL argument");
+    if (st->params->algo != KSW) return fail("gasal2_extend_seeds: the stream must be made with algo = KSW");
+    const uint32_t n = pairs->n;
+    for (uint32_t k = 0; k < n; k++) {
+        const gasal2_seed_t &sd = seeds[k];
+        if ((uint64_t)sd.query_beg + sd.len > pairs->query_lens[k] || (uint64_t)sd.target_beg + sd.len > pairs->target_lens[k] ||
+            sd.score < 0)
+            return fail("gasal2_extend_seeds: a seed lies outside its pair or has a negative score");
+    }
+    const uint8_t n_code = (uint8_t)gasal2_get_n_code();
+    const bool packed = st->params->isPacked;
+    std::vector<int32_t> lscore(n), lq(n, 0), lt(n, 0), rscore(n), rq(n, 0), rt(n, 0);
+    gasal2_stream_stats_t tot;
+    memset(&tot, 0, sizeof tot);
+    auto run_side = [&](OwnedPairs &side, int32_t eb, std::vector<int32_t> &score, std::vector<int32_t> &qe,
+                        std::vector<int32_t> &te) -> int32_t {
+        if (side.index.empty()) return 0;
+        side.finish(packed);
+        for (gasal_gpu_storage_v &v : st->storages) gasal2_storage_set_ksw(&v.a[0], 1, opts->w, opts->zdrop, eb);
+        std::vector<int32_t> sc(side.index.size()), qend(side.index.size()), tend(side.index.size());
+        gasal2_results_t res;
+        memset(&res, 0, sizeof res);
+        res.score = sc.data();
+        res.query_end = qend.data();
+        res.target_end = tend.data();
+        gasal2_pairs_t view = side.view();
+        gasal2_stream_stats_t ss;
+        const int32_t r = gasal2_stream_align(st, &view, &res, &ss);
+        for (gasal_gpu_storage_v &v : st->storages) gasal2_storage_set_ksw(&v.a[0], 0, -1, 0, 5);
+        if (r != 0) return r;
+        for (size_t i = 0; i < side.index.size(); i++) {
+            const uint32_t k = side.index[i];
+            score[k] = sc[i];
+            qe[k] = qend[i];
+            te[k] = tend[i];
+        }
+        tot.seconds += ss.seconds;
+        tot.seconds_from_fill += ss.seconds_from_fill;
+        tot.sub_batches += ss.sub_batches;
+        tot.cells += ss.cells;
+        tot.fill_seconds += ss.fill_seconds;
+        tot.launch_seconds += ss.launch_seconds;
+        tot.wait_seconds += ss.wait_seconds;
+        tot.copy_seconds += ss.copy_seconds;
+        return 0;
+    };
+    // left: reversed prefixes from the seed's score
+    {
+        OwnedPairs left;
+        for (uint32_t k = 0; k < n; k++) {
+            lscore[k] = seeds[k].score;
+            if (seeds[k].query_beg && seeds[k].target_beg)
+                left.add(k, pairs->query + pairs->query_offsets[k], seeds[k].query_beg, pairs->target + pairs->target_offsets[k],
+                         seeds[k].target_beg, true, (uint32_t)seeds[k].score, n_code);
+        }
+        if (run_side(left, opts->end_bonus_left, lscore, lq, lt) != 0) return -1;
+    }
+    // right: suffixes from the left extension's score
+    {
+        OwnedPairs right;
+        for (uint32_t k = 0; k < n; k++) {
+            rscore[k] = lscore[k];
+            const uint32_t qs = seeds[k].query_beg + seeds[k].len, ts = seeds[k].target_beg + seeds[k].len;
+            if (qs < pairs->query_lens[k] && ts < pairs->target_lens[k])
+                right.add(k, pairs->query + pairs->query_offsets[k] + qs, pairs->query_lens[k] - qs,
+                          pairs->target + pairs->target_offsets[k] + ts, pairs->target_lens[k] - ts, false,
+                          (uint32_t)std::max(lscore[k], 0), n_code);
+        }
+        if (run_side(right, opts->end_bonus_right, rscore, rq, rt) != 0) return -1;
+    }
+    for (uint32_t k = 0; k < n; k++) {
+        const gasal2_seed_t &sd = seeds[k];
+        if (out->score) out->score[k] = rscore[k];
+        if (out->left_score) out->left_score[k] = lscore[k];
+        if (out->right_score) out->right_score[k] = rscore[k];
+        if (out->query_beg) out->query_beg[k] = (int32_t)sd.query_beg - lq[k];
+        if (out->target_beg) out->target_beg[k] = (int32_t)sd.target_beg - lt[k];
+        if (out->query_end) out->query_end[k] = (int32_t)(sd.query_beg + sd.len) + rq[k];
+        if (out->target_end) out->target_end[k] = (int32_t)(sd.target_beg + sd.len) + rt[k];
+    }
+    if (stats) *stats = tot;
+    return 0;
 }
 
 extern "C" int32_t gasal2_pairs_align(const gasal2_pairs_t *pairs, gasal2_results_t *out, Parameters *params,

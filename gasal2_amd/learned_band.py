@@ -339,11 +339,13 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
         if log:
             log(f"banded chunk of {len(ch)} pairs: {int(np.count_nonzero(flags))} fallbacks")
     fb = int(np.count_nonzero(flags_all))
-    cells = float(np.dot(batch.qlen.astype(np.int64), batch.tlen.astype(np.int64)))
+    pc = batch.qlen.astype(np.int64) * batch.tlen.astype(np.int64)
+    cells = float(pc.sum())
     stats = {"propose_s": tp, "banded_s": tb_, "fallback_s": tf, "total_s": tp + tb_ + tf,
              "fallback_pairs": fb, "hit_rate": 1.0 - fb / max(n, 1),
              "flag_counts": {str(b): int(np.sum((flags_all >> b) & 1)) for b in range(5)},
-             "band_cells_fraction": band_cells / max(cells, 1.0)}
+             "band_cells_fraction": band_cells / max(cells, 1.0),
+             "fallback_cells_fraction": float(pc[flags_all != 0].sum()) / max(cells, 1.0)}
     return res, stats
 
 

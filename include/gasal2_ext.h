@@ -164,6 +164,15 @@ void gasal2_set_pk16(int32_t mode);
  * only the current 8-row tile; its zdrop is 0, so it never fires there).
  * Defaults -1, 0 reproduce the reference. */
 void gasal2_set_ksw_extension(int32_t w, int32_t zdrop);
+/* KSW end bonus: the score of reaching the query's end is reported (query end =
+ * its length) unless it is <= the best score - end_bonus; with a band it also
+ * enters the clamp of w, as bwa's ksw_extend2 end_bonus does.  Default 5, the
+ * reference's fixed PEN_CLIP5 (src/kernels/ksw_kernel_template.h:20,188). */
+void gasal2_set_ksw_end_bonus(int32_t end_bonus);
+/* KSW settings for one storage (own = 1), overriding the process-wide ones
+ * above for its batches; own = 0 returns it to them.  No reference
+ * counterpart. */
+void gasal2_storage_set_ksw(gasal_gpu_storage_t *s, int32_t own, int32_t w, int32_t zdrop, int32_t end_bonus);
 
 /* Checkpointed traceback for LOCAL WITH_TB.  The DP stores, per 8x8 tile,
  * the left column state (H, F) and, per strip, the bottom row (H, E) -- two

@@ -33,6 +33,7 @@ typedef struct {
     uint8_t *query_ops, *target_ops; /* operation_on_seq per pair */
     char **query_names, **target_names;
     uint32_t n;
+    uint32_t *seed_scores;           /* KSW: h0 per pair (NULL otherwise; the caller's, gasal2_pairs_free leaves it) */
 } gasal2_pairs_t;
 
 /* gasal2_pairs_read flags */
@@ -97,10 +98,41 @@ void gasal2_stream_set_order(gasal2_stream_t *stream, int32_t mode);
  * to its storages, each driven by its own host thread (fill the pinned pages,
  * gasal_aln_async, wait, copy the results out).  With params->isPacked the
  * packed words are handed over (pairs->query_words / target_words must be
- * set).  KSW needs seed scores and is not streamed here.  Returns 0, or -1
- * after a message on stderr. */
+ * set); KSW takes pairs->seed_scores.  Returns 0, or -1 after a message on
+ * stderr. */
 int32_t gasal2_stream_align(gasal2_stream_t *stream, const gasal2_pairs_t *pairs, gasal2_results_t *out,
                             gasal2_stream_stats_t *stats);
+
+/* Two-sided seed extension (the step before alignment in a seed-and-extend
+ * mapper; bwa-mem's left/right ksw_extend2 calls around each seed, the use the
+ * reference's KSW kernel was written for, src/kernels/ksw_kernel_template.h
+ * :27-29): for pair k the seed query[query_beg, +len) ~ target[target_beg, +len)
+ * of score h0 is extended to the left by KSW on the reversed prefixes
+ * (query[0, query_beg), target[0, target_beg)) from h0, then to the right on
+ * the suffixes from the left extension's score.  Each side runs as a
+ * stream_align batch with the given band w, z-drop and end bonus (the KSW
+ * settings of gasal2_storage_set_ksw); a side with an empty query or target is
+ * not extended.  The stream must be made with algo = KSW. */
+typedef struct {
+    uint32_t query_beg, target_beg, len;
+    int32_t score; /* h0 of the left extension, e.g. len * match */
+} gasal2_seed_t;
+
+typedef struct {
+    int32_t w, zdrop;                   /* as gasal2_set_ksw_extension (w < 0: no band, zdrop 0: off) */
+    int32_t end_bonus_left, end_bonus_right; /* as gasal2_set_ksw_end_bonus, per side */
+} gasal2_extend_opts_t;
+
+/* Output arrays of pairs->n entries (NULL = not wanted): the extended
+ * alignment's score (the right extension's, which started from the left
+ * one's), half-open spans [beg, end) on query and target, and each side's
+ * KSW score (the seed's score where a side was not extended). */
+typedef struct {
+    int32_t *score, *query_beg, *query_end, *target_beg, *target_end, *left_score, *right_score;
+} gasal2_extend_out_t;
+
+int32_t gasal2_extend_seeds(gasal2_stream_t *stream, const gasal2_pairs_t *pairs, const gasal2_seed_t *seeds,
+                            const gasal2_extend_opts_t *opts, gasal2_extend_out_t *out, gasal2_stream_stats_t *stats);
 
 /* gasal2_stream_new + gasal2_stream_align + gasal2_stream_free in one call
  * (the workspace is allocated inside, so the first sub-batches pay for it). */

@@ -472,13 +472,13 @@ void banded_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint6
  * at the start of row i.  Parity of the extension is unpinned (bwa is not
  * available here). */
 void ksw_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint64_t tb,
-              uint32_t qlen, uint32_t tlen, uint32_t h0, int32_t w, int32_t zdrop_ext, PairOut &o) {
+              uint32_t qlen, uint32_t tlen, uint32_t h0, int32_t w, int32_t zdrop_ext, int32_t end_bonus, PairOut &o) {
     const uint32_t query_batch_regs = (qlen >> 3) + 1;                            /* :56-57 */
     const uint32_t target_batch_regs = (tlen >> 3) + 1;
     const int zdrop = zdrop_ext;                                                  /* :62 (0 there) */
-    if (w >= 0) {
+    if (w >= 0) {   /* bwa's ksw_extend2 clamp (insertions and deletions cost alike here) */
         const int maxsc = std::max(c.match, 0);
-        int mx = (int)((double)((int)qlen * maxsc - c.gapo) / c.gape + 1.0);
+        int mx = (int)((double)((int)qlen * maxsc + end_bonus - c.gapo) / c.gape + 1.0);
         mx = mx > 1 ? mx : 1;
         w = w < mx ? w : mx;
     }
@@ -562,7 +562,7 @@ void ksw_pair(const Sc &c, const Words &Q, const Words &T, uint64_t qb, uint64_t
         }
     }
     (void)max_off;
-    if (gscore <= 0 || gscore <= max - 5) {                                       /* :188-197, PEN_CLIP5 */
+    if (gscore <= 0 || gscore <= max - end_bonus) {                               /* :188-197, PEN_CLIP5 */
         o.score = max; o.qend = max_j + 1; o.tend = max_i + 1;
     } else {
         o.score = gscore; o.qend = (int32_t)qlen; o.tend = max_ie + 1;
@@ -753,7 +753,7 @@ extern "C" int oracle_align_batch(const oracle_scores_t *scp, const oracle_param
                 banded_pair(c, Q, T, qb, tb, ql, tl, pr->k_band >> 3, o);         /* src/gasal_align.h:80 */
                 break;
             case A_KSW:
-                ksw_pair(c, Q, T, qb, tb, ql, tl, seed_scores ? seed_scores[k] : 0u, pr->ksw_w, pr->ksw_zdrop, o);
+                ksw_pair(c, Q, T, qb, tb, ql, tl, seed_scores ? seed_scores[k] : 0u, pr->ksw_w, pr->ksw_zdrop, pr->ksw_end_bonus, o);
                 break;
             default:
                 break;   /* UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21) */

@@ -43,6 +43,9 @@ typedef struct {
     int32_t is_reverse_complement; /* Parameters::isReverseComplement :43 */
     int32_t ksw_w;                 /* KSW band half-width, < 0: none (the reference: none) */
     int32_t ksw_zdrop;             /* KSW z-drop, 0: off (the reference: 0, ksw_kernel_template.h:62) */
+    int32_t ksw_end_bonus;         /* KSW end bonus: the to-end score wins unless it is <= max - end_bonus
+                                      (the reference: PEN_CLIP5 = 5, ksw_kernel_template.h:20,188); with a
+                                      band it also enters bwa's clamp of w */
 } oracle_params_t;
 
 typedef struct {

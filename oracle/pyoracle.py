@@ -20,7 +20,7 @@ class Scores(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("algo", "start_pos", "second_best", "head", "tail", "k_band", "is_packed",
-                                         "is_reverse_complement", "ksw_w", "ksw_zdrop")]
+                                         "is_reverse_complement", "ksw_w", "ksw_zdrop", "ksw_end_bonus")]
 
 
 class Out(C.Structure):
@@ -73,7 +73,7 @@ def rc_one(words, length, op, n_code=0x4E):
 
 def align(batch, algo, start_pos=0, second_best=False, head=2, tail=2, k_band=0, is_packed=False,
           is_rc=False, scores=(1, 4, 6, 1), n_code=0x4E, n_penalty=None, n_threads=0, with_cigar=True,
-          ksw_w=-1, ksw_zdrop=0):
+          ksw_w=-1, ksw_zdrop=0, ksw_end_bonus=5):
     """Runs the oracle on a gasal2_amd.Batch; returns a dict of numpy arrays.
     CIGARs (WITH_TB) are returned as a list of bytes, one per pair."""
     L = lib()
@@ -81,7 +81,7 @@ def align(batch, algo, start_pos=0, second_best=False, head=2, tail=2, k_band=0,
     sc = Scores(scores[0], scores[1], scores[2], scores[3], n_code, 0 if n_penalty is None else 1,
                 0 if n_penalty is None else int(n_penalty))
     pr = Params(int(algo), int(start_pos), int(bool(second_best)), int(head), int(tail), int(k_band),
-                int(bool(is_packed)), int(bool(is_rc)), int(ksw_w), int(ksw_zdrop))
+                int(bool(is_packed)), int(bool(is_rc)), int(ksw_w), int(ksw_zdrop), int(ksw_end_bonus))
     res = {k: np.zeros(n, np.int32) for k in ("score", "qend", "tend", "qstart", "tstart", "s_score", "s_qend",
                                                "s_tend")}
     res["n_ops"] = np.zeros(n, np.uint32)

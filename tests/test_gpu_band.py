@@ -182,3 +182,36 @@ def test_learned_pipeline_small():
     for k in ("score", "qend", "tend", "qstart", "tstart", "n_ops"):
         assert np.array_equal(res[k], ref[k]), k
     assert res["cigars"] == ref["cigars"]
+
+
+def test_learned_band_beats_the_peak_band():
+    """The ablation: on held-out long pairs, the trained network's bands and
+    the histogram-peak band (both calibrated for 99.5% coverage of the
+    training paths) both reach 100% agreement with the full DP after their
+    fallbacks; the learned band does it with less DP work (band cells plus
+    the full matrices of the pairs it sends back)."""
+    from gasal2_amd import learned_band as LB
+    train = long_batch(12000, 2600)
+    held = long_batch(3000, 2601)
+    ref_tr, _, (dmin, dmax) = LB.full_dp(train, n_penalty=1, labels=True)
+    a = loaded(train)
+    try:
+        feat, anchor, binw = LB.features(a, train.n)
+        torch.cuda.synchronize()
+    finally:
+        a.close()
+    An, Bn = anchor.cpu().numpy(), binw.cpu().numpy()
+    T, w = LB.make_targets(An, Bn, dmin, dmax, ref_tr["score"])
+    net = LB.BandNet(seed=0)
+    LB.train(net, feat, T, w, epochs=8, batch=512, lr=2e-3)
+    _, _, Y = net.forward(feat)
+    mul, add = LB.calibrate_learned(Y.cpu().numpy(), An, Bn, dmin, dmax, ref_tr["score"])
+    half = LB.calibrate_peak(feat.cpu().numpy(), An, Bn, train.qlen, train.tlen, dmin, dmax, ref_tr["score"])
+    ref, _, _ = LB.full_dp(held, n_penalty=1)
+    work = {}
+    for name, prop in (("learned", LB.learned_proposer(net, mul, add)), ("peak", LB.peak_proposer(half))):
+        res, st = LB.band_align(held, prop, n_penalty=1)
+        agree, bad = LB.agreement(res, ref)
+        assert agree == 1.0, (name, bad[:8])
+        work[name] = st["band_cells_fraction"] + st["fallback_cells_fraction"]
+    assert work["learned"] < work["peak"], work

@@ -270,9 +270,28 @@ def test_ksw_random_never_below_seed():
 # follow, is not in the image: parity of the extension is unpinned beyond the
 # properties below (no-op settings, the band constraint on the end cell,
 # z-drop only removing rows, and two constructed cases each setting changes).
-def _ksw(q, t, h0, w=-1, zdrop=0):
-    r = O.align(g.make_batch([q], [t], seeds=[h0]), g.KSW, ksw_w=w, ksw_zdrop=zdrop)
+def _ksw(q, t, h0, w=-1, zdrop=0, end_bonus=5):
+    r = O.align(g.make_batch([q], [t], seeds=[h0]), g.KSW, ksw_w=w, ksw_zdrop=zdrop, ksw_end_bonus=end_bonus)
     return int(r["score"][0]), int(r["qend"][0]), int(r["tend"][0])
+
+
+@pytest.mark.parametrize("n,h0", [(20, 10), (45, 30), (120, 25)])
+def test_ksw_end_bonus_closed_form(n, h0):
+    # query == target except the last base: the best cell is (n-2, n-2) with
+    # h0 + n - 1; reaching the query's end costs the mismatch, h0 + n - 1 - 4.
+    # The to-end score wins exactly when it exceeds best - end_bonus, i.e. when
+    # end_bonus > 4 (the reference's PEN_CLIP5 = 5 takes the end).
+    rng = random.Random(n)
+    s = rnd_seq(rng, n)
+    q, t = s, s[:-1] + (b"A" if s[-1:] != b"A" else b"C")
+    best, to_end = h0 + n - 1, h0 + n - 1 - 4
+    assert _ksw(q, t, h0) == (to_end, n, n)
+    assert _ksw(q, t, h0, end_bonus=5) == (to_end, n, n)
+    for eb in (4, 3, 0):
+        assert _ksw(q, t, h0, end_bonus=eb) == (best, n - 1, n - 1)
+    assert _ksw(q, t, h0, end_bonus=100) == (to_end, n, n)
+    # with a band, the end bonus widens bwa's clamp of w: w = 0 is never clamped
+    assert _ksw(q, t, h0, w=0, end_bonus=eb)[0] == best
 
 
 def test_ksw_extension_noop_settings_equal_reference():
