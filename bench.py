@@ -262,6 +262,7 @@ def main():
                     help="measure the headline configuration only (no other workloads)")
     ap.add_argument("--packed", action="store_true", help="one configuration with isPacked=true host inputs")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
+    ap.add_argument("--band-chunk", type=int, default=65536, help="learned band: pairs per banded launch")
     ap.add_argument("--stream-batches", type=int, default=4,
                     help="steady state: full-feature batches streamed end to end (0: skip)")
     ap.add_argument("--stream-sub", type=int, default=1_000_000, help="steady state: pairs per sub-batch")
@@ -761,7 +762,7 @@ def band_main(args):
         runs += [(f"learned@{cw}" if cw != args.check_widen[0] else "learned", LB.learned_proposer(net, mul, add), cw),
                  (f"peak@{cw}" if cw != args.check_widen[0] else "peak", LB.peak_proposer(half), cw)]
     for name, prop, cw in runs:
-        res, st = LB.band_align(held, prop, check_widen=cw)
+        res, st = LB.band_align(held, prop, check_widen=cw, chunk=args.band_chunk)
         agree, bad = LB.agreement(res, ref)
         out[name] = {"check_widen": cw, "gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
                      "pairs_per_s": round(n / st["total_s"], 1), "hit_rate": round(st["hit_rate"], 5),
@@ -780,7 +781,8 @@ def band_main(args):
         "dtype": "int32 (DP), bf16/fp32 (network)", "data": "synthetic",
         "config": {"workload": f"long divergent pairs: {n} train (seed 5) / {n} held-out (seed 6), Lq~U{{1000..10000}}, "
                                "10-15% divergence, Poisson(1/kbp) indel bursts U{5..50}",
-                   "mode": "LOCAL WITH_TB", "scores": [1, 4, 6, 1], "held_out_cells": cells},
+                   "mode": "LOCAL WITH_TB", "scores": [1, 4, 6, 1], "held_out_cells": cells,
+                   "band_chunk_pairs": args.band_chunk},
         "full_dp": {"gcups": round(cells / full_s / 1e9, 2), "seconds": round(full_s, 3),
                     "pairs_per_s": round(n / full_s, 1)},
         "learned": out["learned"], "peak_band": out["peak"],

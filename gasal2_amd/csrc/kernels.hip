@@ -1578,7 +1578,16 @@ __global__ __launch_bounds__(64) void k_local_band(G2Dev d, G2Scores c) {
                 }
                 bd[(8 * (w - wlo) + kk) * WAVE] = pk16(h[8], e);
                 if (colbest > (rv | 7)) { rv = colbest; rs = st; rc = col; best_edge = edge; }
-                if (TB) dirw[kk] = dir;
+                // keep each column's bookkeeping with its DP (left alone the
+                // compiler holds several columns' values live: 256 VGPRs with
+                // the direction words)
+                if (TB) {
+                    dirw[kk] = dir;
+                    asm volatile("" : "+v"(rv), "+v"(rs), "+v"(rc), "+v"(dirw[kk]));
+                } else {
+                    asm volatile("" : "+v"(rv), "+v"(rs), "+v"(rc));
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
             if (TB) {
                 if (w - wlo < bw) tb_store(tbp + ((size_t)st * bw + (size_t)(w - wlo)) * 8, dirw);

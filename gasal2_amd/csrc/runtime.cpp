@@ -1451,7 +1451,18 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                         (uint64_t)std::max(sc.match, 0) * 8u * std::min(eq, et) <= 32767u &&
                         (uint64_t)sc.gapoe + (uint64_t)std::max(std::max(sc.mismatch, sc.has_np ? sc.np : 0), 0) <= 32767u;
         const uint64_t nreg = ck ? 2 : 1;
-        if (one_size <= g_tb_one_size_max) {
+        // One region size for all groups while it is within the budget and
+        // both regions take at most half of the device memory free now (this
+        // storage's own tile buffer counted as free: it is reused), so that
+        // several storages of a large batch fit beside each other; else
+        // regions sized per group (about half the bytes on mixed lengths).
+        bool one = one_size <= g_tb_one_size_max;
+        if (one) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+                one = nreg * one_size <= ((uint64_t)free_b + m->tb_bytes) / 2;
+        }
+        if (one) {
             ensure(m->tb, m->tb_bytes, (size_t)(nreg * one_size) + 256);
             if (ck) d.tbh = m->tb + one_size / 4;
         } else {
