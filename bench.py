@@ -123,6 +123,7 @@ MODES = {
     "local_score": ("LOCAL", "WITHOUT_START", False, None, None, 0),
     "local_start": ("LOCAL", "WITH_START", False, None, None, 0),
     "global": ("GLOBAL", "WITH_START", False, None, None, 0),
+    "global_tb": ("GLOBAL", "WITH_TB", False, None, None, 0),
     "semi_global": ("SEMI_GLOBAL", "WITH_START", False, "TARGET", "TARGET", 0),
     "ksw": ("KSW", "WITHOUT_START", False, None, None, 0),
     "banded": ("BANDED", "WITHOUT_START", False, None, None, 32),
@@ -145,7 +146,7 @@ def metric_ops(mode):
 def kernel_ops(mode, tb_checkpoint):
     algo, start, second = MODES[mode][:3]
     ops = OPS_SCORE + (OPS_SECOND if second else 0)
-    if start == "WITH_TB" and not tb_checkpoint:
+    if start == "WITH_TB" and (not tb_checkpoint or algo != "LOCAL"):   # checkpoints: LOCAL only
         ops += OPS_DIR
     return ops
 

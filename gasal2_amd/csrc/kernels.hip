@@ -1778,10 +1778,14 @@ __global__ __launch_bounds__(64) void k_global(G2Dev d, G2Scores c) {
                     h[m] = hm;
                 }
                 bd[ridx * WAVE] = pk16(h[8], e);
-                if (TB) dirw[kk] = dir;
                 if (ridx == lastcol && i == TW - 1) {                                      // :98-103
 #pragma unroll
                     for (int m = 1; m < 9; m++) score = (m == rowsel) ? h[m] : score;
+                }
+                if (TB) {   // one column at a time (else the compiler holds several: 256 VGPRs)
+                    dirw[kk] = dir;
+                    asm volatile("" : "+v"(dirw[kk]), "+v"(score));
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
             if (TB) tbv.store(i, j, dirw);
