@@ -263,7 +263,8 @@ def main():
                     help="measure the headline configuration only (no other workloads)")
     ap.add_argument("--packed", action="store_true", help="one configuration with isPacked=true host inputs")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
-    ap.add_argument("--band-chunk", type=int, default=65536, help="learned band: pairs per banded launch")
+    ap.add_argument("--band-chunk", type=int, default=0,
+                    help="learned band: pairs per banded launch (0: planned from the bands and a 64 GiB tile budget)")
     ap.add_argument("--stream-batches", type=int, default=4,
                     help="steady state: full-feature batches streamed end to end (0: skip)")
     ap.add_argument("--stream-sub", type=int, default=1_000_000, help="steady state: pairs per sub-batch")
@@ -763,7 +764,7 @@ def band_main(args):
         runs += [(f"learned@{cw}" if cw != args.check_widen[0] else "learned", LB.learned_proposer(net, mul, add), cw),
                  (f"peak@{cw}" if cw != args.check_widen[0] else "peak", LB.peak_proposer(half), cw)]
     for name, prop, cw in runs:
-        res, st = LB.band_align(held, prop, check_widen=cw, chunk=args.band_chunk)
+        res, st = LB.band_align(held, prop, check_widen=cw, chunk=args.band_chunk or None)
         agree, bad = LB.agreement(res, ref)
         out[name] = {"check_widen": cw, "gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
                      "pairs_per_s": round(n / st["total_s"], 1), "hit_rate": round(st["hit_rate"], 5),

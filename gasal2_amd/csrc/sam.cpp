@@ -74,17 +74,35 @@ inline char letter(uint32_t nib) {
     }
 }
 
+inline uint32_t nib_comp(uint32_t nt) { return nt == 1u ? 4u : nt == 4u ? 1u : nt == 3u ? 7u : nt == 7u ? 3u : nt; }
+
 // The first len bases the DP saw, as letters A C G T N.
 void aligned_letters(const uint8_t *seq, uint32_t len, uint8_t op, bool rc, int32_t n_code, std::vector<uint32_t> &w,
                      std::string &out) {
+    out.resize(len);
+    if (!rc || !op) {   // the bytes as given
+        for (uint32_t i = 0; i < len; i++) out[i] = letter(seq[i] & 15u);
+        return;
+    }
+    if ((uint32_t)n_code > 15u) {
+        // no nibble equals such an N_CODE, so the reverse is a plain reversal
+        // of the padded sequence (padding first), as rc_words computes it
+        const uint32_t padded = 8 * ((len + 7) / 8), pad = (uint32_t)n_code & 15u;
+        for (uint32_t i = 0; i < len; i++) {
+            const uint32_t src = (op & 1u) ? padded - 1 - i : i;
+            uint32_t nt = src < len ? seq[src] & 15u : pad;
+            if (op & 2u) nt = nib_comp(nt);
+            out[i] = letter(nt);
+        }
+        return;
+    }
     const uint32_t regs = (len + 7) / 8;
     w.assign(regs, 0u);
     for (uint32_t i = 0; i < 8 * regs; i++) {
         const uint32_t b = i < len ? seq[i] : (uint32_t)(uint8_t)n_code;
         w[i / 8] |= (b & 15u) << (28 - 4 * (i % 8));
     }
-    if (rc && op) rc_words(w.data(), len, op, n_code);
-    out.resize(len);
+    rc_words(w.data(), len, op, n_code);
     for (uint32_t i = 0; i < len; i++) out[i] = letter((w[i / 8] >> (28 - 4 * (i % 8))) & 15u);
 }
 
@@ -338,12 +356,13 @@ extern "C" int32_t gasal2_sam_write(const char *path, const gasal2_pairs_t *pair
     bool ok = true;
     if (flags & GASAL2_SAM_HEADER) {
         std::string h = "@HD\tVN:1.6\tSO:unsorted\n";
-        std::unordered_map<std::string, uint32_t> seen;
+        std::unordered_map<std::string, uint32_t> seen;   // distinct names only (generated names are unique)
+        if (pairs->target_names) seen.reserve((size_t)n * 2);
         std::string tmp, key;
         for (uint32_t k = 0; k < n; k++) {
             key.clear();
             put_name(key, rname(k, tmp));
-            if (!seen.emplace(key, k).second) continue;
+            if (pairs->target_names && !seen.emplace(key, k).second) continue;
             h.append("@SQ\tSN:");
             h.append(key);
             h.append("\tLN:");

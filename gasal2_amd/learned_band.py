@@ -286,14 +286,44 @@ def calibrate_peak(feat, anchor, binw, qlen, tlen, dmin, dmax, score, coverage=0
     return float(np.quantile(need[ok], coverage)) if ok.any() else 64.0
 
 
+def _band_tb_bytes(qlen, tlen, dlo, dhi):
+    """Band-compressed traceback words of each pair (the library's sizing in
+    gasal2_run_banded): TW strips x the words any strip of the band touches,
+    32 bytes per tile."""
+    QW = (qlen.astype(np.int64) + 7) // 8
+    TW = (tlen.astype(np.int64) + 7) // 8
+    span = dhi.astype(np.int64) - dlo.astype(np.int64)
+    bw = np.where(span < 0, 1, ((span + 14) >> 3) + 1)
+    bw = np.clip(bw, 1, np.maximum(QW, 1))
+    return TW * bw * 32
+
+
+def _plan_chunks(order, est, budget, max_chunk):
+    """Consecutive pieces of `order` whose estimated bytes stay under budget."""
+    out, start, acc = [], 0, 0
+    for i, k in enumerate(order):
+        b = int(est[k])
+        if i > start and (acc + b > budget or i - start >= max_chunk):
+            out.append(order[start:i])
+            start, acc = i, 0
+        acc += b
+    if start < len(order):
+        out.append(order[start:])
+    return out
+
+
 def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WITH_TB, check_widen=CHECK_WIDEN,
-               chunk=65536, log=None):
+               chunk=None, tb_budget=64 << 30, sample=4096, log=None):
     """Banded LOCAL with full-DP fallback over the batch, in length-sorted
-    chunks of at most `chunk` pairs (one storage each; a large chunk keeps
-    enough one-lane-per-pair waves in flight).  `propose(a, n) ->
-    (dlo, dhi, pre)` host arrays (pre: pairs sent to the full DP outright),
-    called on the prepared chunk.  Returns
-    (results over all pairs, stats with timings and flag counts)."""
+    chunks (one storage each).  Large chunks keep enough one-lane-per-pair
+    waves in flight; the band-compressed traceback tiles of a chunk must fit
+    `tb_budget`.  With chunk=None the chunks are planned from the bands
+    proposed for a sample of `sample` pairs (widest band relative to the
+    query, with a margin); a chunk whose proposed bands still exceed the
+    budget is split.  An int `chunk` fixes the chunk size instead.
+    `propose(a, n) -> (dlo, dhi, pre)` host arrays (pre: pairs sent to the
+    full DP outright), called on the prepared chunk.  Returns (results over
+    all pairs, stats with timings and flag counts)."""
     torch = _torch()
     n = batch.n
     res = {k: np.zeros(n, np.int32) for k in ("score", "qend", "tend", "qstart", "tstart")}
@@ -303,8 +333,32 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
     band_cells = 0.0
     tp = tb_ = tf = 0.0
     order = length_order(batch)
-    for s0 in range(0, n, chunk):
-        ch = order[s0:s0 + chunk]
+    QW = (batch.qlen.astype(np.int64) + 7) // 8
+    TW = (batch.tlen.astype(np.int64) + 7) // 8
+    if chunk is None:
+        # band width relative to the query, from a sample's proposals
+        rng = np.random.default_rng(0)
+        smp = np.sort(rng.choice(n, min(sample, n), replace=False))
+        sb = batch.subset(smp)
+        a = _aligner_for(sb, start_pos, scores, n_penalty)
+        try:
+            a.load(sb)
+            a.sync()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            a.prep()
+            dlo, dhi, _ = propose(a, sb.n)
+            tp += time.perf_counter() - t0
+        finally:
+            a.close()
+        ratio = _band_tb_bytes(sb.qlen, sb.tlen, dlo, dhi) / np.maximum(TW[smp] * QW[smp] * 32, 1)
+        r = min(1.0, 1.5 * float(ratio.max()) + 1e-3)
+        est = np.maximum(TW * np.ceil(r * QW) * 32, 32)
+        pieces = _plan_chunks(order, est, tb_budget, 1 << 20)
+    else:
+        pieces = [order[s0:s0 + chunk] for s0 in range(0, n, chunk)]
+    while pieces:
+        ch = pieces.pop(0)
         sub = batch.subset(ch)
         a = _aligner_for(sub, start_pos, scores, n_penalty)
         try:
@@ -315,6 +369,12 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
             a.prep()
             dlo, dhi, pre = propose(a, sub.n)
             t1 = time.perf_counter()
+            need = int(_band_tb_bytes(sub.qlen, sub.tlen, dlo, dhi).sum()) if start_pos == WITH_TB else 0
+            if need > tb_budget and len(ch) > 1:   # split and redo this chunk
+                tp += t1 - t0
+                h = len(ch) // 2
+                pieces[:0] = [ch[:h], ch[h:]]
+                continue
             flags = a.run_banded(dlo, dhi, check_widen=check_widen) | pre
             t2 = time.perf_counter()
             a.run_subset(np.nonzero(flags)[0])

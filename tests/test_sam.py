@@ -266,3 +266,23 @@ def test_writer_matches_single_records(tmp_path):
                             second_score=int(r["s_score"][k]), rc_enabled=True)
         assert body[k] == want, k
     assert any(read_record(l)["flag"] & 16 for l in body[:200])
+
+
+@pytest.mark.parametrize("n_code", [0x0E, 0x04])
+def test_seq_follows_the_reverse_with_a_nibble_n_code(n_code):
+    """With an N_CODE that a nibble can equal, the reverse shifts by the
+    count of such nibbles in the last word (the reference's nbr_N): SEQ must
+    be the oracle's reverse/complement of the padded words, letter by letter."""
+    rng = random.Random(n_code)
+    for length in list(range(1, 25)) + [40, 63]:
+        q = rnd(rng, length)
+        pad = (-length) % 8
+        nib = [c & 15 for c in q] + [n_code & 15] * pad
+        words = [sum(nib[8 * w + i] << (28 - 4 * i) for i in range(8)) for w in range(len(nib) // 8)]
+        for op in (1, 2, 3):
+            got = O.rc_one(words, length, op, n_code)
+            want = "".join("ACGT"["\x01\x03\x07\x04".index(chr((int(got[i // 8]) >> (28 - 4 * (i % 8))) & 15))]
+                           if ((int(got[i // 8]) >> (28 - 4 * (i % 8))) & 15) in (1, 3, 4, 7) else "N"
+                           for i in range(length))
+            line = g.sam_record(q, b"ACGT", 0, query_op=op, rc_enabled=True, n_code=n_code)
+            assert seq_field(line) == want, (length, op)
