@@ -318,8 +318,8 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
     chunks (one storage each).  Large chunks keep enough one-lane-per-pair
     waves in flight; the band-compressed traceback tiles of a chunk must fit
     `tb_budget`.  With chunk=None the chunks are planned from the bands
-    proposed for a sample of `sample` pairs (widest band relative to the
-    query, with a margin); a chunk whose proposed bands still exceed the
+    proposed for a sample of `sample` pairs (their mean share of the full
+    tile grid, with a margin); a chunk whose proposed bands still exceed the
     budget is split.  An int `chunk` fixes the chunk size instead.
     `propose(a, n) -> (dlo, dhi, pre)` host arrays (pre: pairs sent to the
     full DP outright), called on the prepared chunk.  Returns (results over
@@ -351,8 +351,10 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
             tp += time.perf_counter() - t0
         finally:
             a.close()
+        # the mean share of the full tile grid, with a margin: chunks are sums
+        # over many pairs, and one that still comes out over budget is split
         ratio = _band_tb_bytes(sb.qlen, sb.tlen, dlo, dhi) / np.maximum(TW[smp] * QW[smp] * 32, 1)
-        r = min(1.0, 1.5 * float(ratio.max()) + 1e-3)
+        r = min(1.0, 1.5 * float(ratio.mean()) + 1e-3)
         est = np.maximum(TW * np.ceil(r * QW) * 32, 32)
         pieces = _plan_chunks(order, est, tb_budget, 1 << 20)
     else:
