@@ -22,7 +22,7 @@ the same batch on this host and reports the GPU's agreement with it there.
 The default run also measures every other workload of the benchmark set under
 `configs` (each with its own agreement check against the oracle): the 10k-pair
 short-read batch, the full-feature batch's GLOBAL / SEMI_GLOBAL modes and its
-packed-input form, steady state end to end (four full-feature batches
+packed-input form, steady state end to end (eight full-feature batches
 streamed through the library's stream driver, first H2D to the last results
 on the host; and FASTA files in, SAM out), the uneven-length batch, the seed-extension batch (KSW, both
 forms, and tile-banded) and the learned band on long divergent pairs.
@@ -265,7 +265,7 @@ def main():
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
     ap.add_argument("--band-chunk", type=int, default=0,
                     help="learned band: pairs per banded launch (0: planned from the bands and a 64 GiB tile budget)")
-    ap.add_argument("--stream-batches", type=int, default=4,
+    ap.add_argument("--stream-batches", type=int, default=8,
                     help="steady state: full-feature batches streamed end to end (0: skip)")
     ap.add_argument("--stream-sub", type=int, default=1_000_000, help="steady state: pairs per sub-batch")
     ap.add_argument("--stream-storages", type=int, default=3, help="steady state: storages (host threads)")
@@ -613,7 +613,7 @@ def concat_batches(g, bs):
                    cat("qop"), cat("top"))
 
 
-def stream_measure(g, full, headline_res, p, batches, sub_pairs, storages, threads, reps=3, sample=96):
+def stream_measure(g, full, headline_res, p, batches, sub_pairs, storages, threads, reps=5, sample=96):
     """Steady state end to end through the library's stream driver
     (gasal2_stream_align, include/gasal2_stream.h): `batches` full-feature
     batches -- the headline batch, then the same generator with other seeds --

@@ -142,6 +142,27 @@ void pack_words(const uint8_t *b, uint64_t bytes, uint32_t *w) {
     }
 }
 
+// memcpy split over a few threads for large spans (a sub-batch's bases are
+// hundreds of MB; one thread copies them at a fraction of the host's
+// bandwidth)
+void copy_wide(void *dst, const void *src, size_t bytes) {
+    const size_t piece = 32u << 20;
+    const unsigned parts = (unsigned)std::min<size_t>(4, (bytes + piece - 1) / piece);
+    if (parts <= 1) {
+        memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t step = (bytes / parts + 4095) & ~(size_t)4095;
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < parts; k++) {
+        const size_t off = k * step;
+        if (off >= bytes) break;
+        th.emplace_back([=] { memcpy((char *)dst + off, (const char *)src + off, std::min(step, bytes - off)); });
+    }
+    memcpy(dst, src, std::min(step, bytes));
+    for (auto &t : th) t.join();
+}
+
 template <typename T>
 T *copy_out(const std::vector<T> &v) {
     T *p = (T *)malloc(std::max<size_t>(v.size(), 1) * sizeof(T));
@@ -345,11 +366,11 @@ extern "C" int32_t gasal2_stream_align(gasal2_stream_t *stp, const gasal2_pairs_
                 if (qcopy > pq->page_size || tcopy > pt->page_size) contiguous = false;
                 else {
                     if (packed) {
-                        memcpy(pq->data, pairs->query_words + q0 / 8, qcopy);
-                        memcpy(pt->data, pairs->target_words + t0w / 8, tcopy);
+                        copy_wide(pq->data, pairs->query_words + q0 / 8, qcopy);
+                        copy_wide(pt->data, pairs->target_words + t0w / 8, tcopy);
                     } else {
-                        memcpy(pq->data, pairs->query + q0, qcopy);
-                        memcpy(pt->data, pairs->target + t0w, tcopy);
+                        copy_wide(pq->data, pairs->query + q0, qcopy);
+                        copy_wide(pt->data, pairs->target + t0w, tcopy);
                     }
                     pq->data_size = qcopy;
                     pt->data_size = tcopy;

@@ -260,3 +260,14 @@ def test_local_tb_fused_walk(fused, second):
         assert_parity(got, ref, g.LOCAL, g.WITH_TB, second, b)
     finally:
         L.gasal2_set_tb_fused(0)
+
+
+def test_short_read_set_whole():
+    """The short-read workload itself, every pair: seed 1, 10,000 pairs of
+    150 bp queries against 50 + q + 50 targets, LOCAL WITHOUT_START (scores,
+    ends) and LOCAL WITH_TB + second best (CIGARs included)."""
+    b = g.synth_batch(1, 1, 10_000)
+    got, ref = run_both(b, g.LOCAL, g.WITHOUT_START, False)
+    assert_parity(got, ref, g.LOCAL, g.WITHOUT_START, False, b)
+    got, ref = run_both(b, g.LOCAL, g.WITH_TB, True)
+    assert_parity(got, ref, g.LOCAL, g.WITH_TB, True, b)
