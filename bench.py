@@ -264,7 +264,7 @@ def main():
     ap.add_argument("--packed", action="store_true", help="one configuration with isPacked=true host inputs")
     ap.add_argument("--band-pairs", type=int, default=200_000, help="pairs per set for the learned band")
     ap.add_argument("--band-chunk", type=int, default=0,
-                    help="learned band: pairs per banded launch (0: planned from the bands and a 64 GiB tile budget)")
+                    help="learned band: pairs per banded launch (0: planned from the bands and the free device memory)")
     ap.add_argument("--stream-batches", type=int, default=8,
                     help="steady state: full-feature batches streamed end to end (0: skip)")
     ap.add_argument("--stream-sub", type=int, default=1_000_000, help="steady state: pairs per sub-batch")
@@ -764,14 +764,14 @@ def band_main(args):
         runs += [(f"learned@{cw}" if cw != args.check_widen[0] else "learned", LB.learned_proposer(net, mul, add), cw),
                  (f"peak@{cw}" if cw != args.check_widen[0] else "peak", LB.peak_proposer(half), cw)]
     for name, prop, cw in runs:
-        res, st = LB.band_align(held, prop, check_widen=cw, chunk=args.band_chunk or None)
+        res, st = LB.band_align(held, prop, check_widen=cw, chunk=args.band_chunk or None, log=log)
         agree, bad = LB.agreement(res, ref)
         out[name] = {"check_widen": cw, "gcups_equiv": round(cells / st["total_s"] / 1e9, 2), "seconds": round(st["total_s"], 3),
                      "pairs_per_s": round(n / st["total_s"], 1), "hit_rate": round(st["hit_rate"], 5),
                      "final_agreement": agree, "band_cells_fraction": round(st["band_cells_fraction"], 5),
                      "fallback_pairs": st["fallback_pairs"], "flag_counts": st["flag_counts"],
                      "propose_s": round(st["propose_s"], 3), "banded_s": round(st["banded_s"], 3),
-                     "fallback_s": round(st["fallback_s"], 3)}
+                     "fallback_s": round(st["fallback_s"], 3), "chunk_pairs": st["chunk_pairs"]}
         log(f"{name}: {out[name]}")
     line = {
         "metric": "learned-band vs full-DP pairs/s at 100% agreement",

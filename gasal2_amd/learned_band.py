@@ -313,11 +313,12 @@ def _plan_chunks(order, est, budget, max_chunk):
 
 
 def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WITH_TB, check_widen=CHECK_WIDEN,
-               chunk=None, tb_budget=64 << 30, sample=4096, log=None):
+               chunk=None, tb_budget=None, sample=4096, log=None):
     """Banded LOCAL with full-DP fallback over the batch, in length-sorted
     chunks (one storage each).  Large chunks keep enough one-lane-per-pair
     waves in flight; the band-compressed traceback tiles of a chunk must fit
-    `tb_budget`.  With chunk=None the chunks are planned from the bands
+    `tb_budget` (default: 60% of the device memory free at the call).  With
+    chunk=None the chunks are planned from the bands
     proposed for a sample of `sample` pairs (their mean share of the full
     tile grid, with a margin); a chunk whose proposed bands still exceed the
     budget is split.  An int `chunk` fixes the chunk size instead.
@@ -333,6 +334,8 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
     band_cells = 0.0
     tp = tb_ = tf = 0.0
     order = length_order(batch)
+    if tb_budget is None:
+        tb_budget = int(0.6 * torch.cuda.mem_get_info()[0])
     QW = (batch.qlen.astype(np.int64) + 7) // 8
     TW = (batch.tlen.astype(np.int64) + 7) // 8
     if chunk is None:
@@ -359,6 +362,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
         pieces = _plan_chunks(order, est, tb_budget, 1 << 20)
     else:
         pieces = [order[s0:s0 + chunk] for s0 in range(0, n, chunk)]
+    sizes = []
     while pieces:
         ch = pieces.pop(0)
         sub = batch.subset(ch)
@@ -378,6 +382,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
                 pieces[:0] = [ch[:h], ch[h:]]
                 continue
             flags = a.run_banded(dlo, dhi, check_widen=check_widen) | pre
+            sizes.append(len(ch))
             t2 = time.perf_counter()
             a.run_subset(np.nonzero(flags)[0])
             a.sync()
@@ -407,7 +412,8 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
              "fallback_pairs": fb, "hit_rate": 1.0 - fb / max(n, 1),
              "flag_counts": {str(b): int(np.sum((flags_all >> b) & 1)) for b in range(5)},
              "band_cells_fraction": band_cells / max(cells, 1.0),
-             "fallback_cells_fraction": float(pc[flags_all != 0].sum()) / max(cells, 1.0)}
+             "fallback_cells_fraction": float(pc[flags_all != 0].sum()) / max(cells, 1.0),
+             "chunk_pairs": sizes}
     return res, stats
 
 
