@@ -359,7 +359,9 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
         ratio = _band_tb_bytes(sb.qlen, sb.tlen, dlo, dhi) / np.maximum(TW[smp] * QW[smp] * 32, 1)
         r = min(1.0, 1.5 * float(ratio.mean()) + 1e-3)
         est = np.maximum(TW * np.ceil(r * QW) * 32, 32)
-        pieces = _plan_chunks(order, est, tb_budget, 1 << 20)
+        # longest pairs first: the budget's last, partly filled chunk then
+        # holds the shortest pairs, which cost least at its lower occupancy
+        pieces = _plan_chunks(order[::-1], est, tb_budget, 1 << 20)
     else:
         pieces = [order[s0:s0 + chunk] for s0 in range(0, n, chunk)]
     sizes = []
