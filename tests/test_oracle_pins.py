@@ -388,9 +388,8 @@ def test_local_cigar_replays_to_score(seed):
         ops += [b & 3] * (b >> 2)
     # the walk replays from the reported start to the reported end, to the
     # score, on all 60 pairs (19 of them with gaps); the reference's
-    # accounting could move the stop only where it labels a mismatch a match
-    # on leaving a gap (src/kernels/get_tb.h:74-82), which none of these walks
-    # meets
+    # accounting can move the stop only where it labels a mismatch a match on
+    # leaving a gap (src/kernels/get_tb.h:74-82)
     replay, ie, je = apply_cigar_score(q + b"N" * 8, t + b"N" * 8, qs, ts, cg)
     assert replay == score
     assert (ie, je) == (int(r["tend"][0]), int(r["qend"][0]))
