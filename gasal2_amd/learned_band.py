@@ -317,7 +317,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
     """Banded LOCAL with full-DP fallback over the batch, in length-sorted
     chunks (one storage each).  Large chunks keep enough one-lane-per-pair
     waves in flight; the band-compressed traceback tiles of a chunk must fit
-    `tb_budget` (default: 60% of the device memory free at the call).  With
+    `tb_budget` (default: 70% of the device memory free at the call).  With
     chunk=None the chunks are planned from the bands
     proposed for a sample of `sample` pairs (their mean share of the full
     tile grid, with a margin); a chunk whose proposed bands still exceed the
@@ -335,7 +335,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
     tp = tb_ = tf = 0.0
     order = length_order(batch)
     if tb_budget is None:
-        tb_budget = int(0.6 * torch.cuda.mem_get_info()[0])
+        tb_budget = int(0.7 * torch.cuda.mem_get_info()[0])
     QW = (batch.qlen.astype(np.int64) + 7) // 8
     TW = (batch.tlen.astype(np.int64) + 7) // 8
     if chunk is None:
@@ -357,7 +357,7 @@ def band_align(batch, propose, scores=(1, 4, 6, 1), n_penalty=None, start_pos=WI
         # the mean share of the full tile grid, with a margin: chunks are sums
         # over many pairs, and one that still comes out over budget is split
         ratio = _band_tb_bytes(sb.qlen, sb.tlen, dlo, dhi) / np.maximum(TW[smp] * QW[smp] * 32, 1)
-        r = min(1.0, 1.5 * float(ratio.mean()) + 1e-3)
+        r = min(1.0, 1.1 * float(ratio.mean()) + 1e-3)
         est = np.maximum(TW * np.ceil(r * QW) * 32, 32)
         # longest pairs first: the budget's last, partly filled chunk then
         # holds the shortest pairs, which cost least at its lower occupancy
