@@ -2190,12 +2190,20 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // EH16: H and E of the eh row packed as two uint16 (every KSW value is >= 0 and
 // the host checked h0 + match * min(Lq, Lt) <= 65535), halving the row's traffic.
 // LUT: scores from the row's table (batch census allows it), else per cell.
+// qlds: query words per lane held in LDS ([word][lane], the launch gives
+// qlds * 256 bytes; 0: read from global memory).  Every lane re-reads its
+// query on each target row; from global memory those scattered 4-byte reads
+// miss in an L2 that the rows stream through.
+extern __shared__ uint32_t ksw_q[];
 template <bool EH16, bool LUT>
-__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop, int32_t end_bonus) {
+__global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop, int32_t end_bonus,
+                                            uint32_t qlds) {
     const uint32_t census = d.aux[0];
     if (LUT != (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
+    if (qlds)
+        for (int32_t q = 0; q < x.QW; q++) ksw_q[q * WAVE + x.lane] = x.Q[q];   // QW <= qlds: qlen <= the batch's longest
     const int32_t nval = c.has_np ? -c.np : 0;
     const uint32_t qlen = x.qlen, tlen = x.tlen;
     const uint32_t query_batch_regs = (qlen >> 3) + 1, target_batch_regs = (tlen >> 3) + 1;
@@ -2270,7 +2278,7 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // step); a lane loads only its own columns [beg, end)
             int2 nx = jlo < end && jlo >= beg ? ld(jlo) : make_int2(0, 0);
             for (int jj = jlo; jj < end; jj++) {
-                if ((jj & 7) == 0) rpac = x.Q[jj >> 3];
+                if ((jj & 7) == 0) rpac = qlds ? ksw_q[(jj >> 3) * WAVE + x.lane] : x.Q[jj >> 3];
                 int2 pv = nx;
                 if (jj + 1 < end && jj + 1 >= beg) nx = ld(jj + 1);
                 if (jj < beg) continue;
@@ -2831,13 +2839,20 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
             break;
         case A_KSW:
+        {
+            // query words in LDS when the batch's longest query is at most
+            // 1024 bases (32 KiB per one-wave block)
+            const uint32_t qw = (d.eh_cols - 2 + 7) / 8;
+            const uint32_t qlds = qw <= 128 ? qw : 0;
+            const size_t lds = (size_t)qlds * WAVE * 4;
             if (d.ksw16) {
-                k_ksw<true, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
-                k_ksw<true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
+                k_ksw<true, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                k_ksw<true, false><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
             } else {
-                k_ksw<false, true><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
-                k_ksw<false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb);
+                k_ksw<false, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                k_ksw<false, false><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
             }
+        }
             break;
         default:
             break;  // UNKNOWN / MICROLOCAL: nothing launched (src/gasal_align.cu:20-21)
