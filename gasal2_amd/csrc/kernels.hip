@@ -2277,6 +2277,9 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // written by the previous row; this row writes it only at its
             // step); a lane loads only its own columns [beg, end)
             int2 nx = jlo < end && jlo >= beg ? ld(jlo) : make_int2(0, 0);
+            // first and last column of [beg, end) whose new entry is nonzero
+            // (what :178-183 rescan in memory after the row)
+            int fnz = -1, lnz = -1;
             for (int jj = jlo; jj < end; jj++) {
                 if ((jj & 7) == 0) rpac = qlds ? ksw_q[(jj >> 3) * WAVE + x.lane] : x.Q[jj >> 3];
                 int2 pv = nx;
@@ -2296,6 +2299,9 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
                 e -= e_del; e = e > t ? e : t;
                 pv.y = e;
                 st(jj, pv);
+                const bool nz = (pv.x | pv.y) != 0;
+                fnz = (nz && fnz < 0) ? jj : fnz;
+                lnz = nz ? jj : lnz;
                 t = M - oe_ins; t = t > 0 ? t : 0;
                 f -= e_ins; f = f > t ? f : t;
             }
@@ -2320,9 +2326,13 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
                 }
                 if (zstop) break;
             }
-            for (j = beg; (j < end) && ld(j).x == 0 && ld(j).y == 0; ++j) ;
-            beg = j;
-            for (j = end; (j >= beg) && ld(j).x == 0 && ld(j).y == 0; --j) ;
+            // :178-183 from the row's own values: the new beg is the first
+            // nonzero entry of [beg, end) (else end, or beg when beg > end);
+            // the backward scan stops at once when end < beg, else at end
+            // itself when h1 != 0 (stored there as (h1, 0)), else at the last
+            // nonzero entry of the row (else new beg - 1)
+            beg = fnz >= 0 ? fnz : (beg < end ? end : beg);
+            j = (end < beg || h1 != 0) ? end : (lnz >= beg ? lnz : beg - 1);
             end = j + 2 < (int)qlen ? j + 2 : (int)qlen;
         }
     }
