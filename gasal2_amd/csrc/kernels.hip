@@ -2210,23 +2210,44 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
     const uint32_t h0 = d.seeds[x.pair];
     const int o_del = c.gapo, o_ins = c.gapo, e_del = c.gape, e_ins = c.gape;
     const int oe_del = o_del + e_del, oe_ins = o_ins + e_ins;
-    int2 *__restrict__ eh2 = reinterpret_cast<int2 *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
-    uint32_t *__restrict__ eh1 = reinterpret_cast<uint32_t *>(d.eh) + (size_t)x.grp * d.eh_cols * WAVE + x.lane;
-    // The row streams through L2 with non-temporal hints: the rows of the
-    // resident waves exceed it, and cached they would evict the query words
-    // every lane re-reads on each row.
-    uint64_t *__restrict__ eh8 = reinterpret_cast<uint64_t *>(eh2);
+    // The row: [block of 8 columns][lane][8] records, 32 bytes per lane and
+    // block with EH16 (64 without), so a wave reads and writes a block of
+    // columns in one or two 16-byte accesses per lane (2 or 4 KiB contiguous
+    // per wave) instead of one access per column; eh_cols is a multiple of
+    // 8.  The row streams through L2 with non-temporal hints: the rows of the
+    // resident waves exceed it.
+    constexpr int RW = EH16 ? 8 : 16;   // 32-bit words per lane record
+    uint32_t *__restrict__ rec = reinterpret_cast<uint32_t *>(d.eh) + ((size_t)x.grp * d.eh_cols * WAVE) * (EH16 ? 1 : 2) +
+                                 (size_t)x.lane * RW;
+    auto at = [&](int k) -> uint32_t * { return rec + (size_t)(k >> 3) * WAVE * RW + (size_t)(k & 7) * (EH16 ? 1 : 2); };
     auto ld = [&](int k) -> int2 {
         if (EH16) {
-            const uint32_t u = __builtin_nontemporal_load(eh1 + (size_t)k * WAVE);
+            const uint32_t u = __builtin_nontemporal_load(at(k));
             return make_int2((int)(u & 0xFFFFu), (int)(u >> 16));
         }
-        const uint64_t u = __builtin_nontemporal_load(eh8 + (size_t)k * WAVE);
+        const uint64_t u = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(at(k)));
         return make_int2((int)(uint32_t)u, (int)(uint32_t)(u >> 32));
     };
     auto st = [&](int k, int2 v) {
-        if (EH16) __builtin_nontemporal_store((uint32_t)v.x | ((uint32_t)v.y << 16), eh1 + (size_t)k * WAVE);
-        else __builtin_nontemporal_store((uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32), eh8 + (size_t)k * WAVE);
+        if (EH16) __builtin_nontemporal_store((uint32_t)v.x | ((uint32_t)v.y << 16), at(k));
+        else __builtin_nontemporal_store((uint64_t)(uint32_t)v.x | ((uint64_t)(uint32_t)v.y << 32), reinterpret_cast<uint64_t *>(at(k)));
+    };
+    auto load_rec = [&](int b, uint32_t *r) {
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(rec + (size_t)b * WAVE * RW);
+#pragma unroll
+        for (int q = 0; q < RW / 4; q++) {
+            const u32x4 v = __builtin_nontemporal_load(p + q);
+            r[4 * q] = v.x; r[4 * q + 1] = v.y; r[4 * q + 2] = v.z; r[4 * q + 3] = v.w;
+        }
+    };
+    auto store_rec = [&](int b, const uint32_t *r) {
+        u32x4 *p = reinterpret_cast<u32x4 *>(rec + (size_t)b * WAVE * RW);
+#pragma unroll
+        for (int q = 0; q < RW / 4; q++) {
+            u32x4 v;
+            v.x = r[4 * q]; v.y = r[4 * q + 1]; v.z = r[4 * q + 2]; v.w = r[4 * q + 3];
+            __builtin_nontemporal_store(v, p + q);
+        }
     };
     for (uint32_t jj = 0; jj <= qlen + 1; jj++) st((int)jj, make_int2(0, 0));
     int j;
@@ -2276,34 +2297,53 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // the row entry of the next column is loaded one step ahead (it was
             // written by the previous row; this row writes it only at its
             // step); a lane loads only its own columns [beg, end)
-            int2 nx = jlo < end && jlo >= beg ? ld(jlo) : make_int2(0, 0);
             // first and last column of [beg, end) whose new entry is nonzero
             // (what :178-183 rescan in memory after the row)
             int fnz = -1, lnz = -1;
-            for (int jj = jlo; jj < end; jj++) {
-                if ((jj & 7) == 0) rpac = qlds ? ksw_q[(jj >> 3) * WAVE + x.lane] : x.Q[jj >> 3];
-                int2 pv = nx;
-                if (jj + 1 < end && jj + 1 >= beg) nx = ld(jj + 1);
-                if (jj < beg) continue;
-                const uint32_t rbase = (rpac >> (28 - 4 * (jj & 7))) & 0x0Fu;
-                int h, M = pv.x, e = pv.y;
-                pv.x = h1;
-                const int32_t s = LUT ? __builtin_amdgcn_sbfe(lut, (rbase & 7u) << 2, 4) : sub_local(c, nval, rbase, gbase);
-                M = M ? M + s : 0;
-                h = M > e ? M : e;
-                h = h > f ? h : f;
-                h1 = h;
-                mj = m > h ? mj : jj;
-                m = m > h ? m : h;
-                t = M - oe_del; t = t > 0 ? t : 0;
-                e -= e_del; e = e > t ? e : t;
-                pv.y = e;
-                st(jj, pv);
-                const bool nz = (pv.x | pv.y) != 0;
-                fnz = (nz && fnz < 0) ? jj : fnz;
-                lnz = nz ? jj : lnz;
-                t = M - oe_ins; t = t > 0 ? t : 0;
-                f -= e_ins; f = f > t ? f : t;
+            // a block's record is loaded while the previous block computes
+            // (an entry is written only at its own column of this row, so an
+            // early load reads the previous row's value, as the reference
+            // does); columns of a block outside [beg, end) go back unchanged
+            uint32_t rc_[RW], rn_[RW];
+            if (jlo < end) load_rec(jlo >> 3, rc_);
+            for (int jb = jlo; jb < end; jb += 8) {
+                if (jb + 8 < end) load_rec((jb >> 3) + 1, rn_);
+                rpac = qlds ? ksw_q[(jb >> 3) * WAVE + x.lane] : x.Q[jb >> 3];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int jj = jb + k;
+                    if (jj >= beg && jj < end) {
+                        int2 pv = EH16 ? make_int2((int)(rc_[k] & 0xFFFFu), (int)(rc_[k] >> 16))
+                                       : make_int2((int)rc_[2 * k], (int)rc_[2 * k + 1]);
+                        const uint32_t rbase = (rpac >> (28 - 4 * k)) & 0x0Fu;
+                        int h, M = pv.x, e = pv.y;
+                        pv.x = h1;
+                        const int32_t s = LUT ? __builtin_amdgcn_sbfe(lut, (rbase & 7u) << 2, 4) : sub_local(c, nval, rbase, gbase);
+                        M = M ? M + s : 0;
+                        h = M > e ? M : e;
+                        h = h > f ? h : f;
+                        h1 = h;
+                        mj = m > h ? mj : jj;
+                        m = m > h ? m : h;
+                        t = M - oe_del; t = t > 0 ? t : 0;
+                        e -= e_del; e = e > t ? e : t;
+                        pv.y = e;
+                        if (EH16) {
+                            rc_[k] = (uint32_t)pv.x | ((uint32_t)pv.y << 16);
+                        } else {
+                            rc_[2 * k] = (uint32_t)pv.x;
+                            rc_[2 * k + 1] = (uint32_t)pv.y;
+                        }
+                        const bool nz = (pv.x | pv.y) != 0;
+                        fnz = (nz && fnz < 0) ? jj : fnz;
+                        lnz = nz ? jj : lnz;
+                        t = M - oe_ins; t = t > 0 ? t : 0;
+                        f -= e_ins; f = f > t ? f : t;
+                    }
+                }
+                store_rec(jb >> 3, rc_);
+#pragma unroll
+                for (int q = 0; q < RW; q++) rc_[q] = rn_[q];
             }
             {
                 int be = beg > end ? beg : end;

@@ -1297,7 +1297,7 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     ensure(m->bnd, m->bnd_bytes, (size_t)groups * bnd_cols * 64 * 4);
     const bool tb_on = op.start == WITH_TB;
     const bool tb_dp = tb_on && (op.algo == LOCAL || op.algo == GLOBAL);
-    const uint32_t eh_cols = mq + 2;
+    const uint32_t eh_cols = (mq + 2 + 7) & ~7u;   // KSW row: whole blocks of 8 columns
     if (op.algo == KSW) ensure(m->eh, m->eh_bytes, (size_t)groups * eh_cols * 64 * 8);
     const uint32_t cap = g2_cigar_capacity(mq, mt);
     if (tb_on && !px.sub) {
