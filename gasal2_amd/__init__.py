@@ -212,6 +212,7 @@ _SIGS = {
     "gasal2_stage_ms": (C.c_float, [C.POINTER(gasal_gpu_storage_t), C.c_int32]),
     "gasal2_get_cigar": (C.c_uint32, [C.POINTER(gasal_gpu_storage_t), C.c_uint32, C.POINTER(C.c_uint8), C.c_uint32]),
     "gasal2_set_dispatch": (None, [C.c_int32]),
+    "gasal2_get_dispatch": (C.c_int32, []),
     "gasal2_set_wavefront": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "gasal2_set_graph_mode": (None, [C.c_int32]),
     "gasal2_set_batch_order": (None, [C.c_int32]),

@@ -44,6 +44,26 @@ bool g_wf_user = false;   // set by gasal2_set_wavefront: then it holds for ever
 // WITH_TB default (measured on the uneven batch: the traceback's per-lane
 // tiles favour sending more pairs to one wave each)
 constexpr uint32_t WF_TB_MIN_TW = 32, WF_TB_MIN_CELLS = 1u << 18;
+// Small batches: the lane path runs 64 or 128 pairs per wave, so a batch of a
+// few pairs per CU leaves most SIMDs idle and takes one pair's serial time;
+// up to this many pairs per CU every pair runs one wave (auto policy only).
+// Measured on 150 x 250-base pairs: the one-wave path wins below about 160
+// pairs per CU for score-only LOCAL and below about 86 for WITH_TB.
+constexpr uint32_t WF_FEW_PER_CU = 128, WF_FEW_PER_CU_TB = 64;
+// compute units of the current device (cached per device)
+uint32_t g2_cu_count() {
+    static std::atomic<uint32_t> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    uint32_t v = cache[dev].load(std::memory_order_relaxed);
+    if (!v) {
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cu <= 0) return 0;
+        v = (uint32_t)cu;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
 int g_graph_mode = 1;   // capture the device program per batch shape and replay it
 uint64_t g_tb_one_size_max = 64ull << 30;   // lane-path tiles: one size for all groups up to this many bytes
 int g_tb_ckpt = 1;   // checkpointed traceback: 1 the fast LOCAL WITH_TB lane path, 2 also one-wave pairs
@@ -209,6 +229,7 @@ extern "C" void gasal2_set_n_penalty(int32_t has_penalty, int32_t penalty) {
     g_scores.np = penalty;
 }
 extern "C" void gasal2_set_dispatch(int32_t policy) { g2_set_dispatch(policy); }
+extern "C" int32_t gasal2_get_dispatch(void) { return g2_get_dispatch(); }
 extern "C" void gasal2_tune_set(int32_t algo, int32_t start, int32_t second, uint32_t lq_lo, uint32_t lq_hi,
                                 uint32_t min_target_words, uint32_t min_cells, uint32_t max_blocks) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1366,8 +1387,8 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
     uint32_t nlong = 0;
     if (d.fast && op.algo == LOCAL && (op.start == WITHOUT_START || (op.start == WITH_TB && !px.banded)) && !op.second &&
         g2_get_dispatch() != 2) {
-        const bool all = g2_get_dispatch() == 3;
         uint32_t min_tw = g_wf_min_tw, min_cells = g_wf_min_cells, max_blocks = g_wf_max_blocks;
+        bool tuned = false;
         if (!g_wf_user && op.start == WITH_TB) { min_tw = WF_TB_MIN_TW; min_cells = WF_TB_MIN_CELLS; }
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1378,9 +1399,14 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
             for (const TuneEntry &e : g_tune)
                 if (e.algo == op.algo && e.start == op.start && e.second == op.second && mean_q >= e.lq_lo && mean_q < e.lq_hi) {
                     min_tw = e.min_tw; min_cells = e.min_cells; max_blocks = e.max_blocks;
+                    tuned = true;
                     break;
                 }
         }
+        // the small-batch rule yields to user and tuned thresholds
+        const bool few = g2_get_dispatch() == 0 && !g_wf_user && !tuned &&
+                         (uint64_t)n <= (uint64_t)g2_cu_count() * (op.start == WITH_TB ? WF_FEW_PER_CU_TB : WF_FEW_PER_CU);
+        const bool all = g2_get_dispatch() == 3 || few;
         d.wf_min_tw = all ? 1u : min_tw;
         d.wf_min_cells = all ? 0u : min_cells;
         uint32_t ltw = 0, lqw = 0, sq = 0, st = 0;

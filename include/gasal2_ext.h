@@ -88,11 +88,16 @@ uint32_t gasal2_get_cigar(gasal_gpu_storage_t *s, uint32_t k, uint8_t *out, uint
 
 /* Kernel dispatch policy (no reference counterpart; results are identical
  * under every policy):
- *   0 auto;
+ *   0 auto: long pairs run one wave each, and so does every pair of a
+ *     batch of at most 128 pairs per compute unit (64 for WITH_TB), which
+ *     the lane path would spread over a fraction of the SIMDs (unless
+ *     gasal2_set_wavefront or a tuned entry gives the thresholds);
  *   1 reference-order kernels only (one lane per pair, no score tables);
  *   2 one lane per pair only (no one-wave-per-pair path);
- *   3 one wave per pair for every LOCAL score-only pair the path accepts. */
+ *   3 one wave per pair for every pair the path accepts (LOCAL without
+ *     second best, score-only or WITH_TB). */
 void gasal2_set_dispatch(int32_t policy);
+int32_t gasal2_get_dispatch(void);
 
 /* Tuned switch thresholds (written by the autotuner, gasal2_amd/autotune.py):
  * for batches of mode (algo, start, second best) whose mean query length lies

@@ -30,15 +30,40 @@ def run_both(batch, algo, start=g.WITHOUT_START, second=False, head=g.TARGET, ta
                      isPacked=packed)
     a = g.Aligner(p, int(batch.qlen.max()), int(batch.tlen.max()), batch.n, scores=scores, n_code=n_code,
                   n_penalty=n_penalty)
+    L = g.lib()
     try:
         got = a.run(batch, per_pair=per_pair)
         if start == g.WITH_TB and cigars:
             got["cigars"] = [a.cigar(k) for k in range(batch.n)]
+        # The auto policy runs every pair of a small LOCAL batch one wave each;
+        # the lane kernels then run too, with the one-wave path switched off,
+        # and must give the same outputs.
+        if algo == g.LOCAL and not second and start != g.WITH_START and L.gasal2_get_dispatch() == 0:
+            L.gasal2_set_dispatch(2)
+            try:
+                lane = a.run(batch, per_pair=per_pair)
+                if start == g.WITH_TB and cigars:
+                    lane["cigars"] = [a.cigar(k) for k in range(batch.n)]
+            finally:
+                L.gasal2_set_dispatch(0)
+            keys = fields(algo, start, second) + (["n_ops"] + (["cigars"] if cigars else []) if start == g.WITH_TB else [])
+            assert_same(got, lane, keys, "lane path")
     finally:
         a.close()
     ref = O.align(batch, algo, start_pos=start, second_best=second, head=head, tail=tail, k_band=k_band,
                   is_rc=is_rc, scores=scores, n_code=n_code, n_penalty=n_penalty)
     return got, ref
+
+
+def assert_same(got, other, keys, what):
+    """The outputs `keys` of two GPU runs identical."""
+    for f in keys:
+        x, y = got[f], other[f]
+        if isinstance(x, np.ndarray):
+            bad = np.nonzero(x != y)[0]
+            assert len(bad) == 0, f"{what}: {f} differs at {len(bad)} pairs, first {bad[0]}"
+        else:
+            assert x == y, f"{what}: {f} differs"
 
 
 def assert_parity(got, ref, algo, start, second, batch=None):

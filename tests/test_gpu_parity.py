@@ -171,8 +171,17 @@ def test_per_group_tile_regions(algo, second):
 
 def test_last_paths_report_the_kernels_taken():
     """gasal2_last_paths: checkpoints for LOCAL WITH_TB, packed 16-bit for the
-    score modes of the full-feature shapes, neither for WITH_TB off."""
+    score modes of the full-feature shapes, neither for WITH_TB off.  Lane
+    kernels only: a batch this small would otherwise run one wave per pair."""
     b = g.synth_batch(2, 990, 300)
+    g.lib().gasal2_set_dispatch(2)
+    try:
+        _last_paths_cases(b)
+    finally:
+        g.lib().gasal2_set_dispatch(0)
+
+
+def _last_paths_cases(b):
     cases = [(g.LOCAL, g.WITH_TB, True, 4), (g.LOCAL, g.WITHOUT_START, False, 1), (g.GLOBAL, g.WITH_START, False, 1),
              (g.SEMI_GLOBAL, g.WITH_START, False, 1)]
     for algo, start, second, bit in cases:

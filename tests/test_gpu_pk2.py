@@ -40,9 +40,11 @@ def paths_of(b, start, second, mode=2, scores=(1, 4, 6, 1)):
     a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n, scores=scores)
     try:
         L.gasal2_set_pk16(mode)
+        L.gasal2_set_dispatch(2)   # lane kernels only: small batches would otherwise run one wave per pair
         a.run(b)
         return a.last_paths()
     finally:
+        L.gasal2_set_dispatch(0)
         L.gasal2_set_pk16(1)
         a.close()
 

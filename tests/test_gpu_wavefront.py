@@ -95,6 +95,8 @@ def test_uneven_batch_mixed_switch():
 
 
 def test_uneven_batch_default_thresholds():
+    """Default policy: a batch of a few pairs per CU runs every pair one wave
+    (run_both also runs the lane kernels alone and compares)."""
     b = g.synth_batch(3, 1301, 300)
     check(b, n_penalty=1)
 
@@ -200,3 +202,24 @@ def test_census_collision_runs_one_wave_without_tables():
         check_tb(b)
         check_tb(b, ckpt=1)
         check_tb(b, ckpt=0)
+
+
+def test_small_batch_runs_one_wave_per_pair():
+    """Auto policy: a batch of at most 128 pairs per CU (64 for WITH_TB) runs
+    every pair one wave (bit 1 of gasal2_last_paths); the lane-only policy and
+    thresholds set by the caller turn the rule off."""
+    b = g.synth_batch(1, 1350, 2000)
+    for start in (g.WITHOUT_START, g.WITH_TB):
+        p = g.Parameters(algo=g.LOCAL, start_pos=start)
+        a = g.Aligner(p, int(b.qlen.max()), int(b.tlen.max()), b.n)
+        try:
+            a.run(b)
+            assert a.last_paths() & 2, start
+            with policy(dispatch=2):
+                a.run(b)
+                assert not a.last_paths() & 2, start
+            with policy(dispatch=0, wf=(48, 1 << 20, 2048)):
+                a.run(b)
+                assert not a.last_paths() & 2, start
+        finally:
+            a.close()
