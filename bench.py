@@ -206,7 +206,7 @@ class Run:
             for _ in range(self.warmup):
                 a.run_resident(timing=True)
                 a.sync()
-            dp, tb, dev = [], [], []
+            dp, tb, dev, rc = [], [], [], []
             a.sync()
             t0 = time.perf_counter()
             for _ in range(self.steps):
@@ -215,6 +215,7 @@ class Run:
                 dp.append(a.stage_ms(3))
                 tb.append(a.stage_ms(4))
                 dev.append(a.stage_ms(0))
+                rc.append(a.stage_ms(2))
             a.sync()
             wall = time.perf_counter() - t0
         finally:
@@ -222,7 +223,19 @@ class Run:
         ms = 1e3 * wall / self.steps
         return {"gcups": round(self.cells / (wall / self.steps) / 1e9, 3), "ms_per_step": round(ms, 3),
                 "device_ms": round(float(np.mean(dev)), 3), "dp_kernel_ms": round(float(np.mean(dp)), 3),
-                "traceback_ms": round(float(np.mean(tb)), 3), "paths": self.a.last_paths()}
+                "traceback_ms": round(float(np.mean(tb)), 3), "rc_ms": round(float(np.mean(rc)), 4),
+                "paths": self.a.last_paths()}
+
+    def rc_bytes(self):
+        """Algorithmic HBM bytes of the reverse/complement stage: every padded
+        4-bit word of a sequence whose op is nonzero read once and written
+        once (1 byte per base)."""
+        b = self.batch
+        if not self.p.isReverseComplement:
+            return 0
+        qw = (b.qlen.astype(np.int64) + 7) // 8
+        tw = (b.tlen.astype(np.int64) + 7) // 8
+        return int(8 * (qw[b.qop != 0].sum() + tw[b.top != 0].sum()))
 
     def results(self):
         return self.a.download()
@@ -401,6 +414,13 @@ def main():
         f"traceback {t['traceback_ms']:.2f} ms")
     pcie, res = run.pcie_inclusive()
     log(f"gasal_aln_async incl. H2D/D2H: {pcie}")
+    rcb = run.rc_bytes()
+    rc = None
+    if rcb and t["rc_ms"] > 0:
+        rc_gbs = rcb / (t["rc_ms"] * 1e-3) / 1e9
+        rc = {"kernel": "k_rc", "ms": t["rc_ms"], "algorithmic_bytes": rcb, "achieved": round(rc_gbs, 1),
+              "peak": 8000.0, "unit": "GB/s", "frac": round(rc_gbs / 8000.0, 4)}
+        log(f"reverse/complement: {rc}")
 
     traffic = None
     tpath = args.traffic
@@ -474,6 +494,7 @@ def main():
         "cpu_baseline": cpu,
         "device_ms": t["device_ms"],
         "traceback_ms": t["traceback_ms"],
+        "reverse_complement": rc,
         "pcie_inclusive": pcie,
     }
     if lane_only:

@@ -195,15 +195,16 @@ __device__ __forceinline__ uint32_t nib_reverse(uint32_t x) {
     x = ((x & 0x0F0F0F0Fu) << 4) | ((x >> 4) & 0x0F0F0F0Fu);
     return __builtin_bswap32(x);
 }
-__device__ __forceinline__ uint32_t nib_complement(uint32_t w) {
-    uint32_t r = 0;
-#pragma unroll
-    for (int k = 0; k < 32; k += 4) {
-        uint32_t nt = (w >> k) & 15u;
-        nt = nt == 1u ? 4u : nt == 4u ? 1u : nt == 3u ? 7u : nt == 7u ? 3u : nt;
-        r |= nt << k;
-    }
-    return r;
+// Complement of the packed bases, all eight nibbles at once: A (1) <-> T (4)
+// toggles bits 0 and 2, C (3) <-> G (7) bit 2; every other code is kept
+// (src/kernels/pack_rc_seqs.h:56-212 maps nibble by nibble).  Bit 4k of b0..b3
+// is bit 0..3 of nibble k.
+__device__ __forceinline__ uint32_t nib_complement(uint32_t x) {
+    const uint32_t M = 0x11111111u;
+    const uint32_t b0 = x & M, b1 = (x >> 1) & M, b2 = (x >> 2) & M, b3 = (x >> 3) & M;
+    const uint32_t at = (b0 ^ b2) & ~(b1 | b3) & M;   // nibble 1 or 4
+    const uint32_t cg = b0 & b1 & ~b3;                // nibble 3 or 7
+    return x ^ at ^ ((at | cg) << 2);
 }
 
 __device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
@@ -233,58 +234,85 @@ __device__ void rc_one(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code) {
         for (uint32_t i = 0; i < regs; i++) w[i] = nib_complement(w[i]);
 }
 
-// RC_T threads per pair.  When the last word holds no nibble equal to N_CODE
-// (always so for a byte-sized N_CODE such as 'N'), the reference's reverse is a
-// plain reversal of the padded word sequence: thread t swaps words (i,
-// regs-1-i) for i = t, t+RC_T, ...; otherwise the first thread runs the
-// reference's sequential loop.  The last word and the first RC_U word pairs
-// of each thread are loaded together (every index stays inside the pair's
-// words), so a pair costs one memory round trip after its metadata.
-constexpr uint32_t RC_T = 4, RC_U = 4;
+// One sequence (a query or target whose op is nonzero) per RC_T lanes.  When
+// the last word holds no nibble equal to N_CODE (always so for a byte-sized
+// N_CODE such as 'N'), the reference's reverse is a plain reversal of the
+// padded word sequence: lane t swaps words (i, regs-1-i) for i = t, t+RC_T,
+// ..., so each round's loads and stores cover RC_T consecutive words at both
+// ends of the sequence (two 64-byte runs); otherwise the first lane runs the
+// reference's sequential loop.  The last word and the first round's words are
+// loaded together, so a sequence of up to 2 * RC_T words costs one memory round
+// trip.  Every round's loads precede its stores, and rounds touch disjoint words.
+constexpr uint32_t RC_T = 16;
 __device__ __forceinline__ void rc_side(uint32_t *w, uint32_t len, uint32_t op, int32_t n_code, uint32_t t) {
     const uint32_t regs = (len >> 3) + ((len & 7) ? 1 : 0);
     if (regs == 0 || op == 0) return;
     const uint32_t to_swap = (regs >> 1) + (regs & 1);
-    for (uint32_t base = t; base < to_swap; base += RC_T * RC_U) {
-        uint32_t a[RC_U], b[RC_U];
-#pragma unroll
-        for (uint32_t u = 0; u < RC_U; u++) {
-            const uint32_t i = base + u * RC_T;
-            a[u] = i < to_swap ? w[i] : 0u;
-            b[u] = i < to_swap ? w[regs - 1 - i] : 0u;
+    uint32_t i = t;
+    uint32_t a = 0, b = 0;
+    if (i < to_swap) {
+        a = w[i];
+        b = w[regs - 1 - i];
+    }
+    if (op & 1u) {   // decide between the plain reversal and the reference's loop
+        const uint32_t last = w[regs - 1];
+        bool plain = true;
+        for (int jj = 0; jj < 32; jj += 4) plain &= (((last >> jj) & 15u) != (uint32_t)n_code);
+        if (!plain) {
+            if (t == 0) rc_one(w, len, op, n_code);
+            return;
         }
-        if (base == t) {   // first round: decide between the plain reversal and the reference's loop
-            bool plain = true;
-            if (op & 1u) {
-                const uint32_t last = w[regs - 1];
-                for (int jj = 0; jj < 32; jj += 4) plain &= (((last >> jj) & 15u) != (uint32_t)n_code);
-            }
-            if (!plain) {
-                if (t == 0) rc_one(w, len, op, n_code);
-                return;
-            }
+    }
+    while (i < to_swap) {
+        const uint32_t j = regs - 1 - i;
+        uint32_t na = (op & 1u) ? nib_reverse(b) : a;
+        uint32_t nb = (op & 1u) ? nib_reverse(a) : b;
+        if (op & 2u) {
+            na = nib_complement(na);
+            nb = nib_complement(nb);
         }
-#pragma unroll
-        for (uint32_t u = 0; u < RC_U; u++) {
-            const uint32_t i = base + u * RC_T;
-            if (i >= to_swap) break;
-            uint32_t na = (op & 1u) ? nib_reverse(b[u]) : a[u];
-            uint32_t nb = (op & 1u) ? nib_reverse(a[u]) : b[u];
-            if (op & 2u) { na = nib_complement(na); nb = nib_complement(nb); }
-            w[i] = na;
-            if (regs - 1 - i != i) w[regs - 1 - i] = nb;
+        w[i] = na;
+        if (j != i) w[j] = nb;
+        i += RC_T;
+        if (i < to_swap) {
+            a = w[i];
+            b = w[regs - 1 - i];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_rc(G2Dev d, int32_t n_code) {
-    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / RC_T, t = threadIdx.x % RC_T;
-    if (k >= d.n) return;
-    // all metadata in one round trip, before the operation bytes are tested
-    const uint32_t qo = d.qop[k], to = d.top[k];
-    const uint32_t qf = d.qoff[k], ql = d.qlen[k], tf = d.toff[k], tl = d.tlen[k];
-    if (qo) rc_side(d.pq + (qf >> 3), ql, qo, n_code, t);
-    if (to) rc_side(d.pt + (tf >> 3), tl, to, n_code, t);
+// One pair per lane for the metadata (coalesced loads of 64 consecutive
+// pairs), then the wave's sequences with a nonzero op -- queries, then targets
+// -- compacted into LDS and taken four at a time, RC_T lanes each.  Pairs with
+// both ops zero cost no memory beyond their metadata.
+constexpr uint32_t RC_WAVES = 4;
+struct RcItem {
+    uint32_t word, len, op;
+};
+__global__ __launch_bounds__(RC_WAVES * WAVE) void k_rc(G2Dev d, int32_t n_code) {
+    __shared__ RcItem items[RC_WAVES][2 * WAVE];
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const uint32_t k = (blockIdx.x * RC_WAVES + wv) * WAVE + lane;
+    uint32_t qo = 0, to = 0, qf = 0, ql = 0, tf = 0, tl = 0;
+    if (k < d.n) {
+        qo = d.qop[k];
+        to = d.top[k];
+        qf = d.qoff[k];
+        ql = d.qlen[k];
+        tf = d.toff[k];
+        tl = d.tlen[k];
+    }
+    const uint64_t mq = __ballot(qo != 0), mt = __ballot(to != 0);
+    const uint32_t nq = (uint32_t)__popcll(mq), nt = (uint32_t)__popcll(mt);
+    const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;   // lanes under this one
+    if (qo) items[wv][__popcll(mq & below)] = RcItem{qf >> 3, ql, qo};
+    if (to) items[wv][nq + __popcll(mt & below)] = RcItem{tf >> 3, tl, to};
+    __syncthreads();
+    const uint32_t t = lane % RC_T;
+    for (uint32_t g = lane / RC_T; g < nq + nt; g += WAVE / RC_T) {
+        const RcItem it = items[wv][g];
+        rc_side((g < nq ? d.pq : d.pt) + it.word, it.len, it.op, n_code, t);
+    }
 }
 
 // ------------------------------------------------------------------ sort ----
@@ -934,14 +962,20 @@ __device__ void local_reverse_start(const G2Dev &d, const G2Scores &c, const Lan
 // row is the low half's bottom row of the previous step, kept in a register.
 // The high strip's bottom row goes to the boundary row for the next pair of
 // strips.  Used when every value fits int16 (H <= match * min(8QW, 8TW) <=
-// 32767; E and F never fall below -(mismatch + open + extend * 8)), so the
-// values are those of the int32 kernels.  Ends: per strip the first column
+// 32767; E and F are held as max(E, 0) and max(F, 0)), so the H values are
+// those of the int32 kernels.  Ends: per strip the first column
 // whose maximum beats the strip's running best, its first row; strips merged
 // in order (the reference's first strict maximum in (strip, column, row)).
 typedef short pk16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ pk16x2 as_pk(uint32_t u) { return __builtin_bit_cast(pk16x2, u); }
 __device__ __forceinline__ uint32_t as_u32(pk16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ pk16x2 pmax(pk16x2 a, pk16x2 b) { return __builtin_elementwise_max(a, b); }
+// a - b per half, clamped at 0 (unsigned saturation; a, b >= 0)
+__device__ __forceinline__ pk16x2 psubsat(pk16x2 a, pk16x2 b) {
+    typedef unsigned short pu16x2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(pk16x2, __builtin_elementwise_sub_sat(__builtin_bit_cast(pu16x2, a),
+                                                                    __builtin_bit_cast(pu16x2, b)));
+}
 // {lo16(lo), lo16(hi)} and {hi16(lo), hi16(hi)}
 __device__ __forceinline__ uint32_t pack_lo(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x05040100u); }
 __device__ __forceinline__ uint32_t pack_hi(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x07060302u); }
@@ -988,10 +1022,13 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
                 const uint32_t sl = (uint32_t)__builtin_amdgcn_sbfe(lutL[m], qshL, 4);
                 const uint32_t sh = (uint32_t)__builtin_amdgcn_sbfe(lutH[m], qshH, 4);
                 const pk16x2 tmp = p + as_pk(pack_lo(sl, sh));
-                const pk16x2 hm = pmax(pmax(pmax(tmp, f[m]), e), zero2);
+                // E and F kept as max(E, 0) and max(F, 0) (an extension that
+                // saturates at 0, then a max with tt): H = max(tmp, F, E, 0)
+                // then needs no separate max with 0
+                const pk16x2 hm = pmax(pmax(tmp, f[m]), e);
                 const pk16x2 tt = tmp - gapoe2;
-                f[m] = pmax(tt, f[m] - gape2);
-                e = pmax(tt, e - gape2);
+                f[m] = pmax(tt, psubsat(f[m], gape2));
+                e = pmax(tt, psubsat(e, gape2));
                 p = h[m];
                 h[m] = hm;
                 colmax = pmax(colmax, hm);
@@ -1241,6 +1278,13 @@ __global__ __launch_bounds__(64) void k_local_fast(G2Dev d, G2Scores c) {
 #endif
 // 0xFFFF in each 16-bit half whose sign bit is set (v_perm sign-byte selects)
 __device__ __forceinline__ uint32_t half_sign(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x09090808u); }
+// h * 8 + r per half in one packed multiply-add (h >= 0, r < 8 in both halves
+// of r2): left to itself the compiler turns the product into a shift and an or
+__device__ __forceinline__ pk16x2 prank(pk16x2 h, uint32_t r2) {
+    uint32_t v;
+    asm("v_pk_mad_u16 %0, %1, 8, %2 op_sel_hi:[1,0,1]" : "=v"(v) : "v"(as_u32(h)), "s"(r2));
+    return as_pk(v);
+}
 
 template <int START, bool SECOND>
 __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t census, uint32_t wave, uint32_t slot0,
@@ -1388,11 +1432,21 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
                     const uint32_t sa = (uint32_t)__builtin_amdgcn_sbfe(lutA[m], qshA, 4);
                     const uint32_t sb = (uint32_t)__builtin_amdgcn_sbfe(lutB[m], qshB, 4);
                     const pk16x2 tmp = p + as_pk(pack_lo(sa, sb));
-                    const pk16x2 hm = pmax(pmax(pmax(tmp, f[m]), e), zero2);
+                    pk16x2 hm;
                     const pk16x2 tt = tmp - gapoe2;
-                    f[m] = pmax(tt, f[m] - gape2);
-                    e = pmax(tt, e - gape2);
-                    const pk16x2 rk = (hm << (pk16x2){3, 3}) | (pk16x2){(short)(8 - m), (short)(8 - m)};
+                    if (CK) {   // checkpoints hold the reference's E and F: a
+                                // traceback can walk through cells where they are negative
+                        hm = pmax(pmax(pmax(tmp, f[m]), e), zero2);
+                        f[m] = pmax(tt, f[m] - gape2);
+                        e = pmax(tt, e - gape2);
+                    } else {    // E and F kept as max(E, 0) and max(F, 0) (an
+                                // extension that saturates at 0, then a max with
+                                // tt): H = max(tmp, F, E, 0) needs no max with 0
+                        hm = pmax(pmax(tmp, f[m]), e);
+                        f[m] = pmax(tt, psubsat(f[m], gape2));
+                        e = pmax(tt, psubsat(e, gape2));
+                    }
+                    const pk16x2 rk = prank(hm, (uint32_t)(8 - m) * 0x00010001u);
                     colmax = pmax(colmax, rk);
                     if (SECOND) {   // ranked h where h < running maximum, else 0
                         colsec = pmax(colsec, as_pk(as_u32(rk) & half_sign(as_u32(hm - pm2))));
@@ -2780,7 +2834,7 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
     const uint32_t nb256 = (d.n + 255) / 256;
     // 2. reverse / complement (src/gasal_align.cu:232-246)
     if (op.is_rc && !op.skip_prep) {
-        k_rc<<<(d.n + 256 / RC_T - 1) / (256 / RC_T), 256, 0, s>>>(d, sc.n_code);
+        k_rc<<<(d.n + RC_WAVES * WAVE - 1) / (RC_WAVES * WAVE), RC_WAVES * WAVE, 0, s>>>(d, sc.n_code);
         G2_LAUNCH_CHECK();
     }
     if (op.prep_only) return hipSuccess;
