@@ -1446,14 +1446,29 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
                         f[m] = pmax(tt, psubsat(f[m], gape2));
                         e = pmax(tt, psubsat(e, gape2));
                     }
-                    const pk16x2 rk = prank(hm, (uint32_t)(8 - m) * 0x00010001u);
-                    colmax = pmax(colmax, rk);
-                    if (SECOND) {   // ranked h where h < running maximum, else 0
-                        colsec = pmax(colsec, as_pk(as_u32(rk) & half_sign(as_u32(hm - pm2))));
-                        pm2 = pmax(pm2, hm);
-                    }
+                    colmax = pmax(colmax, prank(hm, (uint32_t)(8 - m) * 0x00010001u));
                     p = h[m];
                     h[m] = hm;
+                }
+                if (SECOND) {
+                    // Second best: the ranked h of cells whose h is below the
+                    // running maximum at that cell, in row order.  When every
+                    // valid half's column maximum is below the running maximum
+                    // (most columns of most waves), all the column's cells
+                    // count and the running maximum stays: the column's ranked
+                    // maximum is the answer.  Otherwise the cells are taken in
+                    // order.
+                    const uint32_t rec = ~half_sign(as_u32((colmax >> (pk16x2){3, 3}) - pm2)) & vm2;
+                    if (__any(rec != 0)) {
+#pragma unroll
+                        for (int m = 1; m < 9; m++) {
+                            const pk16x2 rk = prank(h[m], (uint32_t)(8 - m) * 0x00010001u);
+                            colsec = pmax(colsec, as_pk(as_u32(rk) & half_sign(as_u32(h[m] - pm2))));
+                            pm2 = pmax(pm2, h[m]);
+                        }
+                    } else {
+                        colsec = colmax;
+                    }
                 }
                 if (CK) {
                     wA[kk] = pack_lo(as_u32(hb2), as_u32(e));   // H(bottom, col - 1), E(bottom, col)
