@@ -1457,7 +1457,8 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
                     // (most columns of most waves), all the column's cells
                     // count and the running maximum stays: the column's ranked
                     // maximum is the answer.  Otherwise the cells are taken in
-                    // order.
+                    // order.  A valid half's best h is its running maximum, so
+                    // the best can move only in such a column.
                     const uint32_t rec = ~half_sign(as_u32((colmax >> (pk16x2){3, 3}) - pm2)) & vm2;
                     if (__any(rec != 0)) {
 #pragma unroll
@@ -1466,6 +1467,9 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
                             colsec = pmax(colsec, as_pk(as_u32(rk) & half_sign(as_u32(h[m] - pm2))));
                             pm2 = pmax(pm2, h[m]);
                         }
+                        const uint32_t up = half_sign(as_u32(as_pk(best2 | 0x00070007u) - colmax)) & vm2;
+                        best2 = (as_u32(colmax) & up) | (best2 & ~up);
+                        bpos2 = (colid2 & up) | (bpos2 & ~up);
                     } else {
                         colsec = colmax;
                     }
@@ -1483,10 +1487,11 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
                     xb.bd[(j * 8 + kk) * WAVE] = pack_hi(as_u32(h[8]), as_u32(e));
                 }
                 // a half's best moves to this column when its h beats the best's h
-                const uint32_t up = half_sign(as_u32(as_pk(best2 | 0x00070007u) - colmax)) & vm2;
-                best2 = (as_u32(colmax) & up) | (best2 & ~up);
-                bpos2 = (colid2 & up) | (bpos2 & ~up);
-                if (SECOND) {
+                if (!SECOND) {
+                    const uint32_t up = half_sign(as_u32(as_pk(best2 | 0x00070007u) - colmax)) & vm2;
+                    best2 = (as_u32(colmax) & up) | (best2 & ~up);
+                    bpos2 = (colid2 & up) | (bpos2 & ~up);
+                } else {
                     const uint32_t us = half_sign(as_u32(as_pk(sec2 | 0x00070007u) - colsec)) & vm2;
                     sec2 = (as_u32(colsec) & us) | (sec2 & ~us);
                     spos2 = (colid2 & us) | (spos2 & ~us);
