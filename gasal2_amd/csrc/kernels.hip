@@ -2269,14 +2269,17 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
 // query on each target row; from global memory those scattered 4-byte reads
 // miss in an L2 that the rows stream through.
 extern __shared__ uint32_t ksw_q[];
-template <bool EH16, bool LUT>
+// QL: the query words are read from LDS (qlds words per lane), else from
+// global memory; separate code so that an LDS read waits on LDS only, not on
+// the next block's row records in flight.
+template <bool EH16, bool LUT, bool QL>
 __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int32_t zdrop, int32_t end_bonus,
                                             uint32_t qlds) {
     const uint32_t census = d.aux[0];
     if (LUT != (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
-    if (qlds)
+    if (QL)
         for (int32_t q = 0; q < x.QW; q++) ksw_q[q * WAVE + x.lane] = x.Q[q];   // QW <= qlds: qlen <= the batch's longest
     const int32_t nval = c.has_np ? -c.np : 0;
     const uint32_t qlen = x.qlen, tlen = x.tlen;
@@ -2380,39 +2383,48 @@ __global__ __launch_bounds__(64) void k_ksw(G2Dev d, G2Scores c, int32_t w, int3
             // does); columns of a block outside [beg, end) go back unchanged
             uint32_t rc_[RW], rn_[RW];
             if (jlo < end) load_rec(jlo >> 3, rc_);
+            // one cell of this row at column jj, entry k of the block's record
+            auto cell = [&](int k, int jj) {
+                int2 pv = EH16 ? make_int2((int)(rc_[k] & 0xFFFFu), (int)(rc_[k] >> 16))
+                               : make_int2((int)rc_[2 * k], (int)rc_[2 * k + 1]);
+                const uint32_t rbase = (rpac >> (28 - 4 * k)) & 0x0Fu;
+                int h, M = pv.x, e = pv.y;
+                pv.x = h1;
+                const int32_t s = LUT ? __builtin_amdgcn_sbfe(lut, (rbase & 7u) << 2, 4) : sub_local(c, nval, rbase, gbase);
+                M = M ? M + s : 0;
+                h = M > e ? M : e;
+                h = h > f ? h : f;
+                h1 = h;
+                mj = m > h ? mj : jj;
+                m = m > h ? m : h;
+                t = M - oe_del; t = t > 0 ? t : 0;
+                e -= e_del; e = e > t ? e : t;
+                pv.y = e;
+                if (EH16) {
+                    rc_[k] = (uint32_t)pv.x | ((uint32_t)pv.y << 16);
+                } else {
+                    rc_[2 * k] = (uint32_t)pv.x;
+                    rc_[2 * k + 1] = (uint32_t)pv.y;
+                }
+                const bool nz = (pv.x | pv.y) != 0;
+                fnz = (nz && fnz < 0) ? jj : fnz;
+                lnz = nz ? jj : lnz;
+                t = M - oe_ins; t = t > 0 ? t : 0;
+                f -= e_ins; f = f > t ? f : t;
+            };
             for (int jb = jlo; jb < end; jb += 8) {
                 if (jb + 8 < end) load_rec((jb >> 3) + 1, rn_);
-                rpac = qlds ? ksw_q[(jb >> 3) * WAVE + x.lane] : x.Q[jb >> 3];
+                rpac = QL ? ksw_q[(jb >> 3) * WAVE + x.lane] : x.Q[jb >> 3];
+                // a block inside every active lane's [beg, end) runs its
+                // eight cells without per-cell lane masks
+                if (__all(jb >= beg && jb + 8 <= end)) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int jj = jb + k;
-                    if (jj >= beg && jj < end) {
-                        int2 pv = EH16 ? make_int2((int)(rc_[k] & 0xFFFFu), (int)(rc_[k] >> 16))
-                                       : make_int2((int)rc_[2 * k], (int)rc_[2 * k + 1]);
-                        const uint32_t rbase = (rpac >> (28 - 4 * k)) & 0x0Fu;
-                        int h, M = pv.x, e = pv.y;
-                        pv.x = h1;
-                        const int32_t s = LUT ? __builtin_amdgcn_sbfe(lut, (rbase & 7u) << 2, 4) : sub_local(c, nval, rbase, gbase);
-                        M = M ? M + s : 0;
-                        h = M > e ? M : e;
-                        h = h > f ? h : f;
-                        h1 = h;
-                        mj = m > h ? mj : jj;
-                        m = m > h ? m : h;
-                        t = M - oe_del; t = t > 0 ? t : 0;
-                        e -= e_del; e = e > t ? e : t;
-                        pv.y = e;
-                        if (EH16) {
-                            rc_[k] = (uint32_t)pv.x | ((uint32_t)pv.y << 16);
-                        } else {
-                            rc_[2 * k] = (uint32_t)pv.x;
-                            rc_[2 * k + 1] = (uint32_t)pv.y;
-                        }
-                        const bool nz = (pv.x | pv.y) != 0;
-                        fnz = (nz && fnz < 0) ? jj : fnz;
-                        lnz = nz ? jj : lnz;
-                        t = M - oe_ins; t = t > 0 ? t : 0;
-                        f -= e_ins; f = f > t ? f : t;
+                    for (int k = 0; k < 8; k++) cell(k, jb + k);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        const int jj = jb + k;
+                        if (jj >= beg && jj < end) cell(k, jj);
                     }
                 }
                 store_rec(jb >> 3, rc_);
@@ -2970,11 +2982,21 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             const uint32_t qlds = qw <= 128 ? qw : 0;
             const size_t lds = (size_t)qlds * WAVE * 4;
             if (d.ksw16) {
-                k_ksw<true, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
-                k_ksw<true, false><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                if (qlds) {
+                    k_ksw<true, true, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                    k_ksw<true, false, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                } else {
+                    k_ksw<true, true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                    k_ksw<true, false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                }
             } else {
-                k_ksw<false, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
-                k_ksw<false, false><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                if (qlds) {
+                    k_ksw<false, true, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                    k_ksw<false, false, true><<<grid, blk, lds, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                } else {
+                    k_ksw<false, true, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                    k_ksw<false, false, false><<<grid, blk, 0, s>>>(d, sc, op.ksw_w, op.ksw_zdrop, op.ksw_eb, qlds);
+                }
             }
         }
             break;

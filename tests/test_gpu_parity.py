@@ -1,5 +1,6 @@
 """GPU parity: the HIP library (through its C-ABI) against the CPU oracle,
 bit-exact on every output each mode defines, on seeded synthetic batches."""
+import numpy as np
 import pytest
 
 import gasal2_amd as g
@@ -49,6 +50,27 @@ def test_ksw(second):
     b = g.synth_batch(4, 500, 600)
     got, ref = run_both(b, g.KSW, second=second)
     assert_parity(got, ref, g.KSW, g.WITHOUT_START, second, b)
+
+
+@pytest.mark.parametrize("h0", [40, 70000])
+def test_ksw_long_queries(h0):
+    """Queries over 1024 bases: the kernel reads the query words from global
+    memory instead of LDS; h0 = 70000 also takes the int32 row (values past
+    16 bits).  Short pairs in the same batch, and pairs whose targets end
+    early, leave lanes of a block outside their ranges."""
+    rng = np.random.default_rng(4300 + h0)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    qs, ts = [], []
+    for lq, lt in [(1100, 1150), (1500, 700), (300, 320), (1030, 1030), (64, 2000), (1700, 90)] * 3:
+        q = acgt[rng.integers(0, 4, lq)]
+        t = q.copy()[:lt] if lt <= lq else np.concatenate([q, acgt[rng.integers(0, 4, lt - lq)]])
+        flip = rng.random(len(t)) < 0.04
+        t[flip] = acgt[rng.integers(0, 4, int(flip.sum()))]
+        qs.append(q.tobytes())
+        ts.append(t.tobytes())
+    b = g.make_batch(qs, ts, seeds=np.full(len(qs), h0, np.uint32))
+    got, ref = run_both(b, g.KSW)
+    assert_parity(got, ref, g.KSW, g.WITHOUT_START, False, b)
 
 
 @pytest.mark.parametrize("second", [False, True])
