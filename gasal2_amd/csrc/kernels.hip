@@ -16,6 +16,7 @@
 #include "engine.h"
 
 #include <string.h>
+#include <type_traits>
 
 namespace {
 
@@ -1734,18 +1735,11 @@ __device__ void lane_global_pk(const G2Dev &d, const G2Scores &c, const LaneCtx 
             f[m] = neg2;
         }
         uint32_t carry = 0;            // low strip's bottom (H | E << 16) of the previous step
-        uint32_t qshL = 0, qshH = 0;
-        uint32_t rpac = 0;
+        uint32_t qshH = 0;
         const bool last_lo = s0 == TW - 1, last_hi = hi_ok && s0 + 1 == TW - 1;
-        for (int32_t t = 0; t <= C; t++) {
-            qshH = qshL;
-            uint32_t topL = 0;
-            if (t < C) {
-                const int kk = t & 7;
-                if (kk == 0) rpac = x.Qp[(size_t)(t >> 3) * x.ps];
-                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
-                topL = bd[t * WAVE];
-            }
+        // one step: the low strip at column t (query code qshL, row above topL),
+        // the high strip at column t - 1
+        auto step = [&](int32_t t, uint32_t qshL, uint32_t topL, bool ends) {
             pk16x2 e = as_pk(pack_hi(topL, carry));
             pk16x2 p = h[0];
             h[0] = as_pk(pack_lo(topL, carry));
@@ -1761,16 +1755,19 @@ __device__ void lane_global_pk(const G2Dev &d, const G2Scores &c, const LaneCtx 
                 p = h[m];
                 h[m] = hm;
             }
+            qshH = qshL;
             const uint32_t h8 = as_u32(h[8]), e8 = as_u32(e);
             if (t >= 1) bd[(t - 1) * WAVE] = pack_hi(h8, e8);
             carry = pack_lo(h8, e8);
-            if (last_lo && t == lastcol) {                                                 // :98-103
+            if (ends) {
+                if (last_lo && t == lastcol) {                                             // :98-103
 #pragma unroll
-                for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) & 0xFFFFu) : score;
-            }
-            if (last_hi && t - 1 == lastcol) {
+                    for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) & 0xFFFFu) : score;
+                }
+                if (last_hi && t - 1 == lastcol) {
 #pragma unroll
-                for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) >> 16) : score;
+                    for (int m = 1; m < 9; m++) score = (m == rowsel) ? (int32_t)(short)(as_u32(h[m]) >> 16) : score;
+                }
             }
             if (t == 0) {   // the high strip starts at the next step: its column -1 again
 #pragma unroll
@@ -1779,7 +1776,33 @@ __device__ void lane_global_pk(const G2Dev &d, const G2Scores &c, const LaneCtx 
                     f[m] = as_pk((as_u32(f[m]) & 0xFFFFu) | ((uint32_t)NEG16 << 16));
                 }
             }
-        }
+        };
+        // a word of eight columns per iteration, its row-above entries loaded
+        // one word ahead; the score checks run only while some lane of the
+        // wave is in its last pair of strips
+        auto words = [&](auto ENDS) {
+            uint32_t top[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) top[kk] = bd[kk * WAVE];
+            for (int32_t jw = 0; jw < QW; jw++) {
+                const uint32_t rpac = x.Qp[(size_t)jw * x.ps];
+                uint32_t cur[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) cur[kk] = top[kk];
+                if (jw + 1 < QW) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; kk++) top[kk] = bd[(8 * (jw + 1) + kk) * WAVE];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) {
+                    const uint32_t qsh = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
+                    step(8 * jw + kk, qsh, cur[kk], decltype(ENDS)::value);
+                }
+            }
+            step(C, 0, 0u, decltype(ENDS)::value);   // the high strip's last column
+        };
+        if (__any(last_lo || last_hi)) words(std::true_type{});
+        else words(std::false_type{});
     }
     d.score[x.pair] = score;
 }
