@@ -1002,18 +1002,11 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
 #pragma unroll
         for (int m = 0; m < 9; m++) { h[m] = zero2; f[m] = zero2; }
         uint32_t carry = 0;            // low strip's bottom (H | E << 16) of the previous step
-        uint32_t qshL = 0, qshH = 0;
+        uint32_t qshH = 0;
         int32_t lv = 0, lc = 0, lr = 0, hv = 0, hc = 0, hr = 0;
-        uint32_t rpac = 0;
-        for (int32_t t = 0; t <= C; t++) {
-            qshH = qshL;
-            uint32_t topL = 0;
-            if (t < C) {
-                const int kk = t & 7;
-                if (kk == 0) rpac = x.Qp[(size_t)(t >> 3) * x.ps];
-                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
-                topL = bd[t * WAVE];
-            }
+        // one step: the low strip at column t (query code qshL, row above
+        // topL), the high strip at column t - 1
+        auto step = [&](int32_t t, uint32_t qshL, uint32_t topL) {
             pk16x2 e = as_pk(pack_hi(topL, carry));
             pk16x2 p = h[0];
             h[0] = as_pk(pack_lo(topL, carry));
@@ -1034,6 +1027,7 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
                 h[m] = hm;
                 colmax = pmax(colmax, hm);
             }
+            qshH = qshL;
             const uint32_t h8 = as_u32(h[8]), e8 = as_u32(e);
             if (t >= 1) bd[(t - 1) * WAVE] = pack_hi(h8, e8);
             carry = pack_lo(h8, e8);
@@ -1055,7 +1049,26 @@ __device__ void lane_local_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &
 #pragma unroll
                 for (int m = 8; m >= 1; m--) if ((int32_t)(as_u32(h[m]) >> 16) == ch) hr = m - 1;
             }
+        };
+        // a word of eight columns per iteration, its row-above entries loaded
+        // one word ahead
+        uint32_t top[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) top[kk] = bd[kk * WAVE];
+        for (int32_t jw = 0; jw < QW; jw++) {
+            const uint32_t rpac = x.Qp[(size_t)jw * x.ps];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = top[kk];
+            if (jw + 1 < QW) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) top[kk] = bd[(8 * (jw + 1) + kk) * WAVE];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++)
+                step(8 * jw + kk, kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu, cur[kk]);
         }
+        step(C, 0, 0u);   // the high strip's last column
         if (lv > bv) { bv = lv; bs = s0; bc = lc; br = lr; }
         if (hv > bv) { bv = hv; bs = s0 + 1; bc = hc; br = hr; }
     }
@@ -2007,16 +2020,10 @@ __device__ void semi_pass_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x
         }
         const int32_t gL8 = s0 << 3, gH8 = (s0 + 1) << 3;
         bool hi_live = hi_ok;
-        uint32_t carry = 0, qshL = 0, qshH = 0, rpac = 0;
-        for (int32_t t = 0; t <= C; t++) {
-            qshH = qshL;
-            uint32_t topL = 0;
-            if (t < C) {
-                const int kk = t & 7;
-                if (kk == 0) rpac = REV ? rev_word(x.Q, read_len, t >> 3) : x.Q[t >> 3];
-                qshL = kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu;
-                topL = bd[t * WAVE];
-            }
+        uint32_t carry = 0, qshH = 0;
+        // one step: the low strip at column t (query code qshL, row above
+        // topL), the high strip at column t - 1
+        auto step = [&](int32_t t, uint32_t qshL, uint32_t topL) {
             pk16x2 e = as_pk(pack_hi(topL, carry));
             pk16x2 ph = as_pk(pack_lo(topL, carry));   // H of the row above, this column
             pk16x2 prev = ph - gapoe2;
@@ -2075,7 +2082,27 @@ __device__ void semi_pass_pk(const G2Dev &d, const G2Scores &c, const LaneCtx &x
                     f[m] = as_pk((as_u32(f[m]) & 0xFFFFu) | ((uint32_t)NEG16 << 16));
                 }
             }
+            qshH = qshL;
+        };
+        // a word of eight columns per iteration, its row-above entries loaded
+        // one word ahead
+        uint32_t top[8];
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) top[kk] = bd[kk * WAVE];
+        for (int32_t jw = 0; jw < QW; jw++) {
+            const uint32_t rpac = REV ? rev_word(x.Q, read_len, jw) : x.Q[jw];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = top[kk];
+            if (jw + 1 < QW) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) top[kk] = bd[(8 * (jw + 1) + kk) * WAVE];
+            }
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++)
+                step(8 * jw + kk, kk < 7 ? (rpac >> (26 - 4 * kk)) & 0x1Cu : (rpac << 2) & 0x1Cu, cur[kk]);
         }
+        step(C, 0, 0u);   // the high strip's last column
     }
 }
 
