@@ -2253,7 +2253,13 @@ __global__ __launch_bounds__(64) void k_semi(G2Dev d, G2Scores c) {
 
 // --------------------------------------------------------------- BANDED -----
 // gasal_banded_tiled_kernel (src/kernels/banded.h:10-139).
+// LUT: substitution scores from a per-row table of the batch's nibble classes
+// (the census allows it), else per cell.  A word's boundary-row entries are
+// loaded one word ahead.
+template <bool LUT>
 __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_band_width) {
+    const uint32_t census = d.aux[0];
+    if (LUT ? !(d.fast && census_fast(census)) : (d.fast && census_fast(census))) return;
     LaneCtx x;
     if (!lane_setup(d, x)) return;
     const int32_t nval = c.has_np ? -c.np : 0;
@@ -2270,22 +2276,38 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
         const uint32_t gpac = x.T[i];
         uint32_t gb[9];
 #pragma unroll
-        for (int m = 1; m < 9; m++) gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+        for (int m = 1; m < 9; m++) {
+            gb[m] = (gpac >> (32 - 4 * m)) & 15u;
+            if (LUT) gb[m] = score_lut(c, nval, census, gb[m]);
+        }
         const int32_t gidx = i << 3;
         const int32_t j0 = max(0, i - other + 1);                                          // :83-85
         const int32_t last_tile = min(k_band_width + i, QW);
+        uint32_t nxt[8];
+        if (j0 < last_tile) {
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[(j0 * 8 + kk) * WAVE];
+        }
         for (int32_t j = j0; j < last_tile; j++) {
             const uint32_t rpac = x.Q[j];
+            uint32_t cur[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
+            if (j + 1 < last_tile) {
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+            }
 #pragma unroll
             for (int kk = 0; kk < 8; kk++) {
                 const int32_t ridx = j * 8 + kk;
                 const uint32_t rbase = (rpac >> (28 - 4 * kk)) & 15u;
-                uint32_t hd = bd[ridx * WAVE];
+                const uint32_t qsh = (rbase & 7u) << 2;
+                const uint32_t hd = cur[kk];
                 h[0] = sx16(hd);
                 int32_t e = sx16(hd >> 16);
 #pragma unroll
                 for (int m = 1; m < 9; m++) {                                              // :100-115
-                    const int32_t s = sub_local(c, nval, rbase, gb[m]);
+                    const int32_t s = LUT ? __builtin_amdgcn_sbfe(gb[m], qsh, 4) : sub_local(c, nval, rbase, gb[m]);
                     f[m] = max(h[m] - gapoe, f[m] - gape);
                     int32_t hm = p[m] + s;
                     hm = max(hm, f[m]);
@@ -3022,7 +3044,8 @@ hipError_t g2_run(const G2Scores &sc, const G2Opts &op, const G2Dev &d, hipStrea
             }
             break;
         case A_BANDED:
-            k_banded<<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);                            // src/gasal_align.h:80
+            if (d.fast) k_banded<true><<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);          // src/gasal_align.h:80
+            k_banded<false><<<grid, blk, 0, s>>>(d, sc, op.k_band >> 3);
             break;
         case A_KSW:
         {

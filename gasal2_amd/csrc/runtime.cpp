@@ -1043,7 +1043,7 @@ G2Scores g2_scores() { return g_scores; }
 void g2_choose_fast(G2Dev &d, const G2Opts &op, const G2Scores &sc, uint32_t qw, uint32_t tw) {
     d.fast = 0;
     d.enc_ok = 0;
-    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL || op.algo == KSW;
+    const bool lut_algo = op.algo == LOCAL || op.algo == GLOBAL || op.algo == SEMI_GLOBAL || op.algo == KSW || op.algo == BANDED;
     if (lut_algo && g2_get_dispatch() != 1 && qw > 0 && tw > 0) {
         auto fits4 = [](int32_t v) { return v >= -8 && v <= 7; };
         // the N score of the substitution macros (src/gasal_kernels.h:37-56)

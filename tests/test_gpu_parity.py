@@ -45,6 +45,19 @@ def test_banded(k_band):
     assert_parity(got, ref, g.BANDED, g.WITHOUT_START, False, b)
 
 
+@pytest.mark.parametrize("scores", [(1, 4, 6, 1), (2, 9, 5, 2)])
+def test_banded_per_cell_scores(scores):
+    """BANDED without the score tables: a nibble-class collision ('I' with 'A')
+    or a mismatch past the 4-bit range takes the per-cell substitution rule."""
+    b0 = g.synth_batch(4, 470, 300)
+    qs = [b0.query(i) for i in range(b0.n)]
+    ts = [b0.target(i) for i in range(b0.n)]
+    qs[7] = qs[7][:30] + b"IIA" + qs[7][33:]
+    b = g.make_batch(qs, ts)
+    got, ref = run_both(b, g.BANDED, k_band=32, scores=scores, n_penalty=1)
+    assert_parity(got, ref, g.BANDED, g.WITHOUT_START, False, b)
+
+
 @pytest.mark.parametrize("second", [False, True])
 def test_ksw(second):
     b = g.synth_batch(4, 500, 600)
