@@ -924,11 +924,27 @@ __device__ void local_reverse_start(const G2Dev &d, const G2Scores &c, const Lan
             gidx -= 8;
             int32_t ridx = (rend_reg << 3) - 1;
             int32_t gi = 0;
+            // A word at a time, its boundary-row entries loaded one word
+            // ahead; the exit is tested after each word.  The cells a word
+            // runs past the reference's exit change nothing it reports: the
+            // maxima move only on a strictly higher h, and no h here exceeds
+            // the forward score.
+            uint32_t nxt[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[kk * WAVE];
             for (int32_t j = 0; j < rend_reg && go; j++) {
                 const uint32_t rpac = *(Qe - j);
-                for (int kk = 0; kk < 8 && go; kk++) {
+                uint32_t cur[8];
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) cur[kk] = nxt[kk];
+                if (j + 1 < rend_reg) {
+#pragma unroll
+                    for (int kk = 0; kk < 8; kk++) nxt[kk] = bd[((j + 1) * 8 + kk) * WAVE];
+                }
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) {
                     const uint32_t qsh = ((rpac >> (4 * kk)) & 7u) << 2;
-                    uint32_t hd = bd[gi * WAVE];
+                    const uint32_t hd = cur[kk];
                     h[0] = sx16(hd);
                     int32_t e = sx16(hd >> 16);
 #pragma unroll
@@ -947,8 +963,8 @@ __device__ void local_reverse_start(const G2Dev &d, const G2Scores &c, const Lan
                     maxXY_x = (prev_maxHH < maxHH) ? ridx : maxXY_x;
                     prev_maxHH = max(maxHH, prev_maxHH);
                     ridx--; gi++;
-                    go = maxHH < fwd_score;
                 }
+                go = maxHH < fwd_score;
             }
         }
         d.qstart[pr] = maxXY_x;
