@@ -294,6 +294,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=0, metavar="PAIRS",
                     help="tune the lane / one-wave switch on a sample of this many pairs of the batch "
                          "(gasal2_amd.autotune) and time the batch with the winner")
+    ap.add_argument("--tile-budget", type=float, default=None, metavar="GB",
+                    help="traceback tile budget (gasal2_set_tile_budget; library default 64 GB)")
     ap.add_argument("--tb-checkpoint", type=int, default=1, choices=[0, 1, 2],
                     help="LOCAL WITH_TB: checkpointed traceback on the lane path (1, default), also for one-wave "
                          "pairs (2), or direction words stored by the DP (0)")
@@ -380,6 +382,8 @@ def main():
     L = g.lib()
     g.set_ksw_extension(kw, kz)
     L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
+    if args.tile_budget is not None:
+        L.gasal2_set_tile_budget(int(args.tile_budget * (1 << 30)))
     L.gasal2_set_tb_fused(args.tb_fused)
     L.gasal2_set_pk16(args.pk16)
     if args.wf:
@@ -592,11 +596,17 @@ def other_configs(g, args, full, peak_tops, headline_res=None):
             o["local_score_one_lane_per_pair"] = {"gcups": lo["gcups"], "ms_per_step": lo["ms_per_step"]}
         finally:
             r.close()
+        # WITH_TB with checkpoints on the one-wave path too (gasal2_set_tb_checkpoint(2)):
+        # this batch's long pairs store twice the bytes of direction words and
+        # run about 15% faster
+        L.gasal2_set_tb_checkpoint(2)
         r = Run(g, b, params_for(g, "local_tb", False), c, 3, 1)
         try:
             o["local_tb"] = step_line(g, r, "local_tb", peak_tops, thr, 2000, cigar_pairs=500)
+            o["local_tb"]["tb_checkpoint"] = 2
         finally:
             r.close()
+            L.gasal2_set_tb_checkpoint(args.tb_checkpoint)
         o["pairs"] = b.n
         o["cells"] = c
         return o

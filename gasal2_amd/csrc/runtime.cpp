@@ -1440,8 +1440,13 @@ void prepare(gasal_gpu_storage_t *s, uint32_t qb, uint32_t tb, uint32_t n, const
                 d.wf_scratch = nullptr;
             }
             if (wtb) {   // per-pair direction tiles (or checkpoints) of the long pairs and their offsets
-                // checkpoints take twice the bytes of direction words: only within the tile budget
-                const bool wck = g_tb_ckpt == 2 && wtot * 64 <= g_tb_one_size_max &&
+                // checkpoints take twice the bytes of direction words: only while
+                // they fit in 80% of the device memory free now (this storage's
+                // own buffer counted as free: it is reused)
+                size_t free_b = 0, total_b = 0;
+                const bool room = g_tb_ckpt == 2 && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                                  wtot * 64 <= ((uint64_t)free_b + m->wf_tb_bytes) / 10 * 8;
+                const bool wck = g_tb_ckpt == 2 && room &&
                                  (uint64_t)std::max(g_scores.match, 0) * 8u * std::min(lqw, ltw) <= 32767u &&
                                  (uint64_t)g_scores.gapoe +
                                          (uint64_t)std::max(std::max(g_scores.mismatch, g_scores.has_np ? g_scores.np : 0), 0) <=

@@ -187,8 +187,10 @@ void gasal2_storage_set_ksw(gasal_gpu_storage_t *s, int32_t own, int32_t w, int3
  * when H fits 16 bits (match * min(Lq, Lt) <= 32767); results are identical
  * either way.  mode 0: direction words everywhere; 1 (default): checkpoints
  * on the one-lane-per-pair path; 2: also for the pairs that run one wave per
- * pair, when their checkpoints fit the tile budget (their long paths make the
- * recompute cost about what the DP saves). */
+ * pair, when their checkpoints (twice the bytes of direction words) fit in
+ * 80% of the device memory free at the launch.  Mode 2 pays where the long
+ * pairs' matrices dominate (uneven 50-5000 bp batches) and costs where their
+ * traceback walks do (1-10 kbp divergent pairs). */
 void gasal2_set_tb_checkpoint(int32_t mode);
 
 /* Fused traceback for checkpointed lane-path pairs (default off; no

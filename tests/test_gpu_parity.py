@@ -96,7 +96,7 @@ def test_local_tb_checkpoint_switch(ckpt, second):
         got, ref = run_both(b, g.LOCAL, g.WITH_TB, second, is_rc=True, n_penalty=1)
         assert_parity(got, ref, g.LOCAL, g.WITH_TB, second, b)
     finally:
-        g.lib().gasal2_set_tb_checkpoint(1)
+        g.lib().gasal2_set_tb_checkpoint(1)   # the default
 
 
 @pytest.mark.parametrize("start,second", [(g.WITH_TB, True), (g.WITH_TB, False), (g.WITHOUT_START, False),
@@ -238,7 +238,7 @@ def _last_paths_cases(b):
         finally:
             a.close()
     finally:
-        g.lib().gasal2_set_tb_checkpoint(1)
+        g.lib().gasal2_set_tb_checkpoint(1)   # the default
 
 
 def _many_shapes_batch(n, seed, n_code=0x4E):

@@ -161,7 +161,7 @@ def check_tb(b, ckpt=2, **kw):
     try:
         got, ref = run_both(b, g.LOCAL, g.WITH_TB, False, **kw)
     finally:
-        g.lib().gasal2_set_tb_checkpoint(1)
+        g.lib().gasal2_set_tb_checkpoint(1)   # the default
     assert_parity(got, ref, g.LOCAL, g.WITH_TB, False, b)
 
 
