@@ -1390,7 +1390,9 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
         // previous strip's bottom-row checkpoint (H one column behind E; strip
         // 0 reads strip 0's and masks it to zero); else the boundary row (H, E)
         uint32_t nA[8], nB[8];
-        const uint32_t zmask = i > 0 ? 0xFFFFFFFFu : 0u;
+        // pack_hi / pack_lo byte selectors; above strip 0 (CK) every byte 0
+        const uint32_t sel_hi = (!CK || i > 0) ? 0x07060302u : 0x0C0C0C0Cu;
+        const uint32_t sel_lo = (!CK || i > 0) ? 0x05040100u : 0x0C0C0C0Cu;
         auto fetch = [&](int32_t j) {
             if (CK) {
                 hvA.load_nt(max(min(i - 1, ta1), 0), min(j, qa1), nA);
@@ -1415,13 +1417,9 @@ __device__ void lane2_local_pk(const G2Dev &d, const G2Scores &c, uint32_t censu
             // this word's row above, packed per column: E and the diagonal H
             uint32_t pe[8], ph[8];
 #pragma unroll
-            for (int kk = 0; kk < 8; kk++) {
-                pe[kk] = pack_hi(nA[kk], nB[kk]);
-                ph[kk] = pack_lo(nA[kk], nB[kk]);
-                if (CK) {
-                    pe[kk] &= zmask;
-                    ph[kk] &= zmask;
-                }
+            for (int kk = 0; kk < 8; kk++) {   // CK, strip 0: the selectors give zero bytes
+                pe[kk] = __builtin_amdgcn_perm(nB[kk], nA[kk], sel_hi);
+                ph[kk] = __builtin_amdgcn_perm(nB[kk], nA[kk], sel_lo);
             }
             fetch(j + 1);   // past the last word: an index clamped inside the pair (unused)
             // CK: this tile's records, one per pair, stored without branches.
