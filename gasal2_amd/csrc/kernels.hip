@@ -838,17 +838,26 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                         for (int m = 1; m < 9; m++) {
                             const int32_t sc = use_lut ? __builtin_amdgcn_sbfe(lut[m], qcls, 4) : sub_local(c, nval, qcls, lut[m]);
                             const int32_t tmp = p + sc;
-                            const int32_t hm = max(max(max(tmp, f[m]), e), 0);
+                            int32_t hm;
                             const int32_t t = tmp - gapoe;
-                            const int32_t fx = f[m] - gape, ex = e - gape;
-                            if (TB && !CK) {
-                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
-                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
-                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
-                                dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+                            if (TB) {   // a traceback reads the reference's E and F, negative ones too
+                                hm = max(max(max(tmp, f[m]), e), 0);
+                                const int32_t fx = f[m] - gape, ex = e - gape;
+                                if (!CK) {
+                                    dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(tmp - hm), 31);
+                                    dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(f[m] - hm), 31);
+                                    dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(fx - t), 31);
+                                    dir = __builtin_amdgcn_alignbit(dir, (uint32_t)(ex - t), 31);
+                                }
+                                f[m] = max(t, fx);
+                                e = max(t, ex);
+                            } else {   // score and ends: E and F held as max(E, 0) and max(F, 0)
+                                       // (an extension that saturates at 0, then a max with t),
+                                       // so H = max(tmp, F, E, 0) needs no max with 0
+                                hm = max(max(tmp, f[m]), e);
+                                f[m] = max(t, (int32_t)__builtin_elementwise_sub_sat((uint32_t)f[m], (uint32_t)gape));
+                                e = max(t, (int32_t)__builtin_elementwise_sub_sat((uint32_t)e, (uint32_t)gape));
                             }
-                            f[m] = max(t, fx);
-                            e = max(t, ex);
                             const int32_t en = (hm << 3) | (8 - m);
                             colbest = m == 1 ? en : max(colbest, en);
                             p = h[m];
