@@ -960,9 +960,12 @@ __device__ void local_reverse_start(const G2Dev &d, const G2Scores &c, const Lan
                     for (int m = 1; m < 9; m++) {
                         const int32_t s = __builtin_amdgcn_sbfe(gb[m], qsh, 4);
                         const int32_t tmp = p[m] + s;
-                        int32_t hm = max(max(tmp, f[m]), max(e, 0));
-                        f[m] = max(tmp - gapoe, f[m] - gape);
-                        e = max(tmp - gapoe, e - gape);
+                        // E and F held as max(E, 0) and max(F, 0) (saturating
+                        // extensions): H = max(tmp, F, E, 0) with no max with 0;
+                        // only H reaches this pass's outputs
+                        const int32_t hm = max(max(tmp, f[m]), e);
+                        f[m] = max(tmp - gapoe, (int32_t)__builtin_elementwise_sub_sat((uint32_t)f[m], (uint32_t)gape));
+                        e = max(tmp - gapoe, (int32_t)__builtin_elementwise_sub_sat((uint32_t)e, (uint32_t)gape));
                         maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
                         maxHH = max(maxHH, hm);
                         p[m] = h[m - 1];
