@@ -2333,12 +2333,14 @@ __global__ __launch_bounds__(64) void k_banded(G2Dev d, G2Scores c, int32_t k_ba
                 int32_t e = sx16(hd >> 16);
 #pragma unroll
                 for (int m = 1; m < 9; m++) {                                              // :100-115
+                    // E and F held as max(E, 0) and max(F, 0) (saturating
+                    // extensions; the opens come from H >= 0): H = max(diagonal,
+                    // F, E, 0) needs no max with 0, and only H is reported
                     const int32_t s = LUT ? __builtin_amdgcn_sbfe(gb[m], qsh, 4) : sub_local(c, nval, rbase, gb[m]);
-                    f[m] = max(h[m] - gapoe, f[m] - gape);
+                    f[m] = max(h[m] - gapoe, (int32_t)__builtin_elementwise_sub_sat((uint32_t)f[m], (uint32_t)gape));
                     int32_t hm = p[m] + s;
                     hm = max(hm, f[m]);
-                    hm = max(hm, 0);
-                    e = max(h[m - 1] - gapoe, e - gape);
+                    e = max(h[m - 1] - gapoe, (int32_t)__builtin_elementwise_sub_sat((uint32_t)e, (uint32_t)gape));
                     hm = max(hm, e);
                     maxXY_y = (maxHH < hm) ? gidx + (m - 1) : maxXY_y;
                     maxHH = max(maxHH, hm);
