@@ -806,7 +806,10 @@ __device__ void wf_local(const G2Dev &d, const G2Scores &c, uint32_t census) {
                 for (int m = 0; m < 8; m++) win[m] = 0;
                 uint32_t out = 0, qcls = 0;
                 uint32_t qcur = 0, qnext = Q[c0 >> 3];   // query word of lane 0's column, next one prefetched
-                for (int32_t tau = 0; tau < wb + 63; tau++) {
+                // the last active lane (strip) reaches column wb - 1 at step
+                // wb - 1 + (nact - 1); lanes past the pair's strips do nothing
+                const int32_t nact = min(64, TW - 64 * k);
+                for (int32_t tau = 0; tau < wb + nact - 1; tau++) {
                     // inputs of lane 0 (column c0 + tau): top boundary and query class
                     uint32_t top0 = 0, q0 = 0;
                     if (tau < wb) {
